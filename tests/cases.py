@@ -1,0 +1,51 @@
+"""Problem builders shared by the oracle and GPU tests."""
+import json
+import os
+
+import numpy as np
+
+from tdycore_amd import meshgen as mg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDS = json.load(open(os.path.join(HERE, "golden", "reference_golds.json")))
+
+BC = {"dirichlet": mg.BC_DIRICHLET, "neumann": mg.BC_NEUMANN, "seepage": mg.BC_SEEPAGE}
+
+
+def top110k(x):
+    top = (x[:, 0] > 0) & (x[:, 0] < 1) & (x[:, 1] > 0) & (x[:, 1] < 1) & (x[:, 2] > 0.99)
+    return np.where(top, 110000.0, 100000.0)
+
+
+def gold_problem(case):
+    """-> dict(mesh, face_bc, K, porosity, Sr, kwargs, U0) for a reference gold case."""
+    m = mg.box_mesh(*case["mesh"])
+    fbc = m.face_bc(default=BC[case["bc"]])
+    pb = None
+    if case["bc_fn"] == "const1e5":
+        pb = np.full(m.nbnd, 1e5)
+    elif case["bc_fn"] == "top110k":
+        pb = mg.boundary_values(m, top110k)
+    kw = dict(gmatrix=case["gmatrix"], density=case["density"], p_bnd_fn=pb)
+    if case["ic"] == "vec":
+        U0 = np.array(GOLDS["ic_vec"], dtype=np.float64)
+    else:
+        U0 = np.full(m.nc, float(case["ic"]))
+    return dict(mesh=m, face_bc=fbc, K=mg.isotropic_K(m.nc, case["K"]),
+                porosity=np.full(m.nc, case["porosity"]), Sr=np.full(m.nc, case["Sr"]), kw=kw, U0=U0)
+
+
+def cfg0_problem(n=16, seed=20261202, jitter=0.1):
+    """Plain-method configuration: jittered unit cube, rotated full-tensor K,
+    phi ~ U(0.2,0.4), VG n=1.5 (m=1/3), Sr=0.1, hydrostatic IC with the water
+    table at z=0.5, Dirichlet top/bottom (hydrostatic values), no-flow sides."""
+    rng = np.random.default_rng(seed)
+    m = mg.box_mesh(n, n, n, jitter=jitter, seed=seed + 1)
+    K = mg.random_rotated_K(m.nc, rng, -12.0, 0.5)
+    phi = rng.uniform(0.2, 0.4, size=m.nc)
+    Sr = np.full(m.nc, 0.1)
+    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-", "x+", "y-", "y+"])
+    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 0.5))
+    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], 0.5)
+    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
