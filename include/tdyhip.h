@@ -1,0 +1,119 @@
+/*
+ * tdyhip.h -- C ABI of the MI355X MPFA-O Richards path.
+ *
+ * Drop-in boundary: the entry points below replace TDycore's MPFA-O
+ * callbacks and setup for the Richards mode.  All array arguments of the
+ * evaluation calls are DEVICE pointers (HBM); setup takes HOST arrays.
+ * Every function returns 0 on success and a nonzero TDYHIP_ERR_* code on
+ * failure (the PetscErrorCode convention); tdyhip_last_error() describes
+ * the last failure of the calling thread.  No call retains a caller pointer
+ * past its return.  Calls on one context are not thread-safe (the reference
+ * callbacks are called synchronously from SNES/TS, src/tdycore.c:1892-2034).
+ */
+#ifndef TDYHIP_H
+#define TDYHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdyhip_ctx tdyhip_ctx;
+typedef void *tdyhip_stream; /* a hipStream_t; NULL = the context's stream */
+
+enum {
+  TDYHIP_OK = 0,
+  TDYHIP_ERR_ARG = 1,      /* bad argument / unsupported mesh */
+  TDYHIP_ERR_HIP = 2,      /* HIP runtime failure */
+  TDYHIP_ERR_SINGULAR = 3, /* singular interaction-region system (PETSC_ERR_MAT_LU_ZRPVT) */
+  TDYHIP_ERR_NOCONV = 4,   /* Newton / Krylov did not converge */
+  TDYHIP_ERR_MEM = 5       /* device memory exhausted */
+};
+
+/* boundary types per face (face_bc): -1 internal, 0 Dirichlet, 1 Neumann
+ * (no flow), 2 seepage.  One type on every boundary face is the reference's
+ * global -tdy_mpfao_boundary_condition_type (src/fv/mpfao/tdympfao.c:588-596). */
+enum { TDYHIP_BC_INTERNAL = -1, TDYHIP_BC_DIRICHLET = 0, TDYHIP_BC_NEUMANN = 1, TDYHIP_BC_SEEPAGE = 2 };
+
+/* Mesh in the form TDyMeshCreateFromPlex produces (src/fv/share/tdymeshplex.c:3685):
+ * hexahedra, 8 vertices per cell, 6 faces per cell, 4 vertices per face
+ * (cyclic), face_cells = (up, dn) with the unit normal from up to dn and a
+ * boundary face stored as (-(b+1), cell).  Host pointers. */
+typedef struct {
+  int32_t nc, nf, nv, nbnd;
+  const int32_t *cell_verts;  /* nc*8 */
+  const int32_t *cell_faces;  /* nc*6 */
+  const int32_t *face_verts;  /* nf*4 */
+  const int32_t *face_cells;  /* nf*2 */
+  const int32_t *face_bc;     /* nf */
+  const double *vert_xyz;     /* nv*3 */
+  const double *cell_cen;     /* nc*3 */
+  const double *cell_vol;     /* nc   */
+  const double *face_cen;     /* nf*3 */
+  const double *face_area;    /* nf   */
+  const double *face_normal;  /* nf*3 */
+} tdyhip_mesh;
+
+/* Material/constitutive inputs (InitMaterials, src/fv/mpfao/tdympfao.c:630-763;
+ * option defaults src/tdycore.c:260-296).  Host pointers. */
+typedef struct {
+  double vg_m, vg_alpha;                       /* van Genuchten m, alpha [1/Pa] */
+  double poly_x0, poly_x1, poly_x2, poly_dx;   /* Mualem cubic smoothing */
+  double gravity;                              /* |g| (z axis) */
+  double pref;                                 /* reference pressure (101325) */
+  int32_t density_type;                        /* 0 constant, 1 exponential */
+  int32_t gmatrix_method;                      /* 0 MPFAO, 1 TPF */
+  const double *K;                             /* nc*9 row-major tensors */
+  const double *porosity;                      /* nc */
+  const double *Sr;                            /* nc residual saturation */
+  const double *p_bnd_fn;                      /* nbnd boundary-pressure function values
+                                                  at boundary face centroids, or NULL
+                                                  (then P_bnd = p of the adjacent cell) */
+} tdyhip_materials;
+
+const char *tdyhip_last_error(void);
+
+/* Setup: TDySetup_Richards_MPFAO (src/fv/mpfao/tdympfao.c:2277-2331).
+ * Topology on the host, then the G matrices, the per-vertex local systems
+ * (batched LU) and the gravity term on the device. */
+int tdyhip_create(const tdyhip_mesh *mesh, const tdyhip_materials *mat, tdyhip_ctx **out);
+int tdyhip_destroy(tdyhip_ctx *ctx);
+int tdyhip_set_stream(tdyhip_ctx *ctx, tdyhip_stream stream);
+int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out /* [0]=nc [1]=jac nnz [2]=nf [3]=nbnd */);
+
+/* Jacobian sparsity (vertex adjacency, TDyDiscretizationCreateJacobianMatrix,
+ * src/tdydiscretization.c:156-181): HOST arrays rowptr[nc+1], cols[nnz]. */
+int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols);
+/* The same pattern already on the device (read-only, owned by ctx). */
+int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr, const int32_t **cols);
+
+/* TS residual: replaces TDyMPFAOIFunction (src/fv/mpfao/tdympfao_ts.c:153). R is overwritten. */
+int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R,
+                     tdyhip_stream stream);
+/* TS Jacobian: replaces TDyMPFAOIJacobian (src/fv/mpfao/tdympfao_ts.c:520).  Uses the
+ * state of the preceding tdyhip_ifunction, like the reference.  Jvals (nnz, in the
+ * pattern order) is overwritten. */
+int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift,
+                     double *Jvals, tdyhip_stream stream);
+/* TS residual and Jacobian from one pass over the transmissibilities. */
+int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot,
+                               double shift, double *R, double *Jvals, tdyhip_stream stream);
+
+/* SNES form: TDySetDtimeForSNESSolver (src/tdycore.c:1575), TDyMPFAOSNESPreSolve,
+ * TDyMPFAOSNESFunction, TDyMPFAOSNESJacobian (src/fv/mpfao/tdympfao_snes.c:32,67,143). */
+int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt);
+int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream stream);
+int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream stream);
+int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *Jvals, tdyhip_stream stream);
+
+/* Diagnostics: transmissibility (rows face*4+subface, columns cell | nc+bnd) and the
+ * gravity term, copied to HOST arrays, for inspection and parity tests. */
+int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz);
+int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals);
+int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,85 @@
+"""Loader of the in-tree HIP library ``libtdyhip.so`` (ctypes).
+
+There is no fallback: if the library is missing or the GPU is absent, calls
+raise.  ``load()`` only opens the shared object (it works without a GPU);
+device memory is needed only for the evaluation calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtdyhip.so")
+
+_lib = None
+
+# every symbol declared in include/tdyhip.h
+EXPORTS = [
+    "tdyhip_last_error", "tdyhip_create", "tdyhip_destroy", "tdyhip_set_stream", "tdyhip_sizes",
+    "tdyhip_jacobian_pattern", "tdyhip_jacobian_pattern_device", "tdyhip_ifunction", "tdyhip_ijacobian",
+    "tdyhip_ifunction_ijacobian", "tdyhip_set_dtime", "tdyhip_snes_presolve", "tdyhip_snes_function",
+    "tdyhip_snes_jacobian", "tdyhip_trans_nnz", "tdyhip_trans_csr", "tdyhip_gravdis",
+]
+
+
+class TdyHipError(RuntimeError):
+    pass
+
+
+class Mesh(C.Structure):
+    _fields_ = [("nc", C.c_int32), ("nf", C.c_int32), ("nv", C.c_int32), ("nbnd", C.c_int32),
+                ("cell_verts", C.c_void_p), ("cell_faces", C.c_void_p), ("face_verts", C.c_void_p),
+                ("face_cells", C.c_void_p), ("face_bc", C.c_void_p), ("vert_xyz", C.c_void_p),
+                ("cell_cen", C.c_void_p), ("cell_vol", C.c_void_p), ("face_cen", C.c_void_p),
+                ("face_area", C.c_void_p), ("face_normal", C.c_void_p)]
+
+
+class Materials(C.Structure):
+    _fields_ = [("vg_m", C.c_double), ("vg_alpha", C.c_double), ("poly_x0", C.c_double),
+                ("poly_x1", C.c_double), ("poly_x2", C.c_double), ("poly_dx", C.c_double),
+                ("gravity", C.c_double), ("pref", C.c_double), ("density_type", C.c_int32),
+                ("gmatrix_method", C.c_int32), ("K", C.c_void_p), ("porosity", C.c_void_p),
+                ("Sr", C.c_void_p), ("p_bnd_fn", C.c_void_p)]
+
+
+def load():
+    """Open libtdyhip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TdyHipError("libtdyhip.so is not built; run __graft_entry__.build() or python -m tdycore_amd.build")
+    # torch ships its own HIP runtime under the unversioned name libamdhip64.so.
+    # Loading torch first makes this library's libamdhip64.so.7 dependency
+    # resolve to that already-loaded runtime (same SONAME), so the process has
+    # one HIP runtime and device pointers from torch tensors are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(LIB_PATH)
+    P, D, I, L64 = C.c_void_p, C.c_double, C.c_int, C.c_int64
+    L.tdyhip_last_error.restype = C.c_char_p
+    sig = {
+        "tdyhip_create": [P, P, P], "tdyhip_destroy": [P], "tdyhip_set_stream": [P, P],
+        "tdyhip_sizes": [P, P], "tdyhip_jacobian_pattern": [P, P, P],
+        "tdyhip_jacobian_pattern_device": [P, P, P],
+        "tdyhip_ifunction": [P, D, P, P, P, P], "tdyhip_ijacobian": [P, D, P, P, D, P, P],
+        "tdyhip_ifunction_ijacobian": [P, D, P, P, D, P, P, P], "tdyhip_set_dtime": [P, D],
+        "tdyhip_snes_presolve": [P, P, P], "tdyhip_snes_function": [P, P, P, P],
+        "tdyhip_snes_jacobian": [P, P, P, P], "tdyhip_trans_nnz": [P, P],
+        "tdyhip_trans_csr": [P, P, P, P], "tdyhip_gravdis": [P, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().tdyhip_last_error().decode(errors="replace")
+        raise TdyHipError(f"tdyhip error {rc}: {msg}")

@@ -1,0 +1,426 @@
+// C ABI of the MI355X MPFA-O Richards library (include/tdyhip.h).
+// Unity build: the kernel files are compiled into this translation unit.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "flow_kernels.hip"
+#include "setup_kernels.hip"
+#include "tdyhip_internal.h"
+
+namespace tdyhip {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+struct HipError {};
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+      return e_ == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP;                 \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+static int dalloc(tdyhip_ctx *ctx, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP;
+  }
+  ctx->allocations.push_back(*p);
+  return 0;
+}
+
+template <class T>
+static int dupload(tdyhip_ctx *ctx, T **p, const T *h, size_t n) {
+  int rc = dalloc(ctx, p, n);
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static StateParams state_params(const tdyhip_ctx *ctx, int form, double shift) {
+  StateParams sp;
+  sp.vg_m = ctx->vg_m; sp.vg_alpha = ctx->vg_alpha; sp.x0 = ctx->x0; sp.x1 = ctx->x1; sp.dx = ctx->dx;
+  sp.pref = ctx->pref; sp.dtime = ctx->dtime; sp.shift = shift;
+  sp.c0 = ctx->mualem_c[0]; sp.c1 = ctx->mualem_c[1]; sp.c2 = ctx->mualem_c[2]; sp.c3 = ctx->mualem_c[3];
+  sp.density_type = ctx->density_type; sp.form = form;
+  return sp;
+}
+
+// Mualem cubic coefficients: the 4x4 system of CubicPolynomialSetup
+// (src/materials/tdycharacteristiccurves.c:509-535) solved by Gaussian
+// elimination with partial pivoting.
+static int mualem_cubic(double m, double x0, double x1, double x2, double dx, double *c) {
+  double Kr, dKr;
+  mualem_kr(m, x0, 2.0, dx, c, x1, Kr, dKr);  // x1 < 2: unsmoothed branch
+  double b[4] = {1.0, Kr, 0.0, dKr};
+  const double t1 = (x1 - x0) / dx, t2 = (x2 - x0) / dx;
+  double A[4][4] = {{1.0, t2, t2 * t2, t2 * t2 * t2},
+                    {1.0, t1, t1 * t1, t1 * t1 * t1},
+                    {0.0, 1.0 / dx, 2.0 * t2 / dx, 3.0 * t2 * t2 / dx},
+                    {0.0, 1.0 / dx, 2.0 * t1 / dx, 3.0 * t1 * t1 / dx}};
+  for (int k = 0; k < 4; k++) {
+    int p = k;
+    for (int i = k + 1; i < 4; i++)
+      if (fabs(A[i][k]) > fabs(A[p][k])) p = i;
+    if (A[p][k] == 0.0) return 1;
+    if (p != k) {
+      for (int j = 0; j < 4; j++) std::swap(A[k][j], A[p][j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < 4; i++) {
+      const double l = A[i][k] / A[k][k];
+      for (int j = k; j < 4; j++) A[i][j] -= l * A[k][j];
+      b[i] -= l * b[k];
+    }
+  }
+  for (int i = 3; i >= 0; i--) {
+    double s = b[i];
+    for (int j = i + 1; j < 4; j++) s -= A[i][j] * c[j];
+    c[i] = s / A[i][i];
+  }
+  return 0;
+}
+
+static int check_stream(tdyhip_ctx *ctx, tdyhip_stream s, hipStream_t *out) {
+  *out = s ? (hipStream_t)s : ctx->stream;
+  return 0;
+}
+
+static RowArgs row_args(tdyhip_ctx *ctx, const double *U, double *J) {
+  RowArgs a;
+  a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
+  a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts; a.face_flags = ctx->d_face_flags;
+  a.vcols = ctx->d_vcols; a.T = ctx->d_T; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd;
+  a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
+  a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
+  a.jrp = ctx->d_jrp; a.flux = ctx->d_flux; a.J = J;
+  return a;
+}
+
+// state (+boundary) at U
+static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double *Udot, hipStream_t st) {
+  StateParams sp = state_params(ctx, form, 0.0);
+  k_cell_state<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, Udot, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol,
+                                                   ctx->d_accum_prev, ctx->d_rho, ctx->d_ukvr, ctx->d_drho,
+                                                   ctx->d_dukvr, ctx->d_dM, ctx->d_d2M, ctx->d_acc);
+  if (ctx->nbnd > 0)
+    k_bnd_state<<<nblk(ctx->nbnd, 256), 256, 0, st>>>(ctx->nbnd, sp, U, ctx->d_bnd_cell, ctx->d_Sr,
+                                                      ctx->d_pbnd_fn, ctx->d_Pbnd, ctx->d_ukvr_bnd);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// rows + gather.  want_R / want_J select the outputs.
+static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
+                       double *J, hipStream_t st) {
+  const int64_t nrows = (int64_t)4 * ctx->nf;
+  RowArgs a = row_args(ctx, U, J);
+  if (J) k_fill<<<nblk(ctx->jnnz, 256), 256, 0, st>>>(ctx->jnnz, J, 0.0);
+  if (R && J) k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
+  else if (R) k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a);
+  else k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
+  const unsigned g = nblk(ctx->nc, 256);
+  if (R && J)
+    k_cell_gather<true, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                 ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                 Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+  else if (R)
+    k_cell_gather<true, false><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                  ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                  Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+  else
+    k_cell_gather<false, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                  ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                  Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace tdyhip
+
+using namespace tdyhip;
+
+extern "C" {
+
+const char *tdyhip_last_error(void) { return g_err.c_str(); }
+
+int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx **out) {
+  if (!m || !mat || !out) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  *out = nullptr;
+  if (m->nc <= 0 || m->nf <= 0 || m->nv <= 0 || m->nbnd < 0) { set_error("empty mesh"); return TDYHIP_ERR_ARG; }
+  if ((int64_t)m->nf * 4 * kMaxCols > ((int64_t)1 << 40)) { set_error("mesh too large"); return TDYHIP_ERR_ARG; }
+  HostTopology topo;
+  std::string msg = build_topology(*m, topo);
+  if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+
+  tdyhip_ctx *ctx = new tdyhip_ctx();
+  ctx->nc = m->nc; ctx->nf = m->nf; ctx->nv = m->nv; ctx->nbnd = m->nbnd;
+  ctx->vg_m = mat->vg_m; ctx->vg_alpha = mat->vg_alpha; ctx->x0 = mat->poly_x0; ctx->x1 = mat->poly_x1;
+  ctx->x2 = mat->poly_x2; ctx->dx = mat->poly_dx; ctx->gravity = mat->gravity; ctx->pref = mat->pref;
+  ctx->density_type = mat->density_type; ctx->gmatrix = mat->gmatrix_method;
+  if (mualem_cubic(ctx->vg_m, ctx->x0, ctx->x1, ctx->x2, ctx->dx, ctx->mualem_c)) {
+    set_error("Mualem smoothing polynomial is singular");
+    delete ctx;
+    return TDYHIP_ERR_ARG;
+  }
+  int rc = 0;
+  auto fail = [&](int code) {
+    tdyhip_destroy(ctx);
+    return code;
+  };
+  hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) { set_error(std::string("hipStreamCreate: ") + hipGetErrorString(he)); delete ctx; return TDYHIP_ERR_HIP; }
+  ctx->own_stream = true;
+  const int nc = m->nc, nf = m->nf, nv = m->nv, nbnd = m->nbnd;
+  const int64_t nrows = (int64_t)4 * nf;
+  ctx->jnnz = topo.jrp[nc];
+
+  if ((rc = dupload(ctx, &ctx->d_face_cells, m->face_cells, (size_t)2 * nf))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_face_verts, m->face_verts, (size_t)4 * nf))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_cell_faces, m->cell_faces, (size_t)6 * nc))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_face_flags, topo.face_flags.data(), nf))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_cell_vol, m->cell_vol, nc))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_phi, mat->porosity, nc))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_Sr, mat->Sr, nc))) return fail(rc);
+  if (mat->p_bnd_fn) {
+    ctx->has_pbnd_fn = true;
+    if ((rc = dupload(ctx, &ctx->d_pbnd_fn, mat->p_bnd_fn, nbnd))) return fail(rc);
+  }
+  if ((rc = dupload(ctx, &ctx->d_bnd_cell, topo.bnd_cell.data(), nbnd))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_vcols, topo.vcols.data(), topo.vcols.size()))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_row_ul, topo.row_ul.data(), topo.row_ul.size()))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_jslot, topo.jslot.data(), topo.jslot.size()))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_jrp, topo.jrp.data(), topo.jrp.size()))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_jcols, topo.jcols.data(), topo.jcols.size()))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_jdiag, topo.jdiag.data(), topo.jdiag.size()))) return fail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_T, (size_t)kMaxCols * nrows))) return fail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_G, (size_t)nrows))) return fail(rc);
+  for (double **p : {&ctx->d_rho, &ctx->d_ukvr, &ctx->d_drho, &ctx->d_dukvr, &ctx->d_dM, &ctx->d_d2M, &ctx->d_acc,
+                     &ctx->d_accum_prev})
+    if ((rc = dalloc(ctx, p, nc))) return fail(rc);
+  for (double **p : {&ctx->d_Pbnd, &ctx->d_ukvr_bnd, &ctx->d_rho_bnd})
+    if ((rc = dalloc(ctx, p, nbnd))) return fail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_flux, (size_t)nrows))) return fail(rc);
+
+  // setup temporaries
+  int32_t *d_scf = nullptr; IRDesc *d_ir = nullptr; double *d_Gsub = nullptr, *d_K = nullptr, *d_cc = nullptr,
+          *d_fc = nullptr, *d_fa = nullptr, *d_fn = nullptr; int *d_status = nullptr;
+  auto tmp_fail = [&](int code) {
+    for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc,
+                    (void *)d_fa, (void *)d_fn, (void *)d_status})
+      if (p) (void)hipFree(p);
+    return fail(code);
+  };
+#define TMPALLOC(ptr, n, src)                                                                       \
+  do {                                                                                              \
+    hipError_t e_ = hipMalloc((void **)&ptr, (size_t)(n) * sizeof(*ptr));                           \
+    if (e_ != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e_));          \
+      return tmp_fail(e_ == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP); }               \
+    if (src) { e_ = hipMemcpy(ptr, src, (size_t)(n) * sizeof(*ptr), hipMemcpyHostToDevice);        \
+      if (e_ != hipSuccess) { set_error(std::string("hipMemcpy: ") + hipGetErrorString(e_)); return tmp_fail(TDYHIP_ERR_HIP); } } \
+  } while (0)
+  TMPALLOC(d_scf, topo.sc_faces.size(), topo.sc_faces.data());
+  TMPALLOC(d_ir, topo.ir.size(), topo.ir.data());
+  TMPALLOC(d_Gsub, (size_t)nc * 72, (double *)nullptr);
+  TMPALLOC(d_K, (size_t)nc * 9, mat->K);
+  TMPALLOC(d_cc, (size_t)nc * 3, m->cell_cen);
+  TMPALLOC(d_fc, (size_t)nf * 3, m->face_cen);
+  TMPALLOC(d_fa, (size_t)nf, m->face_area);
+  TMPALLOC(d_fn, (size_t)nf * 3, m->face_normal);
+  TMPALLOC(d_status, 1, (int *)nullptr);
+#undef TMPALLOC
+  hipStream_t st = ctx->stream;
+  k_fill<<<nblk((int64_t)kMaxCols * nrows, 256), 256, 0, st>>>((int64_t)kMaxCols * nrows, ctx->d_T, 0.0);
+  if (hipMemsetAsync(d_status, 0, sizeof(int), st) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
+  k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa,
+                                                        d_fn, d_K, d_Gsub);
+  k_local_systems<<<nblk(nv, 64), 64, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows, d_status);
+  k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
+                                           d_fc, d_fa, d_fn, d_K, ctx->d_G);
+  // boundary rho, mu frozen at (Pref, Tref) (AllocateMemoryForBoundaryValues, tdympfao.c:297-336)
+  {
+    double rb, d1, d2;
+    water_density(ctx->density_type, ctx->pref, rb, d1, d2);
+    if (nbnd > 0) k_fill<<<nblk(nbnd, 256), 256, 0, st>>>(nbnd, ctx->d_rho_bnd, rb);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    set_error("setup kernels failed");
+    return tmp_fail(TDYHIP_ERR_HIP);
+  }
+  int status = 0;
+  if (hipMemcpy(&status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
+  for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc, (void *)d_fa,
+                  (void *)d_fn, (void *)d_status})
+    (void)hipFree(p);
+  if (status) {
+    set_error("singular interaction-region system (zero LU pivot)");
+    tdyhip_destroy(ctx);
+    return TDYHIP_ERR_SINGULAR;
+  }
+  ctx->h_jrp = std::move(topo.jrp);
+  ctx->h_jcols = std::move(topo.jcols);
+  ctx->h_vcols = std::move(topo.vcols);
+  ctx->h_face_verts.assign(m->face_verts, m->face_verts + (size_t)4 * nf);
+  ctx->h_face_flags = std::move(topo.face_flags);
+  *out = ctx;
+  return 0;
+}
+
+int tdyhip_destroy(tdyhip_ctx *ctx) {
+  if (!ctx) return 0;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocations) (void)hipFree(p);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int tdyhip_set_stream(tdyhip_ctx *ctx, tdyhip_stream stream) {
+  if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
+  if (!stream) { set_error("null stream"); return TDYHIP_ERR_ARG; }
+  if (ctx->own_stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->own_stream = false;
+  }
+  ctx->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out) {
+  if (!ctx || !out) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  out[0] = ctx->nc; out[1] = ctx->jnnz; out[2] = ctx->nf; out[3] = ctx->nbnd;
+  return 0;
+}
+
+int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols) {
+  if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  std::memcpy(rowptr, ctx->h_jrp.data(), sizeof(int32_t) * (ctx->nc + 1));
+  std::memcpy(cols, ctx->h_jcols.data(), sizeof(int32_t) * ctx->jnnz);
+  return 0;
+}
+
+int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr, const int32_t **cols) {
+  if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  *rowptr = ctx->d_jrp;
+  *cols = ctx->d_jcols;
+  return 0;
+}
+
+int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R, tdyhip_stream s) {
+  (void)t;
+  if (!ctx || !U || !Udot || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  int rc = launch_state(ctx, 0, U, Udot, st);
+  if (rc) return rc;
+  return launch_flux(ctx, 0, 0.0, U, Udot, R, nullptr, st);
+}
+
+int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift, double *J,
+                     tdyhip_stream s) {
+  (void)t;
+  if (!ctx || !U || !Udot || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  return launch_flux(ctx, 0, shift, U, Udot, nullptr, J, st);
+}
+
+int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift,
+                               double *R, double *J, tdyhip_stream s) {
+  (void)t;
+  if (!ctx || !U || !Udot || !R || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  int rc = launch_state(ctx, 0, U, Udot, st);
+  if (rc) return rc;
+  return launch_flux(ctx, 0, shift, U, Udot, R, J, st);
+}
+
+int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt) {
+  if (!ctx || !(dt > 0.0)) { set_error("bad time step"); return TDYHIP_ERR_ARG; }
+  ctx->dtime = dt;
+  return 0;
+}
+
+int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream s) {
+  if (!ctx || !Uprev) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  StateParams sp = state_params(ctx, 1, 0.0);
+  k_snes_presolve<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, Uprev, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol,
+                                                      ctx->d_accum_prev);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream s) {
+  if (!ctx || !U || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  int rc = launch_state(ctx, 1, U, nullptr, st);
+  if (rc) return rc;
+  return launch_flux(ctx, 1, 0.0, U, nullptr, R, nullptr, st);
+}
+
+int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *J, tdyhip_stream s) {
+  if (!ctx || !U || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  return launch_flux(ctx, 1, 0.0, U, nullptr, nullptr, J, st);
+}
+
+int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz) {
+  if (!ctx || !nnz) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  int64_t n = 0;
+  for (int f = 0; f < ctx->nf; f++) {
+    if (ctx->h_face_flags[f] & kFaceNoFlow) continue;
+    for (int q = 0; q < 4; q++) {
+      const int v = ctx->h_face_verts[f * 4 + q];
+      for (int k = 0; k < kMaxCols; k++) n += ctx->h_vcols[(size_t)v * kMaxCols + k] >= 0;
+    }
+  }
+  *nnz = n;
+  return 0;
+}
+
+int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals) {
+  if (!ctx || !rowptr || !cols || !vals) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  const int64_t nrows = (int64_t)4 * ctx->nf;
+  std::vector<double> T((size_t)kMaxCols * nrows);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(T.data(), ctx->d_T, T.size() * sizeof(double), hipMemcpyDeviceToHost));
+  int64_t n = 0;
+  rowptr[0] = 0;
+  for (int64_t r = 0; r < nrows; r++) {
+    const int f = (int)(r >> 2);
+    if (!(ctx->h_face_flags[f] & kFaceNoFlow)) {
+      const int v = ctx->h_face_verts[r];
+      // insertion-sort by column
+      int64_t base = n;
+      for (int k = 0; k < kMaxCols; k++) {
+        const int col = ctx->h_vcols[(size_t)v * kMaxCols + k];
+        if (col < 0) continue;
+        const double val = T[(size_t)k * nrows + r];
+        int64_t p = n;
+        while (p > base && cols[p - 1] > col) { cols[p] = cols[p - 1]; vals[p] = vals[p - 1]; p--; }
+        cols[p] = col; vals[p] = val; n++;
+      }
+    }
+    rowptr[r + 1] = (int32_t)n;
+  }
+  return 0;
+}
+
+int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out) {
+  if (!ctx || !out) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(out, ctx->d_G, sizeof(double) * 4 * ctx->nf, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

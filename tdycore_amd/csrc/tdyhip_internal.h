@@ -1,0 +1,101 @@
+// Internal declarations of the MI355X MPFA-O Richards library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tdyhip.h"
+
+namespace tdyhip {
+
+// Hexahedral interaction regions: at most 8 cells and 12 faces per vertex,
+// and at most 8 transmissibility columns (cells + Dirichlet faces) per row.
+constexpr int kMaxIRCells = 8;
+constexpr int kMaxIRFaces = 12;
+constexpr int kMaxCols = 8;
+
+// face flags
+constexpr uint8_t kFaceBoundary = 1;
+constexpr uint8_t kFaceNoFlow = 2;    // Neumann boundary face: no flux row
+constexpr uint8_t kFaceSeepage = 4;
+
+// Per-vertex description of the local MPFA-O system, built on the host and
+// consumed by the batched setup kernel (all indices vertex-local).
+struct IRDesc {
+  uint8_t ncell, nface, nint, nneu, ndir, pad[3];
+  uint8_t role[kMaxIRFaces];   // 0 internal, 1 no-flow, 2 Dirichlet/seepage
+  uint8_t fidx[kMaxIRFaces];   // unknown index (internal/no-flow) or Dirichlet column index
+  uint8_t fup_i[kMaxIRFaces];  // local cell of the subcell on the up side (or the only one)
+  uint8_t fup_l[kMaxIRFaces];  // its local face (0..2)
+  uint8_t fdn_i[kMaxIRFaces];
+  uint8_t fdn_l[kMaxIRFaces];
+  uint8_t sck[kMaxIRCells][3]; // subcell (local cell i, face l) -> vertex face index
+  int32_t row[kMaxIRFaces];    // output flux row (face*4+subface) per vertex face, -1 for no-flow
+  int32_t subcell[kMaxIRCells];// global subcell id (cell*8 + corner)
+};
+
+struct HostTopology {
+  int nc = 0, nf = 0, nv = 0, nbnd = 0;
+  std::vector<int32_t> vcols;     // nv*kMaxCols: cell ids then nc+b, -1 padding
+  std::vector<IRDesc> ir;         // nv
+  std::vector<int32_t> sc_faces;  // nc*8*3 subcell face triples (oriented)
+  std::vector<uint8_t> row_ul;    // 4*nf*2: (up, dn) local IR index of the row's cells, 255 = none
+  std::vector<uint8_t> jslot;     // nv*64: position of column cell k in Jacobian row of cell i
+  std::vector<int32_t> jrp, jcols, jdiag;
+  std::vector<int32_t> bnd_cell;  // nbnd
+  std::vector<uint8_t> face_flags;// nf
+};
+
+// Builds the host topology; returns an error message or empty string.
+std::string build_topology(const tdyhip_mesh &m, HostTopology &t);
+
+void set_error(const std::string &msg);
+
+}  // namespace tdyhip
+
+struct tdyhip_ctx {
+  int nc = 0, nf = 0, nv = 0, nbnd = 0;
+  int64_t jnnz = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+
+  double vg_m, vg_alpha, x0, x1, x2, dx, gravity, pref;
+  int density_type, gmatrix;
+  double mualem_c[4];
+  double dtime = 1.0;
+  bool has_pbnd_fn = false;
+
+  // device: mesh
+  int32_t *d_face_cells = nullptr;  // nf*2
+  int32_t *d_face_verts = nullptr;  // nf*4
+  int32_t *d_cell_faces = nullptr;  // nc*6
+  uint8_t *d_face_flags = nullptr;  // nf
+  double *d_cell_vol = nullptr, *d_phi = nullptr, *d_Sr = nullptr;
+  double *d_pbnd_fn = nullptr;      // nbnd
+  int32_t *d_bnd_cell = nullptr;    // nbnd
+
+  // device: discretisation
+  int32_t *d_vcols = nullptr;       // nv*kMaxCols
+  double *d_T = nullptr;            // kMaxCols * 4nf (column-major ELL over face-major rows)
+  double *d_G = nullptr;            // 4nf gravity term
+  uint8_t *d_row_ul = nullptr;      // 4nf*2
+  uint8_t *d_jslot = nullptr;       // nv*64
+  int32_t *d_jrp = nullptr, *d_jcols = nullptr, *d_jdiag = nullptr;
+
+  // device: state
+  double *d_rho = nullptr, *d_ukvr = nullptr, *d_drho = nullptr, *d_dukvr = nullptr;
+  double *d_dM = nullptr, *d_d2M = nullptr, *d_acc = nullptr;
+  double *d_Pbnd = nullptr, *d_ukvr_bnd = nullptr, *d_rho_bnd = nullptr;
+  double *d_flux = nullptr;         // 4nf
+  double *d_accum_prev = nullptr;   // nc
+
+  // host copies used by diagnostics
+  std::vector<int32_t> h_jrp, h_jcols;
+  std::vector<int32_t> h_vcols;
+  std::vector<int32_t> h_face_verts;
+  std::vector<uint8_t> h_face_flags;
+  std::vector<void *> allocations;
+};

@@ -1,0 +1,212 @@
+// Host-side topology of the MPFA-O interaction regions (integer work only).
+//
+// Plays the part of TDyMeshCreateFromPlex's vertex/subcell bookkeeping
+// (src/fv/share/tdymeshplex.c:1531-1713 SetupUpwindFacesForSubcell,
+// 2829-2880 SetupSubcells, 236-275 FindFaceIDsOfACellCommonToAVertex,
+// 1384-1412 UpdateFaceOrientationAroundAVertex) and of the vertex-adjacency
+// Jacobian pattern (src/tdydiscretization.c:156-181).  The floating-point
+// work (G matrices, local solves, gravity) runs on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tdyhip_internal.h"
+
+namespace tdyhip {
+
+static int face_pos(const int32_t *fv, int f, int v) {
+  for (int q = 0; q < 4; q++)
+    if (fv[f * 4 + q] == v) return q;
+  return -1;
+}
+
+std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
+  const int nc = m.nc, nf = m.nf, nv = m.nv, nbnd = m.nbnd;
+  t.nc = nc; t.nf = nf; t.nv = nv; t.nbnd = nbnd;
+  const int32_t *cv = m.cell_verts, *cf = m.cell_faces, *fv = m.face_verts, *fc = m.face_cells;
+
+  t.face_flags.assign(nf, 0);
+  t.bnd_cell.assign(nbnd, -1);
+  for (int f = 0; f < nf; f++) {
+    int up = fc[2 * f], dn = fc[2 * f + 1];
+    if (up < 0 && dn < 0) return "face with two boundary sides";
+    if (up < 0 || dn < 0) {
+      if (up >= 0) return "boundary face must be stored as (-(b+1), cell)";
+      int b = -up - 1;
+      if (b >= nbnd) return "boundary index out of range";
+      t.bnd_cell[b] = dn;
+      uint8_t fl = kFaceBoundary;
+      int bc = m.face_bc[f];
+      if (bc == TDYHIP_BC_NEUMANN) fl |= kFaceNoFlow;
+      else if (bc == TDYHIP_BC_SEEPAGE) fl |= kFaceSeepage;
+      else if (bc != TDYHIP_BC_DIRICHLET) return "boundary face without a boundary type";
+      t.face_flags[f] = fl;
+    }
+  }
+  for (int b = 0; b < nbnd; b++)
+    if (t.bnd_cell[b] < 0) return "boundary index without a face";
+
+  // subcell face triples, oriented (normal of the face-centroid triangle toward the cell centroid)
+  t.sc_faces.assign((size_t)nc * 24, -1);
+  for (int c = 0; c < nc; c++) {
+    for (int s = 0; s < 8; s++) {
+      int v = cv[c * 8 + s];
+      int fid[3], n = 0;
+      for (int lf = 0; lf < 6; lf++) {
+        int f = cf[c * 6 + lf];
+        if (face_pos(fv, f, v) >= 0) {
+          if (n < 3) fid[n] = f;
+          n++;
+        }
+      }
+      if (n != 3) return "hexahedral subcell without exactly 3 faces at its vertex";
+      double a[3], b[3], cvec[3];
+      for (int d = 0; d < 3; d++) {
+        a[d] = m.face_cen[3 * fid[1] + d] - m.face_cen[3 * fid[0] + d];
+        b[d] = m.face_cen[3 * fid[2] + d] - m.face_cen[3 * fid[0] + d];
+        cvec[d] = m.cell_cen[3 * c + d] - m.vert_xyz[3 * v + d];
+      }
+      double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      if (axb[0] * cvec[0] + axb[1] * cvec[1] + axb[2] * cvec[2] < 0) std::swap(fid[1], fid[2]);
+      for (int k = 0; k < 3; k++) t.sc_faces[(size_t)(c * 8 + s) * 3 + k] = fid[k];
+    }
+  }
+
+  // vertex -> cells (cell order), vertex -> faces (internal then boundary, face order)
+  std::vector<int32_t> vc_n(nv, 0), vf_n(nv, 0);
+  std::vector<int32_t> vcell((size_t)nv * kMaxIRCells, -1), vsub((size_t)nv * kMaxIRCells, -1);
+  std::vector<int32_t> vface((size_t)nv * kMaxIRFaces, -1), vsubf((size_t)nv * kMaxIRFaces, -1);
+  for (int c = 0; c < nc; c++)
+    for (int s = 0; s < 8; s++) {
+      int v = cv[c * 8 + s];
+      if (v < 0 || v >= nv) return "cell vertex out of range";
+      if (vc_n[v] >= kMaxIRCells) return "more than 8 cells at a vertex";
+      vcell[(size_t)v * kMaxIRCells + vc_n[v]] = c;
+      vsub[(size_t)v * kMaxIRCells + vc_n[v]] = s;
+      vc_n[v]++;
+    }
+  for (int pass = 0; pass < 2; pass++)
+    for (int f = 0; f < nf; f++) {
+      if (((t.face_flags[f] & kFaceBoundary) != 0) != (pass == 1)) continue;
+      for (int q = 0; q < 4; q++) {
+        int v = fv[f * 4 + q];
+        if (vf_n[v] >= kMaxIRFaces) return "more than 12 faces at a vertex";
+        vface[(size_t)v * kMaxIRFaces + vf_n[v]] = f;
+        vsubf[(size_t)v * kMaxIRFaces + vf_n[v]] = q;
+        vf_n[v]++;
+      }
+    }
+
+  t.ir.assign(nv, IRDesc{});
+  t.vcols.assign((size_t)nv * kMaxCols, -1);
+  t.row_ul.assign((size_t)4 * nf * 2, 255);
+  for (int v = 0; v < nv; v++) {
+    IRDesc &d = t.ir[v];
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < kMaxIRFaces; k++) d.row[k] = -1;
+    int ncell = vc_n[v], nface = vf_n[v];
+    d.ncell = (uint8_t)ncell;
+    d.nface = (uint8_t)nface;
+    int nint = 0, nneu = 0, ndir = 0;
+    for (int k = 0; k < nface; k++)
+      if (!(t.face_flags[vface[(size_t)v * kMaxIRFaces + k]] & kFaceBoundary)) nint++;
+    for (int k = 0; k < nface; k++) {
+      int f = vface[(size_t)v * kMaxIRFaces + k];
+      uint8_t fl = t.face_flags[f];
+      if (!(fl & kFaceBoundary)) { d.role[k] = 0; d.fidx[k] = (uint8_t)k; }
+      else if (fl & kFaceNoFlow) { d.role[k] = 1; d.fidx[k] = (uint8_t)(nint + nneu); nneu++; }
+      else { d.role[k] = 2; d.fidx[k] = (uint8_t)ndir; ndir++; }
+      d.fup_i[k] = d.fdn_i[k] = 255;
+    }
+    d.nint = (uint8_t)nint; d.nneu = (uint8_t)nneu; d.ndir = (uint8_t)ndir;
+    if (ncell + ndir > kMaxCols) return "more than 8 transmissibility columns at a vertex";
+    // columns: cells, then Dirichlet faces (boundary index)
+    for (int i = 0; i < ncell; i++) {
+      t.vcols[(size_t)v * kMaxCols + i] = vcell[(size_t)v * kMaxIRCells + i];
+      d.subcell[i] = vcell[(size_t)v * kMaxIRCells + i] * 8 + vsub[(size_t)v * kMaxIRCells + i];
+    }
+    for (int k = 0; k < nface; k++)
+      if (d.role[k] == 2) {
+        int f = vface[(size_t)v * kMaxIRFaces + k];
+        t.vcols[(size_t)v * kMaxCols + ncell + d.fidx[k]] = nc + (-fc[2 * f] - 1);
+      }
+    // subcell faces -> vertex face index; up/down subcell of each face
+    for (int i = 0; i < ncell; i++) {
+      int c = vcell[(size_t)v * kMaxIRCells + i], s = vsub[(size_t)v * kMaxIRCells + i];
+      for (int l = 0; l < 3; l++) {
+        int f = t.sc_faces[(size_t)(c * 8 + s) * 3 + l];
+        int k = 0;
+        while (k < nface && vface[(size_t)v * kMaxIRFaces + k] != f) k++;
+        if (k == nface) return "subcell face not at its vertex";
+        d.sck[i][l] = (uint8_t)k;
+        if (fc[2 * f] == c || (t.face_flags[f] & kFaceBoundary)) { d.fup_i[k] = (uint8_t)i; d.fup_l[k] = (uint8_t)l; }
+        else { d.fdn_i[k] = (uint8_t)i; d.fdn_l[k] = (uint8_t)l; }
+      }
+    }
+    for (int k = 0; k < nface; k++) {
+      if (d.fup_i[k] == 255 || (d.role[k] == 0 && d.fdn_i[k] == 255)) return "face without its subcells";
+      int f = vface[(size_t)v * kMaxIRFaces + k], q = vsubf[(size_t)v * kMaxIRFaces + k];
+      if (d.role[k] != 1) d.row[k] = f * 4 + q;
+      int r = f * 4 + q;
+      int up = fc[2 * f], dn = fc[2 * f + 1];
+      uint8_t ul = 255, dl = 255;
+      for (int i = 0; i < ncell; i++) {
+        int cc = vcell[(size_t)v * kMaxIRCells + i];
+        if (cc == up) ul = (uint8_t)i;
+        if (cc == dn) dl = (uint8_t)i;
+      }
+      t.row_ul[(size_t)r * 2] = ul;
+      t.row_ul[(size_t)r * 2 + 1] = dl;
+    }
+  }
+
+  // Jacobian pattern: cells sharing a vertex, sorted
+  t.jrp.assign(nc + 1, 0);
+  std::vector<int32_t> mark(nc, -1), tmp;
+  tmp.reserve(64);
+  for (int pass = 0; pass < 2; pass++) {
+    for (int c = 0; c < nc; c++) {
+      tmp.clear();
+      for (int s = 0; s < 8; s++) {
+        int v = cv[c * 8 + s];
+        for (int i = 0; i < vc_n[v]; i++) {
+          int cc = vcell[(size_t)v * kMaxIRCells + i];
+          if (mark[cc] != c) { mark[cc] = c; tmp.push_back(cc); }
+        }
+      }
+      if (pass == 0) t.jrp[c + 1] = (int32_t)tmp.size();
+      else {
+        std::sort(tmp.begin(), tmp.end());
+        std::copy(tmp.begin(), tmp.end(), t.jcols.begin() + t.jrp[c]);
+      }
+    }
+    if (pass == 0) {
+      for (int c = 0; c < nc; c++) t.jrp[c + 1] += t.jrp[c];
+      t.jcols.assign(t.jrp[nc], 0);
+      std::fill(mark.begin(), mark.end(), -1);
+    }
+  }
+  t.jdiag.assign(nc, -1);
+  for (int c = 0; c < nc; c++)
+    for (int k = t.jrp[c]; k < t.jrp[c + 1]; k++)
+      if (t.jcols[k] == c) t.jdiag[c] = k;
+  // slot of (row cell i, column cell k) within row i, per vertex
+  t.jslot.assign((size_t)nv * 64, 255);
+  for (int v = 0; v < nv; v++) {
+    int ncell = vc_n[v];
+    for (int i = 0; i < ncell; i++) {
+      int ci = vcell[(size_t)v * kMaxIRCells + i];
+      const int32_t *row = t.jcols.data() + t.jrp[ci];
+      int len = t.jrp[ci + 1] - t.jrp[ci];
+      if (len > 255) return "Jacobian row longer than 255";
+      for (int k = 0; k < ncell; k++) {
+        int ck = vcell[(size_t)v * kMaxIRCells + k];
+        int pos = (int)(std::lower_bound(row, row + len, ck) - row);
+        t.jslot[(size_t)v * 64 + i * 8 + k] = (uint8_t)pos;
+      }
+    }
+  }
+  return std::string();
+}
+
+}  // namespace tdyhip

@@ -1,0 +1,140 @@
+"""Host-side mirror of TDycore's MPFA-O Richards interface over the HIP library.
+
+``MPFAO`` plays the role of a ``TDy`` set up with ``TDySetMode(RICHARDS)`` and
+``TDySetDiscretization(MPFA_O)``: construction is ``TDySetup``
+(src/fv/mpfao/tdympfao.c:2277), and the methods are the callbacks PETSc's
+TS/SNES invoke:
+
+===========================  =====================================================
+method                       reference
+===========================  =====================================================
+``IFunction(t,U,U_t,R)``     TDyMPFAOIFunction   src/fv/mpfao/tdympfao_ts.c:153
+``IJacobian(t,U,U_t,a,J)``   TDyMPFAOIJacobian   src/fv/mpfao/tdympfao_ts.c:520
+``SetDtime(dt)``             TDySetDtimeForSNESSolver  src/tdycore.c:1575
+``SNESPreSolve(Uprev)``      TDyMPFAOSNESPreSolve  src/fv/mpfao/tdympfao_snes.c:32
+``SNESFunction(U,R)``        TDyMPFAOSNESFunction  src/fv/mpfao/tdympfao_snes.c:67
+``SNESJacobian(U,J)``        TDyMPFAOSNESJacobian  src/fv/mpfao/tdympfao_snes.c:143
+===========================  =====================================================
+
+Vectors are float64 torch tensors on the GPU; ``J`` holds the values of the
+vertex-adjacency CSR pattern (``jacobian_pattern()``).  Like the reference,
+``IJacobian`` uses the state of the preceding ``IFunction`` call.
+Errors raise ``TdyHipError`` carrying the library's message (the role of a
+nonzero PetscErrorCode).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .meshgen import Mesh
+
+DENSITY = {"constant": 0, "exponential": 1}
+GMATRIX = {"mpfao": 0, "tpf": 1}
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _cur_stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class MPFAO:
+    def __init__(self, mesh: Mesh, face_bc, K, porosity, Sr, *, vg_m=0.8, vg_alpha=1e-4,
+                 poly=(0.995, 0.99, 1.0, 0.005), gravity=9.8068, density="constant",
+                 gmatrix="mpfao", pref=101325.0, p_bnd_fn: Optional[np.ndarray] = None):
+        L = _lib.load()
+        self.mesh = mesh
+        self.nc, self.nf, self.nbnd = mesh.nc, mesh.nf, mesh.nbnd
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        ms = _lib.Mesh(mesh.nc, mesh.nf, mesh.nv, mesh.nbnd,
+                       arr(mesh.cell_verts, np.int32), arr(mesh.cell_faces, np.int32),
+                       arr(mesh.face_verts, np.int32), arr(mesh.face_cells, np.int32),
+                       arr(face_bc, np.int32), arr(mesh.vert_xyz, np.float64),
+                       arr(mesh.cell_cen, np.float64), arr(mesh.cell_vol, np.float64),
+                       arr(mesh.face_cen, np.float64), arr(mesh.face_area, np.float64),
+                       arr(mesh.face_normal, np.float64))
+        mt = _lib.Materials(vg_m, vg_alpha, poly[0], poly[1], poly[2], poly[3], gravity, pref,
+                            DENSITY[density], GMATRIX[gmatrix], arr(K, np.float64),
+                            arr(porosity, np.float64), arr(Sr, np.float64),
+                            None if p_bnd_fn is None else arr(p_bnd_fn, np.float64))
+        h = C.c_void_p()
+        _lib.check(L.tdyhip_create(C.byref(ms), C.byref(mt), C.byref(h)))
+        self._h = h
+        sz = np.zeros(4, np.int64)
+        _lib.check(L.tdyhip_sizes(self._h, sz.ctypes.data))
+        self.jac_nnz = int(sz[1])
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().tdyhip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- pattern / diagnostics ----
+    def jacobian_pattern(self):
+        rp = np.zeros(self.nc + 1, np.int32)
+        cols = np.zeros(self.jac_nnz, np.int32)
+        _lib.check(_lib.load().tdyhip_jacobian_pattern(self._h, rp.ctypes.data, cols.ctypes.data))
+        return rp, cols
+
+    def trans_csr(self):
+        L = _lib.load()
+        nnz = C.c_int64()
+        _lib.check(L.tdyhip_trans_nnz(self._h, C.byref(nnz)))
+        rp = np.zeros(4 * self.nf + 1, np.int32)
+        cols = np.zeros(nnz.value, np.int32)
+        vals = np.zeros(nnz.value, np.float64)
+        _lib.check(L.tdyhip_trans_csr(self._h, rp.ctypes.data, cols.ctypes.data, vals.ctypes.data))
+        return rp, cols, vals
+
+    def gravdis(self):
+        out = np.zeros(4 * self.nf)
+        _lib.check(_lib.load().tdyhip_gravdis(self._h, out.ctypes.data))
+        return out
+
+    # ---- TS callbacks ----
+    def IFunction(self, t, U, U_t, R):
+        _lib.check(_lib.load().tdyhip_ifunction(self._h, float(t), _ptr(U), _ptr(U_t), _ptr(R), _cur_stream()))
+
+    def IJacobian(self, t, U, U_t, shift, J):
+        _lib.check(_lib.load().tdyhip_ijacobian(self._h, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(J),
+                                                _cur_stream()))
+
+    def IFunctionIJacobian(self, t, U, U_t, shift, R, J):
+        _lib.check(_lib.load().tdyhip_ifunction_ijacobian(self._h, float(t), _ptr(U), _ptr(U_t), float(shift),
+                                                          _ptr(R), _ptr(J), _cur_stream()))
+
+    # ---- SNES callbacks ----
+    def SetDtime(self, dt):
+        _lib.check(_lib.load().tdyhip_set_dtime(self._h, float(dt)))
+
+    def SNESPreSolve(self, Uprev):
+        _lib.check(_lib.load().tdyhip_snes_presolve(self._h, _ptr(Uprev), _cur_stream()))
+
+    def SNESFunction(self, U, R):
+        _lib.check(_lib.load().tdyhip_snes_function(self._h, _ptr(U), _ptr(R), _cur_stream()))
+
+    def SNESJacobian(self, U, J):
+        _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
