@@ -107,6 +107,38 @@ int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream str
 int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream stream);
 int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *Jvals, tdyhip_stream stream);
 
+/* ---- Newton / Krylov (PETSc SNESNEWTONLS + KSPGMRES + PCBJACOBI/ILU(0) defaults) ---- */
+typedef struct {
+  double snes_rtol, snes_atol, snes_stol; /* 1e-8, 1e-50, 1e-8 */
+  int32_t snes_max_it;                    /* 50 */
+  double ksp_rtol, ksp_atol;              /* 1e-5, 1e-50 (preconditioned residual) */
+  int32_t ksp_max_it, ksp_restart;        /* 10000, 30 */
+  int32_t krylov_fp32;                    /* 0: FP64 Jacobian + Krylov; 1: FP32 Jacobian copy,
+                                             FP32 ILU(0) and Krylov vectors, FP32 accumulation */
+  int32_t pc_block;                       /* rows per block-Jacobi ILU(0) block, 1..256 */
+} tdyhip_solver_opts;
+
+typedef struct {
+  int64_t newton_its, linear_its, steps, cuts;
+  int32_t status;                         /* 0 converged, <0 SNESConvergedReason-like failure */
+  int32_t pad;
+  double last_fnorm;
+} tdyhip_solver_stats;
+
+void tdyhip_solver_defaults(tdyhip_solver_opts *o);
+/* One SNESSolve of the SNES form at the current dt / accumulation (U device, in/out). */
+int tdyhip_newton_snes(tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
+/* Fixed-dt SNES stepping: per step TDyMPFAOSNESPreSolve, SNESSolve, U_prev = U. */
+int tdyhip_run_snes_steps(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
+                          tdyhip_solver_stats *st);
+/* TS backward Euler with fixed dt (TSBEULER over TDyMPFAOIFunction/IJacobian). */
+int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
+                     tdyhip_solver_stats *st);
+/* TDyTimeIntegratorRunToTime (src/tdyti.c:100-196): dt growth/cap, sync to final time; a
+ * failed Newton solve cuts dt by 4 and retries from the previous solution (at most 25 cuts). */
+int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max,
+                     double final_time, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
+
 /* Diagnostics: transmissibility (rows face*4+subface, columns cell | nc+bnd) and the
  * gravity term, copied to HOST arrays, for inspection and parity tests. */
 int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz);

@@ -20,6 +20,8 @@ EXPORTS = [
     "tdyhip_jacobian_pattern", "tdyhip_jacobian_pattern_device", "tdyhip_ifunction", "tdyhip_ijacobian",
     "tdyhip_ifunction_ijacobian", "tdyhip_set_dtime", "tdyhip_snes_presolve", "tdyhip_snes_function",
     "tdyhip_snes_jacobian", "tdyhip_trans_nnz", "tdyhip_trans_csr", "tdyhip_gravdis",
+    "tdyhip_solver_defaults", "tdyhip_newton_snes", "tdyhip_run_snes_steps", "tdyhip_run_ts_be",
+    "tdyhip_run_tdyti",
 ]
 
 
@@ -41,6 +43,18 @@ class Materials(C.Structure):
                 ("gravity", C.c_double), ("pref", C.c_double), ("density_type", C.c_int32),
                 ("gmatrix_method", C.c_int32), ("K", C.c_void_p), ("porosity", C.c_void_p),
                 ("Sr", C.c_void_p), ("p_bnd_fn", C.c_void_p)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("snes_rtol", C.c_double), ("snes_atol", C.c_double), ("snes_stol", C.c_double),
+                ("snes_max_it", C.c_int32), ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double),
+                ("ksp_max_it", C.c_int32), ("ksp_restart", C.c_int32), ("krylov_fp32", C.c_int32),
+                ("pc_block", C.c_int32)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("newton_its", C.c_int64), ("linear_its", C.c_int64), ("steps", C.c_int64),
+                ("cuts", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32), ("last_fnorm", C.c_double)]
 
 
 def load():
@@ -70,11 +84,15 @@ def load():
         "tdyhip_snes_presolve": [P, P, P], "tdyhip_snes_function": [P, P, P, P],
         "tdyhip_snes_jacobian": [P, P, P, P], "tdyhip_trans_nnz": [P, P],
         "tdyhip_trans_csr": [P, P, P, P], "tdyhip_gravdis": [P, P],
+        "tdyhip_newton_snes": [P, P, P, P], "tdyhip_run_snes_steps": [P, P, D, C.c_int32, P, P],
+        "tdyhip_run_ts_be": [P, P, D, C.c_int32, P, P], "tdyhip_run_tdyti": [P, P, D, D, D, D, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = I
+    L.tdyhip_solver_defaults.argtypes = [P]
+    L.tdyhip_solver_defaults.restype = None
     _lib = L
     return L
 

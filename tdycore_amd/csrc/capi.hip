@@ -9,6 +9,11 @@
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
+struct SolverWS;
+static void ws_destroy(SolverWS *ws);
+}  // namespace tdyhip
+
+namespace tdyhip {
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
@@ -279,6 +284,8 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
 int tdyhip_destroy(tdyhip_ctx *ctx) {
   if (!ctx) return 0;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  tdyhip::ws_destroy(ctx->ws);
+  ctx->ws = nullptr;
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -424,3 +431,13 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out) {
 }
 
 }  // extern "C"
+
+#include "solver.hip"
+
+namespace tdyhip {
+static void ws_destroy(SolverWS *ws) {
+  if (!ws) return;
+  ws_free(*ws);
+  delete ws;
+}
+}  // namespace tdyhip

@@ -1,0 +1,576 @@
+// Newton (SNESNEWTONLS + backtracking line search + SNESConvergedDefault) and
+// GMRES(m) with block-Jacobi ILU(0) driven from the host on device vectors,
+// plus the time loops: TS backward Euler (TSBEULER), fixed-dt SNES stepping
+// (demo/transient/transient_snes_mpfaof90.F90:406-465) and
+// TDyTimeIntegratorRunToTime (src/tdyti.c:71-196).
+#include <algorithm>
+#include <cmath>
+#include <deque>
+
+#include "krylov.hip"
+#include "tdyhip_internal.h"
+
+namespace tdyhip {
+
+struct HostBJ {
+  int nblocks = 0;
+  std::vector<int32_t> row, jpos;
+  std::vector<uint8_t> lcol, diag, nslot;
+  std::vector<uint16_t> levL, levU;
+  std::vector<int32_t> nlev;
+};
+
+// Breadth-first partition of the cells into blocks of at most `bs` cells over
+// the Jacobian's vertex-adjacency graph, then the in-block ILU(0) slots and
+// level schedules.
+static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols, int nc, int bs,
+                            HostBJ &out) {
+  if (bs < 1 || bs > kBlk) return "block size must be in [1,256]";
+  std::vector<int32_t> blk(nc, -1), local(nc, -1);
+  std::vector<std::vector<int32_t>> members;
+  std::vector<int32_t> queue;
+  queue.reserve(4096);
+  int seed = 0;
+  while (true) {
+    while (seed < nc && blk[seed] >= 0) seed++;
+    if (seed >= nc) break;
+    const int b = (int)members.size();
+    members.emplace_back();
+    auto &mem = members.back();
+    queue.clear();
+    queue.push_back(seed);
+    blk[seed] = b;
+    size_t head = 0;
+    while (head < queue.size() && (int)mem.size() < bs) {
+      const int c = queue[head++];
+      mem.push_back(c);
+      for (int k = jrp[c]; k < jrp[c + 1]; k++) {
+        const int nb = jcols[k];
+        if (blk[nb] < 0 && (int)queue.size() < bs) {
+          blk[nb] = b;
+          queue.push_back(nb);
+        }
+      }
+    }
+    // cells queued but not taken (cannot happen: queue is capped at bs) -> release
+    for (size_t q = head; q < queue.size(); q++)
+      if ((int)mem.size() < bs) mem.push_back(queue[q]);
+      else blk[queue[q]] = -1;
+    std::sort(mem.begin(), mem.end());
+  }
+  const int nb = (int)members.size();
+  out.nblocks = nb;
+  out.row.assign((size_t)nb * kBlk, -1);
+  out.jpos.assign((size_t)nb * kW * kBlk, -1);
+  out.lcol.assign((size_t)nb * kW * kBlk, 255);
+  out.diag.assign((size_t)nb * kBlk, 0);
+  out.nslot.assign((size_t)nb * kBlk, 0);
+  out.levL.assign((size_t)nb * kBlk, 0);
+  out.levU.assign((size_t)nb * kBlk, 0);
+  out.nlev.assign((size_t)nb * 2, 0);
+  for (int b = 0; b < nb; b++) {
+    const auto &mem = members[b];
+    const int m = (int)mem.size();
+    for (int i = 0; i < m; i++) local[mem[i]] = i;
+    int maxL = 0, maxU = 0;
+    for (int i = 0; i < m; i++) {
+      const int r = mem[i];
+      out.row[(size_t)b * kBlk + i] = r;
+      int ns = 0;
+      for (int k = jrp[r]; k < jrp[r + 1]; k++) {
+        const int c = jcols[k];
+        if (blk[c] != b) continue;
+        if (ns >= kW) return "more than 27 in-block entries in a row";
+        out.jpos[(size_t)b * kW * kBlk + (size_t)ns * kBlk + i] = k;
+        out.lcol[(size_t)b * kW * kBlk + (size_t)ns * kBlk + i] = (uint8_t)local[c];
+        if (c == r) out.diag[(size_t)b * kBlk + i] = (uint8_t)ns;
+        ns++;
+      }
+      out.nslot[(size_t)b * kBlk + i] = (uint8_t)ns;
+    }
+    // levels: L forward, U backward
+    for (int i = 0; i < m; i++) {
+      int lv = 0;
+      const int di = out.diag[(size_t)b * kBlk + i];
+      for (int p = 0; p < di; p++) {
+        const int k = out.lcol[(size_t)b * kW * kBlk + (size_t)p * kBlk + i];
+        lv = std::max(lv, (int)out.levL[(size_t)b * kBlk + k] + 1);
+      }
+      out.levL[(size_t)b * kBlk + i] = (uint16_t)lv;
+      maxL = std::max(maxL, lv + 1);
+    }
+    for (int i = m - 1; i >= 0; i--) {
+      int lv = 0;
+      const int di = out.diag[(size_t)b * kBlk + i], ns = out.nslot[(size_t)b * kBlk + i];
+      for (int p = di + 1; p < ns; p++) {
+        const int k = out.lcol[(size_t)b * kW * kBlk + (size_t)p * kBlk + i];
+        lv = std::max(lv, (int)out.levU[(size_t)b * kBlk + k] + 1);
+      }
+      out.levU[(size_t)b * kBlk + i] = (uint16_t)lv;
+      maxU = std::max(maxU, lv + 1);
+    }
+    out.nlev[2 * b] = maxL;
+    out.nlev[2 * b + 1] = maxU;
+    for (int i = 0; i < m; i++) local[mem[i]] = -1;
+  }
+  return std::string();
+}
+
+struct SolverWS {
+  int n = 0, restart = 0, fp32 = -1, bs = 0;
+  int64_t nnz = 0;
+  BJLayout bj;
+  std::vector<void *> allocs;
+  // Newton vectors (FP64)
+  double *F = nullptr, *Y = nullptr, *W = nullptr, *G = nullptr, *JY = nullptr, *Udot = nullptr, *J = nullptr;
+  double *Uprev_ts = nullptr, *Usave = nullptr;
+  // Krylov (TV)
+  void *V = nullptr, *w = nullptr, *z = nullptr, *b = nullptr, *x = nullptr, *Jk = nullptr, *fac = nullptr,
+       *partial = nullptr;
+  double *dpartial = nullptr, *h = nullptr;
+  int *status = nullptr;
+  int nchunk = 0;
+};
+
+static void ws_free(SolverWS &ws) {
+  for (void *p : ws.allocs) (void)hipFree(p);
+  ws.allocs.clear();
+  ws = SolverWS();
+}
+
+template <class T>
+static int ws_alloc(SolverWS &ws, T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc (solver workspace): ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP;
+  }
+  ws.allocs.push_back(*p);
+  return 0;
+}
+
+static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
+  if (!ctx->ws) ctx->ws = new SolverWS();
+  SolverWS &ws = *ctx->ws;
+  const int fp32 = o.krylov_fp32 ? 1 : 0;
+  if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.fp32 == fp32 && ws.bs == o.pc_block) return 0;
+  ws_free(ws);
+  HostBJ hb;
+  std::string msg = build_bj(ctx->h_jrp, ctx->h_jcols, ctx->nc, o.pc_block, hb);
+  if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+  const int n = ctx->nc;
+  const int64_t nnz = ctx->jnnz;
+  ws.n = n; ws.restart = o.ksp_restart; ws.fp32 = fp32; ws.bs = o.pc_block; ws.nnz = nnz;
+  ws.nchunk = (int)((n + kChunk - 1) / kChunk);
+  int rc = 0;
+  const size_t tv = fp32 ? sizeof(float) : sizeof(double);
+  auto up = [&](auto **p, const auto &vec) -> int {
+    int r = ws_alloc(ws, p, vec.size());
+    if (r) return r;
+    if (hipMemcpy(*p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("hipMemcpy (block-Jacobi layout)");
+      return TDYHIP_ERR_HIP;
+    }
+    return 0;
+  };
+  ws.bj.nblocks = hb.nblocks;
+  if ((rc = up(&ws.bj.d_row, hb.row)) || (rc = up(&ws.bj.d_jpos, hb.jpos)) || (rc = up(&ws.bj.d_lcol, hb.lcol)) ||
+      (rc = up(&ws.bj.d_diag, hb.diag)) || (rc = up(&ws.bj.d_nslot, hb.nslot)) || (rc = up(&ws.bj.d_levL, hb.levL)) ||
+      (rc = up(&ws.bj.d_levU, hb.levU)) || (rc = up(&ws.bj.d_nlev, hb.nlev)))
+    return rc;
+  for (double **p : {&ws.F, &ws.Y, &ws.W, &ws.G, &ws.JY, &ws.Udot, &ws.Uprev_ts, &ws.Usave})
+    if ((rc = ws_alloc(ws, p, n))) return rc;
+  if ((rc = ws_alloc(ws, &ws.J, nnz))) return rc;
+  char **cp[] = {(char **)&ws.V, (char **)&ws.w, (char **)&ws.z, (char **)&ws.b, (char **)&ws.x, (char **)&ws.fac,
+                 (char **)&ws.partial};
+  size_t cnt[] = {(size_t)(o.ksp_restart + 1) * n, (size_t)n, (size_t)n, (size_t)n, (size_t)n,
+                  (size_t)hb.nblocks * kW * kBlk, (size_t)(o.ksp_restart + 2) * ws.nchunk};
+  for (int i = 0; i < 7; i++)
+    if ((rc = ws_alloc(ws, cp[i], cnt[i] * tv))) return rc;
+  if (fp32) {
+    if ((rc = ws_alloc(ws, (float **)&ws.Jk, nnz))) return rc;
+  } else {
+    ws.Jk = ws.J;
+  }
+  if ((rc = ws_alloc(ws, &ws.dpartial, ws.nchunk))) return rc;
+  if ((rc = ws_alloc(ws, &ws.h, o.ksp_restart + 2))) return rc;
+  if ((rc = ws_alloc(ws, &ws.status, 1))) return rc;
+  return 0;
+}
+
+#define SCHK(expr)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+      return TDYHIP_ERR_HIP;                                                            \
+    }                                                                                   \
+  } while (0)
+
+static int dev_dot(tdyhip_ctx *ctx, const double *x, const double *y, double *out, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  k_dot<double, double><<<ws.nchunk, kRedBlock, 0, st>>>(ws.n, x, y, ws.dpartial);
+  k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
+  SCHK(hipMemcpyAsync(out, ws.h, sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t st) {
+  double s;
+  int rc = dev_dot(ctx, x, x, &s, st);
+  *out = std::sqrt(s);
+  return rc;
+}
+
+// Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with GMRES.
+template <class TV>
+static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec, int *its_out,
+                 hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const int n = ws.n, m = o.ksp_restart;
+  const int64_t N = n;
+  TV *V = (TV *)ws.V, *w = (TV *)ws.w, *z = (TV *)ws.z, *b = (TV *)ws.b, *x = (TV *)ws.x, *fac = (TV *)ws.fac;
+  TV *Jk = (TV *)ws.Jk, *part = (TV *)ws.partial;
+  const unsigned g256 = (unsigned)((N + 255) / 256);
+  if (!std::is_same<TV, double>::value) k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
+  SCHK(hipMemsetAsync(ws.status, 0, sizeof(int), st));
+  k_bj_factor<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, ws.J, fac, ws.status);
+  k_convert<double, TV><<<g256, 256, 0, st>>>(N, bvec, b);
+  k_fill<<<g256, 256, 0, st>>>(N, yvec, 0.0);
+  SCHK(hipGetLastError());
+  int status = 0;
+  SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
+
+  auto apply_op = [&](const TV *in, TV *out) {  // out = M^-1 A in
+    k_spmv<TV><<<g256, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, Jk, in, z);
+    k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, z, out);
+  };
+  auto norm_of = [&](const TV *v, double *out) -> int {
+    k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
+    k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
+    SCHK(hipMemcpyAsync(out, ws.h, sizeof(double), hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    *out = std::sqrt(*out);
+    return 0;
+  };
+  // r0 = M^-1 b  (x0 = 0)
+  k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, b, w);
+  double beta;
+  if (norm_of(w, &beta)) return TDYHIP_ERR_HIP;
+  if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
+  const double tol = std::max(o.ksp_rtol * beta, o.ksp_atol);
+  int its = 0;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hcol(m + 2), yk(m);
+  bool conv = !(beta > tol);
+  // x (TV) accumulates the solution
+  if (std::is_same<TV, double>::value) {
+    x = (TV *)yvec;
+  } else {
+    k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)0, b, x);
+  }
+  while (!conv && its < o.ksp_max_it) {
+    k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)(1.0 / beta), w, V);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int j = 0;
+    for (; j < m; j++) {
+      TV *vj = V + (int64_t)j * N;
+      apply_op(vj, w);
+      k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk);
+      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h);
+      k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part);
+      k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1);
+      SCHK(hipMemcpyAsync(hcol.data(), ws.h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+      SCHK(hipStreamSynchronize(st));
+      const double hn = std::sqrt(hcol[j + 1]);
+      for (int i = 0; i <= j; i++) H[i + (size_t)j * (m + 1)] = hcol[i];
+      H[j + 1 + (size_t)j * (m + 1)] = hn;
+      if (hn > 0) k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)(1.0 / hn), w, V + (int64_t)(j + 1) * N);
+      for (int i = 0; i < j; i++) {
+        const double a = H[i + (size_t)j * (m + 1)], c = H[i + 1 + (size_t)j * (m + 1)];
+        H[i + (size_t)j * (m + 1)] = cs[i] * a + sn[i] * c;
+        H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * a + cs[i] * c;
+      }
+      const double a = H[j + (size_t)j * (m + 1)], c = H[j + 1 + (size_t)j * (m + 1)];
+      const double d = std::sqrt(a * a + c * c);
+      cs[j] = d > 0 ? a / d : 1.0;
+      sn[j] = d > 0 ? c / d : 0.0;
+      H[j + (size_t)j * (m + 1)] = d;
+      H[j + 1 + (size_t)j * (m + 1)] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      its++;
+      if (std::fabs(gv[j + 1]) <= tol || its >= o.ksp_max_it || hn == 0.0) { j++; conv = std::fabs(gv[j]) <= tol || hn == 0.0; break; }
+    }
+    if (j > m) j = m;
+    for (int i = j - 1; i >= 0; i--) {
+      double s = gv[i];
+      for (int k = i + 1; k < j; k++) s -= H[i + (size_t)k * (m + 1)] * yk[k];
+      yk[i] = s / H[i + (size_t)i * (m + 1)];
+    }
+    SCHK(hipMemcpyAsync(ws.h, yk.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+    k_gemv_update<TV, TV><<<g256, 256, 0, st>>>(N, j, V, N, ws.h, x);
+    if (conv || its >= o.ksp_max_it) break;
+    // restart: w = M^-1 (b - A x)
+    k_spmv<TV><<<g256, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, Jk, x, z);
+    k_sub<TV><<<g256, 256, 0, st>>>(N, b, z, z);
+    k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, z, w);
+    if (norm_of(w, &beta)) return TDYHIP_ERR_HIP;
+    conv = !(beta > tol);
+  }
+  if (!std::is_same<TV, double>::value) k_convert<TV, double><<<g256, 256, 0, st>>>(N, x, yvec);
+  SCHK(hipGetLastError());
+  *its_out = its;
+  return 0;
+}
+
+}  // namespace tdyhip
+
+namespace tdyhip {
+
+struct Form {
+  int type;             // 0 TS backward Euler, 1 SNES
+  const double *Uprev;  // TS only
+  double dt;            // TS only
+};
+
+__global__ void k_udot(int64_t n, double inv_dt, const double *__restrict__ U, const double *__restrict__ Up,
+                       double *__restrict__ Ud) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Ud[i] = (U[i] - Up[i]) * inv_dt;
+}
+
+static int eval_F(tdyhip_ctx *ctx, const Form &fm, const double *X, double *F, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  if (fm.type == 0) {
+    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
+    int rc = launch_state(ctx, 0, X, ws.Udot, st);
+    if (rc) return rc;
+    return launch_flux(ctx, 0, 0.0, X, ws.Udot, F, nullptr, st);
+  }
+  int rc = launch_state(ctx, 1, X, nullptr, st);
+  if (rc) return rc;
+  return launch_flux(ctx, 1, 0.0, X, nullptr, F, nullptr, st);
+}
+
+static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  if (fm.type == 0) {
+    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
+    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st);
+  }
+  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st);
+}
+
+static int linsolve(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *b, double *y, int *its,
+                    hipStream_t st) {
+  return o.krylov_fp32 ? gmres<float>(ctx, o, b, y, its, st) : gmres<double>(ctx, o, b, y, its, st);
+}
+
+// SNESSolve_NEWTONLS with SNESLINESEARCHBT and SNESConvergedDefault
+// (PETSc 3.17 defaults: atol 1e-50, stol 1e-8, maxit 50, divtol 1e4,
+// line-search alpha 1e-4, minlambda 1e-12, maxstep 1e8).
+static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solver_opts &o, tdyhip_solver_stats &stats,
+                  hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const int n = ctx->nc;
+  const unsigned g = nblk(n, 256);
+  const double alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8, divtol = 1e4;
+  int rc;
+  if ((rc = eval_F(ctx, fm, X, ws.F, st))) return rc;
+  double fnorm;
+  if ((rc = dev_norm(ctx, ws.F, &fnorm, st))) return rc;
+  const double fnorm0 = fnorm, ttol = o.snes_rtol * fnorm;
+  int status = 0, it = 0;
+  if (!(fnorm == fnorm)) { stats.status = -4; return 0; }
+  if (fnorm < o.snes_atol) { stats.status = 0; stats.last_fnorm = fnorm; return 0; }
+  for (it = 1; it <= o.snes_max_it; it++) {
+    if ((rc = eval_J(ctx, fm, X, ws.J, st))) return rc;
+    int lits = 0;
+    rc = linsolve(ctx, o, ws.F, ws.Y, &lits, st);
+    stats.linear_its += lits;
+    if (rc == TDYHIP_ERR_SINGULAR) { status = -3; break; }
+    if (rc) return rc;
+    double ynorm;
+    if ((rc = dev_norm(ctx, ws.Y, &ynorm, st))) return rc;
+    if (ynorm == 0.0) break;
+    if (ynorm > maxstep) {
+      k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
+      ynorm = maxstep;
+    }
+    k_spmv<double><<<g, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, ws.J, ws.Y, ws.JY);
+    double initslope;
+    if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
+    if (initslope > 0.0) initslope = -initslope;
+    if (initslope == 0.0) initslope = -1.0;
+    double lambda = 1.0, gnorm;
+    k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
+    if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
+    if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
+    if (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
+      double lambdaprev = lambda, gnormprev = gnorm;
+      double lt = -initslope / (gnorm * gnorm - fnorm * fnorm - 2.0 * lambda * initslope);
+      if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+      lambda = (lt <= 0.1 * lambda) ? 0.1 * lambda : lt;
+      k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
+      if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
+      if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
+      bool failed = false;
+      while (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
+        if (lambda <= minlambda) { failed = true; break; }
+        const double t1 = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lambda * initslope;
+        const double t2 = 0.5 * (gnormprev * gnormprev - fnorm * fnorm) - lambdaprev * initslope;
+        const double a = (t1 / (lambda * lambda) - t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
+        const double bb = (-lambdaprev * t1 / (lambda * lambda) + lambda * t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
+        double d = bb * bb - 3 * a * initslope;
+        if (d < 0.0) d = 0.0;
+        lt = (a == 0.0) ? -initslope / (2.0 * bb) : (-bb + std::sqrt(d)) / (3.0 * a);
+        lambdaprev = lambda; gnormprev = gnorm;
+        if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+        lambda = (lt <= 0.1 * lambda) ? 0.1 * lambda : lt;
+        k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
+        if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
+        if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
+      }
+      if (failed) {
+        double xnorm;
+        if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
+        // SNES_CONVERGED_SNORM_RELATIVE when the step is negligible
+        status = (ynorm < o.snes_stol * xnorm) ? 0 : -6;
+        break;
+      }
+    }
+    SCHK(hipMemcpyAsync(X, ws.W, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    std::swap(ws.F, ws.G);
+    fnorm = gnorm;
+    double xnorm;
+    if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
+    ynorm *= lambda;
+    if (!(fnorm == fnorm)) { status = -4; break; }
+    if (fnorm < o.snes_atol) break;
+    if (fnorm <= ttol) break;
+    if (ynorm < o.snes_stol * xnorm) break;
+    if (fnorm > divtol * fnorm0) { status = -2; break; }
+  }
+  if (it > o.snes_max_it) { status = -5; it = o.snes_max_it; }
+  stats.newton_its += it;
+  stats.status = status;
+  stats.last_fnorm = fnorm;
+  return 0;
+}
+
+}  // namespace tdyhip
+
+extern "C" {
+
+void tdyhip_solver_defaults(tdyhip_solver_opts *o) {
+  if (!o) return;
+  o->snes_rtol = 1e-8; o->snes_atol = 1e-50; o->snes_stol = 1e-8; o->snes_max_it = 50;
+  o->ksp_rtol = 1e-5; o->ksp_atol = 1e-50; o->ksp_max_it = 10000; o->ksp_restart = 30;
+  o->krylov_fp32 = 0; o->pc_block = 256;
+}
+
+static int check_opts(const tdyhip_solver_opts *o) {
+  if (!o) { tdyhip::set_error("null solver options"); return TDYHIP_ERR_ARG; }
+  if (o->ksp_restart < 1 || o->ksp_restart > 200 || o->pc_block < 1 || o->pc_block > tdyhip::kBlk ||
+      o->snes_max_it < 0 || o->ksp_max_it < 1) {
+    tdyhip::set_error("bad solver options");
+    return TDYHIP_ERR_ARG;
+  }
+  return 0;
+}
+
+int tdyhip_newton_snes(tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, tdyhip_solver_stats *st) {
+  using namespace tdyhip;
+  if (!ctx || !U || !st) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  int rc = check_opts(o);
+  if (rc || (rc = ensure_ws(ctx, *o))) return rc;
+  *st = tdyhip_solver_stats{};
+  Form fm{1, nullptr, 0.0};
+  rc = newton(ctx, fm, U, *o, *st, ctx->stream);
+  if (!rc) { SCHK(hipStreamSynchronize(ctx->stream)); }
+  return rc;
+}
+
+int tdyhip_run_snes_steps(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
+                          tdyhip_solver_stats *st) {
+  using namespace tdyhip;
+  if (!ctx || !U || !st || nsteps < 0) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  int rc = check_opts(o);
+  if (rc || (rc = ensure_ws(ctx, *o))) return rc;
+  if ((rc = tdyhip_set_dtime(ctx, dt))) return rc;
+  *st = tdyhip_solver_stats{};
+  Form fm{1, nullptr, 0.0};
+  for (int s = 0; s < nsteps; s++) {
+    if ((rc = tdyhip_snes_presolve(ctx, U, ctx->stream))) return rc;
+    if ((rc = newton(ctx, fm, U, *o, *st, ctx->stream))) return rc;
+    st->steps++;
+    if (st->status) { set_error("Newton did not converge"); return TDYHIP_ERR_NOCONV; }
+  }
+  SCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
+                     tdyhip_solver_stats *st) {
+  using namespace tdyhip;
+  if (!ctx || !U || !st || nsteps < 0 || !(dt > 0)) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  int rc = check_opts(o);
+  if (rc || (rc = ensure_ws(ctx, *o))) return rc;
+  *st = tdyhip_solver_stats{};
+  SolverWS &ws = *ctx->ws;
+  Form fm{0, ws.Uprev_ts, dt};
+  for (int s = 0; s < nsteps; s++) {
+    SCHK(hipMemcpyAsync(ws.Uprev_ts, U, sizeof(double) * ctx->nc, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = newton(ctx, fm, U, *o, *st, ctx->stream))) return rc;
+    st->steps++;
+    if (st->status) { set_error("Newton did not converge"); return TDYHIP_ERR_NOCONV; }
+  }
+  SCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max, double final_time,
+                     const tdyhip_solver_opts *o, tdyhip_solver_stats *st) {
+  using namespace tdyhip;
+  if (!ctx || !U || !st || !(dt_init > 0)) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  int rc = check_opts(o);
+  if (rc || (rc = ensure_ws(ctx, *o))) return rc;
+  *st = tdyhip_solver_stats{};
+  SolverWS &ws = *ctx->ws;
+  Form fm{1, nullptr, 0.0};
+  double time = 0.0, dt = dt_init, dt_save = 0.0;
+  int icut = 0;
+  while (time < final_time) {
+    const double dt_for_sync = final_time - time;
+    if (dt > dt_for_sync) { dt_save = dt; dt = dt_for_sync; }
+    if ((rc = tdyhip_set_dtime(ctx, dt))) return rc;
+    bool conv = false;
+    while (!conv) {
+      SCHK(hipMemcpyAsync(ws.Usave, U, sizeof(double) * ctx->nc, hipMemcpyDeviceToDevice, ctx->stream));
+      if ((rc = tdyhip_snes_presolve(ctx, U, ctx->stream))) return rc;
+      if ((rc = newton(ctx, fm, U, *o, *st, ctx->stream))) return rc;
+      if (st->status < 0) {
+        icut++;
+        st->cuts++;
+        dt *= 0.25;
+        if ((rc = tdyhip_set_dtime(ctx, dt))) return rc;
+        SCHK(hipMemcpyAsync(U, ws.Usave, sizeof(double) * ctx->nc, hipMemcpyDeviceToDevice, ctx->stream));
+        if (icut == 25) { set_error("time step cut 25 times"); return TDYHIP_ERR_NOCONV; }
+      } else {
+        conv = true;
+        time += dt;
+        st->steps++;
+      }
+    }
+    if (dt_save > 0.) { time = final_time; dt = dt_save; dt_save = 0.; }
+    dt *= growth;
+    if (dt > dt_max) dt = dt_max;
+  }
+  SCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -54,6 +54,8 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t);
 
 void set_error(const std::string &msg);
 
+struct SolverWS;
+
 }  // namespace tdyhip
 
 struct tdyhip_ctx {
@@ -98,4 +100,5 @@ struct tdyhip_ctx {
   std::vector<int32_t> h_face_verts;
   std::vector<uint8_t> h_face_flags;
   std::vector<void *> allocations;
+  tdyhip::SolverWS *ws = nullptr;  // Newton/Krylov workspace, allocated on first solve
 };

@@ -138,3 +138,47 @@ class MPFAO:
 
     def SNESJacobian(self, U, J):
         _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
+
+    # ---- Newton / time loops on the device ----
+    @staticmethod
+    def solver_options(**kw):
+        """PETSc-default solver options (SNES rtol 1e-8, GMRES(30), KSP rtol 1e-5,
+        block-Jacobi ILU(0)); keyword overrides by field name."""
+        o = _lib.SolverOpts()
+        _lib.load().tdyhip_solver_defaults(C.byref(o))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown solver option {k}")
+            setattr(o, k, v)
+        return o
+
+    @staticmethod
+    def _stats_dict(st):
+        return {"newton_its": st.newton_its, "linear_its": st.linear_its, "steps": st.steps,
+                "cuts": st.cuts, "status": st.status, "last_fnorm": st.last_fnorm}
+
+    def newton_snes(self, U, opts=None):
+        o = opts or self.solver_options()
+        st = _lib.SolverStats()
+        _lib.check(_lib.load().tdyhip_newton_snes(self._h, _ptr(U), C.byref(o), C.byref(st)))
+        return self._stats_dict(st)
+
+    def run_snes_steps(self, U, dt, nsteps, opts=None):
+        o = opts or self.solver_options()
+        st = _lib.SolverStats()
+        _lib.check(_lib.load().tdyhip_run_snes_steps(self._h, _ptr(U), float(dt), int(nsteps), C.byref(o),
+                                                     C.byref(st)))
+        return self._stats_dict(st)
+
+    def run_ts_be(self, U, dt, nsteps, opts=None):
+        o = opts or self.solver_options()
+        st = _lib.SolverStats()
+        _lib.check(_lib.load().tdyhip_run_ts_be(self._h, _ptr(U), float(dt), int(nsteps), C.byref(o), C.byref(st)))
+        return self._stats_dict(st)
+
+    def run_tdyti(self, U, dt_init, growth, dt_max, final_time, opts=None):
+        o = opts or self.solver_options()
+        st = _lib.SolverStats()
+        _lib.check(_lib.load().tdyhip_run_tdyti(self._h, _ptr(U), float(dt_init), float(growth), float(dt_max),
+                                                float(final_time), C.byref(o), C.byref(st)))
+        return self._stats_dict(st)
