@@ -126,6 +126,10 @@ typedef struct {
 } tdyhip_solver_stats;
 
 void tdyhip_solver_defaults(tdyhip_solver_opts *o);
+/* Block-Jacobi partition: block id per cell (HOST array, nc entries, every block at
+ * most pc_block cells), the role of PCBJacobiSetLocalBlocks; NULL restores the
+ * default breadth-first partition over the Jacobian graph. */
+int tdyhip_set_pc_partition(tdyhip_ctx *ctx, const int32_t *block_of_cell);
 /* One SNESSolve of the SNES form at the current dt / accumulation (U device, in/out). */
 int tdyhip_newton_snes(tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
 /* Fixed-dt SNES stepping: per step TDyMPFAOSNESPreSolve, SNESSolve, U_prev = U. */

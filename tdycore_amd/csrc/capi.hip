@@ -2,6 +2,7 @@
 // Unity build: the kernel files are compiled into this translation unit.
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 
 #include "flow_kernels.hip"
@@ -101,14 +102,15 @@ static int check_stream(tdyhip_ctx *ctx, tdyhip_stream s, hipStream_t *out) {
   return 0;
 }
 
-static RowArgs row_args(tdyhip_ctx *ctx, const double *U, double *J) {
+static RowArgs row_args(tdyhip_ctx *ctx, const double *U, double *J, bool ell) {
   RowArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
   a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts; a.face_flags = ctx->d_face_flags;
   a.vcols = ctx->d_vcols; a.T = ctx->d_T; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd;
   a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
   a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
-  a.jrp = ctx->d_jrp; a.flux = ctx->d_flux; a.J = J;
+  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? (int64_t)ctx->nc : 1;
+  a.flux = ctx->d_flux; a.J = J;
   return a;
 }
 
@@ -126,11 +128,14 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
 }
 
 // rows + gather.  want_R / want_J select the outputs.
+// ell = true: J is the solver's slot-major ELL array (jell_w*nc), else CSR values.
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
-                       double *J, hipStream_t st) {
+                       double *J, hipStream_t st, bool ell = false) {
   const int64_t nrows = (int64_t)4 * ctx->nf;
-  RowArgs a = row_args(ctx, U, J);
-  if (J) k_fill<<<nblk(ctx->jnnz, 256), 256, 0, st>>>(ctx->jnnz, J, 0.0);
+  RowArgs a = row_args(ctx, U, J, ell);
+  const int64_t jlen = ell ? (int64_t)ctx->jell_w * ctx->nc : ctx->jnnz;
+  const int32_t *jrp = ell ? nullptr : ctx->d_jrp;
+  if (J) k_fill<<<nblk(jlen, 256), 256, 0, st>>>(jlen, J, 0.0);
   if (R && J) k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
   else if (R) k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a);
   else k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
@@ -138,15 +143,15 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
   if (R && J)
     k_cell_gather<true, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
                                                  ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                 Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+                                                 Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
   else if (R)
     k_cell_gather<true, false><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
                                                   ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                  Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+                                                  Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
   else
     k_cell_gather<false, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
                                                   ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                  Udot, ctx->d_cell_vol, ctx->d_jdiag, R, J);
+                                                  Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -207,7 +212,23 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   if ((rc = dupload(ctx, &ctx->d_jslot, topo.jslot.data(), topo.jslot.size()))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jrp, topo.jrp.data(), topo.jrp.size()))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jcols, topo.jcols.data(), topo.jcols.size()))) return fail(rc);
-  if ((rc = dupload(ctx, &ctx->d_jdiag, topo.jdiag.data(), topo.jdiag.size()))) return fail(rc);
+  {
+    std::vector<uint8_t> dslot(nc);
+    int W = 0;
+    for (int c = 0; c < nc; c++) {
+      dslot[c] = (uint8_t)(topo.jdiag[c] - topo.jrp[c]);
+      W = std::max(W, topo.jrp[c + 1] - topo.jrp[c]);
+    }
+    ctx->jell_w = W;
+    std::vector<int32_t> ecols((size_t)W * nc);
+    for (int c = 0; c < nc; c++)
+      for (int k = 0; k < W; k++) {
+        const int p = topo.jrp[c] + k;
+        ecols[(size_t)k * nc + c] = p < topo.jrp[c + 1] ? topo.jcols[p] : c;
+      }
+    if ((rc = dupload(ctx, &ctx->d_jdslot, dslot.data(), dslot.size()))) return fail(rc);
+    if ((rc = dupload(ctx, &ctx->d_ecols, ecols.data(), ecols.size()))) return fail(rc);
+  }
   if ((rc = dalloc(ctx, &ctx->d_T, (size_t)kMaxCols * nrows))) return fail(rc);
   if ((rc = dalloc(ctx, &ctx->d_G, (size_t)nrows))) return fail(rc);
   for (double **p : {&ctx->d_rho, &ctx->d_ukvr, &ctx->d_drho, &ctx->d_dukvr, &ctx->d_dM, &ctx->d_d2M, &ctx->d_acc,

@@ -110,7 +110,8 @@ struct RowArgs {
   const double *rho_bnd, *ukvr_bnd;
   const uint8_t *row_ul;
   const uint8_t *jslot;
-  const int32_t *jrp;
+  const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
+  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or n (ELL)
   double *flux;
   double *J;
 };
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
     const uint8_t ul = a.row_ul[2 * r], dl = a.row_ul[2 * r + 1];
     const uint8_t *slot = a.jslot + (int64_t)v * 64;
-    const int rup = up >= 0 ? a.jrp[up] : 0, rdn = dn >= 0 ? a.jrp[dn] : 0;
+    const int64_t rup = up >= 0 ? (a.jrp ? a.jrp[up] : up) : 0, rdn = dn >= 0 ? (a.jrp ? a.jrp[dn] : dn) : 0;
 #pragma unroll
     for (int k = 0; k < kMaxCols; k++) {
       const int col = ck[k];
@@ -191,8 +192,8 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
       else
         Jac = den * ukvr * Tk[k];
       Jac = -Jac;
-      if (up >= 0) unsafeAtomicAdd(a.J + rup + slot[ul * 8 + k], Jac);
-      if (dn >= 0) unsafeAtomicAdd(a.J + rdn + slot[dl * 8 + k], -Jac);
+      if (up >= 0) unsafeAtomicAdd(a.J + rup + (int64_t)slot[ul * 8 + k] * a.jstride, Jac);
+      if (dn >= 0) unsafeAtomicAdd(a.J + rdn + (int64_t)slot[dl * 8 + k] * a.jstride, -Jac);
     }
   }
 }
@@ -202,8 +203,9 @@ __global__ void k_cell_gather(int nc, int form, double shift, double dtime, cons
                               const int32_t *__restrict__ face_cells, const double *__restrict__ flux,
                               const double *__restrict__ acc, const double *__restrict__ dM,
                               const double *__restrict__ d2M, const double *__restrict__ Udot,
-                              const double *__restrict__ vol, const int32_t *__restrict__ jdiag,
-                              double *__restrict__ R, double *__restrict__ J) {
+                              const double *__restrict__ vol, const int32_t *__restrict__ jrp,
+                              const uint8_t *__restrict__ jdslot, int64_t jstride, double *__restrict__ R,
+                              double *__restrict__ J) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   if (WITH_R) {
@@ -219,8 +221,23 @@ __global__ void k_cell_gather(int nc, int form, double shift, double dtime, cons
   }
   if (WITH_J) {
     const double v = (form == 0) ? (shift * dM[c] + d2M[c] * Udot[c]) * vol[c] : dM[c] * vol[c] * (1.0 / dtime);
-    J[jdiag[c]] += v;
+    J[(jrp ? jrp[c] : (int64_t)c) + (int64_t)jdslot[c] * jstride] += v;
   }
+}
+
+// ELL SpMV over the slot-major layout: slot s of row i at s*n + i.
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_ell(int n, int W, const int32_t *__restrict__ ecol,
+                                                  const TV *__restrict__ a, const TV *__restrict__ x,
+                                                  TV *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TV s = 0;
+  for (int k = 0; k < W; k++) {
+    const int64_t p = (int64_t)k * n + i;
+    s += a[p] * x[ecol[p]];
+  }
+  y[i] = s;
 }
 
 __global__ void k_fill(int64_t n, double *__restrict__ x, double val) {

@@ -1,68 +1,64 @@
-// Newton's inner linear solve on the device (subsystem (d)): CSR SpMV,
-// block-Jacobi ILU(0) and GMRES(m), templated on the Krylov precision.
+// Newton's inner linear solve on the device (subsystem (d)): ELL SpMV,
+// block-Jacobi ILU(0) and the BLAS-1 pieces of GMRES(m), templated on the
+// Krylov precision.
 //
 // The reference leaves this to PETSc (KSPGMRES, restart 30, left
 // preconditioning, classical Gram-Schmidt; PCILU(0) serial / PCBJACOBI+ILU(0)
 // parallel; driven from src/tdyti.c:135,188).
 //
-// Block-Jacobi ILU(0): cells are partitioned (host, breadth-first over the
-// vertex-adjacency graph) into compact blocks of at most kBlk cells.  Inside a
-// block rows keep their global order; entries coupling different blocks are
-// dropped.  One workgroup owns one block: the factorization (IKJ) and the two
-// triangular solves advance level by level (levels of the in-block
-// dependency DAG, computed on the host) with the block's rows in LDS, so no
-// grid-wide synchronisation is ever needed.
+// Block-Jacobi ILU(0): cells are partitioned (host) into compact blocks of at
+// most BS cells (BS = 64: one wavefront per block; BS = 256: one 4-wave
+// workgroup per block).  Inside a block rows keep their global order;
+// entries coupling different blocks are dropped.  The factorization (IKJ)
+// and the two triangular solves advance level by level (levels of the
+// in-block dependency DAG, computed on the host) with the block's rows in
+// LDS, so no grid-wide synchronisation is ever needed.  The Jacobian is held
+// slot-major (slot s of row i at s*n + i) so every sweep is coalesced.
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
 
-constexpr int kBlk = 256;  // max rows per block (one workgroup)
-constexpr int kW = 27;     // max in-block entries per row (hexahedral vertex adjacency)
+constexpr int kBlkMax = 256;  // max rows per block
+constexpr int kW = 27;        // max in-block entries per row (hexahedral vertex adjacency)
+constexpr int kLU = 13;       // max strictly lower (upper) entries per row
+// Factored block layout per row (slot-major, stride BS): slots 0..12 L
+// multipliers, 13..25 U entries, 26 the inverse diagonal; padded L/U slots hold
+// 0 and point at the zero entry x[BS] of the apply kernel's LDS vector.
 
+// Per-block arrays are strided by the block width BS (64 or 256).
 struct BJLayout {
   int nblocks = 0;
-  int maxlev = 0;
-  int32_t *d_row = nullptr;    // nblocks*kBlk global row (-1 pad)
-  int32_t *d_jpos = nullptr;   // nblocks*kW*kBlk position in J values (-1 pad), slot-major
-  uint8_t *d_lcol = nullptr;   // nblocks*kW*kBlk local column (255 pad)
-  uint8_t *d_diag = nullptr;   // nblocks*kBlk slot of the diagonal
-  uint8_t *d_nslot = nullptr;  // nblocks*kBlk number of slots
-  uint16_t *d_levL = nullptr;  // nblocks*kBlk
-  uint16_t *d_levU = nullptr;  // nblocks*kBlk
+  int bs = 0;                  // stride / launch width (64 or 256)
+  int32_t *d_row = nullptr;    // nblocks*bs global row (-1 pad)
+  int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian (-1 pad), slot-major
+  uint8_t *d_lcol = nullptr;   // nblocks*kW*bs local column (255 pad)
+  uint8_t *d_diag = nullptr;   // nblocks*bs slot of the diagonal
+  uint8_t *d_nslot = nullptr;  // nblocks*bs number of slots
+  uint16_t *d_levL = nullptr;  // nblocks*bs
+  uint16_t *d_levU = nullptr;  // nblocks*bs
   int32_t *d_nlev = nullptr;   // nblocks*2 (L, U level counts)
+  uint16_t *d_cL = nullptr;    // nblocks*kLU*bs local column of L slot (bs = zero entry)
+  uint16_t *d_cU = nullptr;    // nblocks*kLU*bs local column of U slot
 };
 
-// ---------------------------------------------------------------- SpMV
-template <class TV>
-__global__ void __launch_bounds__(256) k_spmv(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ cj,
-                                              const TV *__restrict__ a, const TV *__restrict__ x, TV *__restrict__ y) {
-  // one row per 4-lane group would help long rows; 27-entry rows: one row per lane
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  TV s = 0;
-  const int e = rp[i + 1];
-  for (int k = rp[i]; k < e; k++) s += a[k] * x[cj[k]];
-  y[i] = s;
-}
-
 // ---------------------------------------------------------------- ILU(0) factor
-template <class TV>
-__global__ void __launch_bounds__(kBlk) k_bj_factor(BJLayout L, const double *__restrict__ J, TV *__restrict__ fac,
-                                                    int *__restrict__ status) {
-  __shared__ TV a[kW][kBlk];
-  __shared__ uint8_t lc[kW][kBlk];
-  __shared__ uint8_t dg[kBlk];
-  __shared__ uint8_t ns[kBlk];
+template <class TV, int BS>
+__global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__restrict__ J, TV *__restrict__ fac,
+                                                  int *__restrict__ status) {
+  __shared__ TV a[kW][BS];
+  __shared__ uint8_t lc[kW][BS];
+  __shared__ uint8_t dg[BS];
+  __shared__ uint8_t ns[BS];
   const int b = blockIdx.x, t = threadIdx.x;
-  const int64_t base = (int64_t)b * kBlk;
-  const int64_t sbase = (int64_t)b * kW * kBlk;
+  const int64_t base = (int64_t)b * BS;
+  const int64_t sbase = (int64_t)b * kW * BS;
   const int nsl = L.d_nslot[base + t];
   ns[t] = (uint8_t)nsl;
   dg[t] = L.d_diag[base + t];
   for (int p = 0; p < kW; p++) {
-    const int32_t pos = L.d_jpos[sbase + (int64_t)p * kBlk + t];
+    const int32_t pos = L.d_jpos[sbase + (int64_t)p * BS + t];
     a[p][t] = (pos >= 0) ? (TV)J[pos] : (TV)0;
-    lc[p][t] = L.d_lcol[sbase + (int64_t)p * kBlk + t];
+    lc[p][t] = L.d_lcol[sbase + (int64_t)p * BS + t];
   }
   const int lev = L.d_levL[base + t];
   const int nlev = L.d_nlev[2 * b];
@@ -88,52 +84,58 @@ __global__ void __launch_bounds__(kBlk) k_bj_factor(BJLayout L, const double *__
     }
     __syncthreads();
   }
-  for (int p = 0; p < kW; p++) fac[sbase + (int64_t)p * kBlk + t] = a[p][t];
+  // split layout: L multipliers, U entries, inverse diagonal
+  const int di = dg[t];
+  for (int p = 0; p < kLU; p++) {
+    fac[sbase + (int64_t)p * BS + t] = (p < di) ? a[p][t] : (TV)0;
+    const int q = di + 1 + p;
+    fac[sbase + (int64_t)(kLU + p) * BS + t] = (q < nsl) ? a[q][t] : (TV)0;
+  }
+  fac[sbase + (int64_t)(2 * kLU) * BS + t] = nsl > 0 ? (TV)1 / a[di][t] : (TV)0;
 }
 
 // ---------------------------------------------------------------- ILU(0) apply: y = (LU)^-1 r
-template <class TV>
-__global__ void __launch_bounds__(kBlk) k_bj_apply(BJLayout L, const TV *__restrict__ fac, const TV *__restrict__ r,
-                                                   TV *__restrict__ y) {
-  __shared__ TV x[kBlk];
+template <class TV, int BS>
+__global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restrict__ fac, const TV *__restrict__ r,
+                                                 TV *__restrict__ y, const int *__restrict__ stop = nullptr) {
+  __shared__ TV x[BS + 1];  // x[BS] == 0 is the target of padded slots
+  if (stop && *stop) return;  // uniform over the workgroup
   const int b = blockIdx.x, t = threadIdx.x;
-  const int64_t base = (int64_t)b * kBlk;
-  const int64_t sbase = (int64_t)b * kW * kBlk;
+  const int64_t base = (int64_t)b * BS;
+  const int64_t sbase = (int64_t)b * kW * BS;
+  const int64_t cbase = (int64_t)b * kLU * BS;
   const int row = L.d_row[base + t];
-  const int nsl = L.d_nslot[base + t];
-  const int di = L.d_diag[base + t];
-  TV v[kW];
-  uint8_t c[kW];
+  TV vl[kLU], vu[kLU];
+  uint16_t cl[kLU], cu[kLU];
 #pragma unroll
-  for (int p = 0; p < kW; p++) {
-    v[p] = fac[sbase + (int64_t)p * kBlk + t];
-    c[p] = L.d_lcol[sbase + (int64_t)p * kBlk + t];
+  for (int p = 0; p < kLU; p++) {
+    vl[p] = fac[sbase + (int64_t)p * BS + t];
+    vu[p] = fac[sbase + (int64_t)(kLU + p) * BS + t];
+    cl[p] = L.d_cL[cbase + (int64_t)p * BS + t];
+    cu[p] = L.d_cU[cbase + (int64_t)p * BS + t];
   }
+  const TV dinv = fac[sbase + (int64_t)(2 * kLU) * BS + t];
   x[t] = row >= 0 ? r[row] : (TV)0;
+  if (t == 0) x[BS] = (TV)0;
   const int levL = L.d_levL[base + t], levU = L.d_levU[base + t];
   const int nL = L.d_nlev[2 * b], nU = L.d_nlev[2 * b + 1];
   __syncthreads();
+  // forward: branch-free inner loop, all LDS reads of a level in flight together
   for (int l = 0; l < nL; l++) {
-    if (levL == l && row >= 0) {
+    if (levL == l) {
       TV s = x[t];
 #pragma unroll
-      for (int p = 0; p < kW; p++)
-        if (p < di) s -= v[p] * x[c[p]];
+      for (int p = 0; p < kLU; p++) s -= vl[p] * x[cl[p]];
       x[t] = s;
     }
     __syncthreads();
   }
   for (int l = 0; l < nU; l++) {
-    if (levU == l && row >= 0) {
+    if (levU == l) {
       TV s = x[t];
 #pragma unroll
-      for (int p = 0; p < kW; p++)
-        if (p > di && p < nsl) s -= v[p] * x[c[p]];
-      TV dv = v[0];
-#pragma unroll
-      for (int p = 0; p < kW; p++)
-        if (p == di) dv = v[p];
-      x[t] = s / dv;
+      for (int p = 0; p < kLU; p++) s -= vu[p] * x[cu[p]];
+      x[t] = s * dinv;
     }
     __syncthreads();
   }
@@ -161,8 +163,10 @@ __device__ inline TV block_sum(TV v, TV *sh) {
 // partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1
 template <class TV>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
-                                                    const TV *__restrict__ w, TV *__restrict__ partial, int nchunk) {
+                                                    const TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
+                                                    const int *__restrict__ stop = nullptr) {
   __shared__ TV sh[kRedBlock / 64];
+  if (stop && *stop) return;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
   TV wv[kRedPerThread];
 #pragma unroll
@@ -184,8 +188,10 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
 
 // out[j] = sum_c partial[j*nchunk + c] (fixed order)
 template <class TV>
-__global__ void k_reduce_partials(int nv, int nchunk, const TV *__restrict__ partial, double *__restrict__ out) {
+__global__ void k_reduce_partials(int nv, int nchunk, const TV *__restrict__ partial, double *__restrict__ out,
+                                  const int *__restrict__ stop = nullptr) {
   __shared__ TV sh[kRedBlock / 64];
+  if (stop && *stop) return;
   const int j = blockIdx.x;
   TV s = 0;
   for (int c = threadIdx.x; c < nchunk; c += kRedBlock) s += partial[(int64_t)j * nchunk + c];
@@ -197,8 +203,10 @@ __global__ void k_reduce_partials(int nv, int nchunk, const TV *__restrict__ par
 template <class TV>
 __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                           const double *__restrict__ h, TV *__restrict__ w,
-                                                          TV *__restrict__ partial) {
+                                                          TV *__restrict__ partial,
+                                                          const int *__restrict__ stop = nullptr) {
   __shared__ TV sh[kRedBlock / 64];
+  if (stop && *stop) return;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
   TV s = 0;
 #pragma unroll
@@ -259,6 +267,123 @@ template <class TV>
 __global__ void k_sub(int64_t n, const TV *__restrict__ a, const TV *__restrict__ b, TV *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] - b[i];
+}
+
+// ---------------------------------------------------------------- device-resident GMRES cycle
+// State shared by the kernels of one restart cycle; `done` makes every later
+// kernel of the cycle return at once, so the host launches a whole cycle and
+// synchronises once.
+struct GmresDev {
+  double tol;      // stopping tolerance on the preconditioned residual
+  double inv_h;    // 1/h(j+1,j) or 1/beta for the next basis vector
+  double resid;    // current residual estimate |g(j+1)|
+  int done;        // cycle finished (converged, breakdown, or iteration cap)
+  int converged;
+  int its;         // total Arnoldi steps
+  int jcur;        // basis vectors completed in this cycle
+  int maxit;
+  int pad;
+};
+
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_ell_stop(int n, int W, const int32_t *__restrict__ ecol,
+                                                       const TV *__restrict__ a, const TV *__restrict__ x,
+                                                       TV *__restrict__ y, const GmresDev *__restrict__ gd) {
+  if (gd->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TV s = 0;
+  for (int k = 0; k < W; k++) {
+    const int64_t p = (int64_t)k * n + i;
+    s += a[p] * x[ecol[p]];
+  }
+  y[i] = s;
+}
+
+// y = s * x with s read on the device (1/beta or 1/h(j+1,j))
+template <class TV>
+__global__ void k_scale_dev(int64_t n, const TV *__restrict__ x, TV *__restrict__ y, const GmresDev *__restrict__ gd) {
+  if (gd->done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (TV)gd->inv_h * x[i];
+}
+
+__global__ void k_gmres_init(GmresDev *gd, int *status, int maxit) {
+  gd->its = 0;
+  gd->done = 0;
+  gd->converged = 0;
+  gd->jcur = 0;
+  gd->maxit = maxit;
+  *status = 0;
+}
+
+// Start of a cycle: beta = sqrt(h[0]) from a norm reduction; sets tol on the first cycle.
+__global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double *__restrict__ g, int m, double rtol,
+                              double atol, int first) {
+  const double beta = sqrt(h[0]);
+  if (first) gd->tol = fmax(rtol * beta, atol);
+  for (int i = 0; i <= m; i++) g[i] = 0.0;
+  g[0] = beta;
+  gd->resid = beta;
+  gd->jcur = 0;
+  gd->inv_h = beta > 0.0 ? 1.0 / beta : 0.0;
+  const int conv = !(beta > gd->tol);
+  gd->converged = conv;
+  gd->done = conv || gd->its >= gd->maxit;
+}
+
+// Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
+// previous Givens rotations, the new rotation, residual estimate and stop test.
+__global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
+                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
+  if (gd->done) return;
+  const int ld = m + 1;
+  const double hn = sqrt(h[j + 1]);
+  for (int i = 0; i <= j; i++) H[i + j * ld] = h[i];
+  H[j + 1 + j * ld] = hn;
+  for (int i = 0; i < j; i++) {
+    const double a = H[i + j * ld], c = H[i + 1 + j * ld];
+    H[i + j * ld] = cs[i] * a + sn[i] * c;
+    H[i + 1 + j * ld] = -sn[i] * a + cs[i] * c;
+  }
+  const double a = H[j + j * ld], c = H[j + 1 + j * ld];
+  const double d = sqrt(a * a + c * c);
+  cs[j] = d > 0 ? a / d : 1.0;
+  sn[j] = d > 0 ? c / d : 0.0;
+  H[j + j * ld] = d;
+  H[j + 1 + j * ld] = 0.0;
+  g[j + 1] = -sn[j] * g[j];
+  g[j] = cs[j] * g[j];
+  gd->its++;
+  gd->jcur = j + 1;
+  gd->resid = fabs(g[j + 1]);
+  gd->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
+  const int conv = gd->resid <= gd->tol || hn == 0.0;
+  gd->converged = conv;
+  if (conv || gd->its >= gd->maxit || j + 1 == m) gd->done = 1;
+}
+
+// y = H(1:j,1:j)^-1 g(1:j) for the j = jcur columns of the cycle
+__global__ void k_hess_solve(const GmresDev *gd, int m, const double *__restrict__ H, const double *__restrict__ g,
+                             double *__restrict__ y) {
+  const int ld = m + 1, j = gd->jcur;
+  for (int i = j - 1; i >= 0; i--) {
+    double s = g[i];
+    for (int k = i + 1; k < j; k++) s -= H[i + k * ld] * y[k];
+    y[i] = s / H[i + i * ld];
+  }
+}
+
+// x += sum_{k<jcur} y[k] V_k
+template <class TV, class TX>
+__global__ void k_gemv_update_dev(int64_t n, int mmax, const TV *__restrict__ V, int64_t ldv,
+                                  const double *__restrict__ c, const GmresDev *__restrict__ gd, TX *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nv = gd->jcur < mmax ? gd->jcur : mmax;
+  TV s = 0;
+  for (int j = 0; j < nv; j++) s += (TV)c[j] * V[j * ldv + i];
+  x[i] += (TX)s;
 }
 
 // w = x - lambda*y

@@ -13,10 +13,10 @@
 namespace tdyhip {
 
 struct HostBJ {
-  int nblocks = 0;
+  int nblocks = 0, bs = 0;
   std::vector<int32_t> row, jpos;
   std::vector<uint8_t> lcol, diag, nslot;
-  std::vector<uint16_t> levL, levU;
+  std::vector<uint16_t> levL, levU, cL, cU;
   std::vector<int32_t> nlev;
 };
 
@@ -24,13 +24,31 @@ struct HostBJ {
 // the Jacobian's vertex-adjacency graph, then the in-block ILU(0) slots and
 // level schedules.
 static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols, int nc, int bs,
-                            HostBJ &out) {
-  if (bs < 1 || bs > kBlk) return "block size must be in [1,256]";
+                            const std::vector<int32_t> &part, HostBJ &out) {
+  if (bs < 1 || bs > kBlkMax) return "block size must be in [1,256]";
+  const int S = bs <= 64 ? 64 : kBlkMax;  // per-block stride = launch width
+  out.bs = S;
   std::vector<int32_t> blk(nc, -1), local(nc, -1);
   std::vector<std::vector<int32_t>> members;
   std::vector<int32_t> queue;
   queue.reserve(4096);
   int seed = 0;
+  if (!part.empty()) {
+    int nbk = 0;
+    for (int c = 0; c < nc; c++) {
+      if (part[c] < 0) return "negative block id in the preconditioner partition";
+      nbk = std::max(nbk, part[c] + 1);
+    }
+    members.assign(nbk, {});
+    for (int c = 0; c < nc; c++) { blk[c] = part[c]; members[part[c]].push_back(c); }
+    for (auto &mem : members)
+      if ((int)mem.size() > bs) return "preconditioner block larger than pc_block";
+    members.erase(std::remove_if(members.begin(), members.end(), [](const std::vector<int32_t> &m) { return m.empty(); }),
+                  members.end());
+    for (size_t b = 0; b < members.size(); b++)
+      for (int c : members[b]) blk[c] = (int32_t)b;
+    seed = nc;
+  }
   while (true) {
     while (seed < nc && blk[seed] >= 0) seed++;
     if (seed >= nc) break;
@@ -60,13 +78,15 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   }
   const int nb = (int)members.size();
   out.nblocks = nb;
-  out.row.assign((size_t)nb * kBlk, -1);
-  out.jpos.assign((size_t)nb * kW * kBlk, -1);
-  out.lcol.assign((size_t)nb * kW * kBlk, 255);
-  out.diag.assign((size_t)nb * kBlk, 0);
-  out.nslot.assign((size_t)nb * kBlk, 0);
-  out.levL.assign((size_t)nb * kBlk, 0);
-  out.levU.assign((size_t)nb * kBlk, 0);
+  out.row.assign((size_t)nb * S, -1);
+  out.jpos.assign((size_t)nb * kW * S, -1);
+  out.lcol.assign((size_t)nb * kW * S, 255);
+  out.diag.assign((size_t)nb * S, 0);
+  out.nslot.assign((size_t)nb * S, 0);
+  out.levL.assign((size_t)nb * S, 0);
+  out.levU.assign((size_t)nb * S, 0);
+  out.cL.assign((size_t)nb * kLU * S, (uint16_t)S);
+  out.cU.assign((size_t)nb * kLU * S, (uint16_t)S);
   out.nlev.assign((size_t)nb * 2, 0);
   for (int b = 0; b < nb; b++) {
     const auto &mem = members[b];
@@ -75,49 +95,62 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     int maxL = 0, maxU = 0;
     for (int i = 0; i < m; i++) {
       const int r = mem[i];
-      out.row[(size_t)b * kBlk + i] = r;
+      out.row[(size_t)b * S + i] = r;
       int ns = 0;
       for (int k = jrp[r]; k < jrp[r + 1]; k++) {
         const int c = jcols[k];
         if (blk[c] != b) continue;
         if (ns >= kW) return "more than 27 in-block entries in a row";
-        out.jpos[(size_t)b * kW * kBlk + (size_t)ns * kBlk + i] = k;
-        out.lcol[(size_t)b * kW * kBlk + (size_t)ns * kBlk + i] = (uint8_t)local[c];
-        if (c == r) out.diag[(size_t)b * kBlk + i] = (uint8_t)ns;
+        // position of (r, slot k-jrp[r]) in the slot-major ELL Jacobian
+        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (k - jrp[r]) * nc + r;
+        out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint8_t)local[c];
+        if (c == r) out.diag[(size_t)b * S + i] = (uint8_t)ns;
         ns++;
       }
-      out.nslot[(size_t)b * kBlk + i] = (uint8_t)ns;
+      out.nslot[(size_t)b * S + i] = (uint8_t)ns;
     }
     // levels: L forward, U backward
     for (int i = 0; i < m; i++) {
       int lv = 0;
-      const int di = out.diag[(size_t)b * kBlk + i];
+      const int di = out.diag[(size_t)b * S + i];
       for (int p = 0; p < di; p++) {
-        const int k = out.lcol[(size_t)b * kW * kBlk + (size_t)p * kBlk + i];
-        lv = std::max(lv, (int)out.levL[(size_t)b * kBlk + k] + 1);
+        const int k = out.lcol[(size_t)b * kW * S + (size_t)p * S + i];
+        lv = std::max(lv, (int)out.levL[(size_t)b * S + k] + 1);
       }
-      out.levL[(size_t)b * kBlk + i] = (uint16_t)lv;
+      out.levL[(size_t)b * S + i] = (uint16_t)lv;
       maxL = std::max(maxL, lv + 1);
     }
     for (int i = m - 1; i >= 0; i--) {
       int lv = 0;
-      const int di = out.diag[(size_t)b * kBlk + i], ns = out.nslot[(size_t)b * kBlk + i];
+      const int di = out.diag[(size_t)b * S + i], ns = out.nslot[(size_t)b * S + i];
       for (int p = di + 1; p < ns; p++) {
-        const int k = out.lcol[(size_t)b * kW * kBlk + (size_t)p * kBlk + i];
-        lv = std::max(lv, (int)out.levU[(size_t)b * kBlk + k] + 1);
+        const int k = out.lcol[(size_t)b * kW * S + (size_t)p * S + i];
+        lv = std::max(lv, (int)out.levU[(size_t)b * S + k] + 1);
       }
-      out.levU[(size_t)b * kBlk + i] = (uint16_t)lv;
+      out.levU[(size_t)b * S + i] = (uint16_t)lv;
       maxU = std::max(maxU, lv + 1);
     }
     out.nlev[2 * b] = maxL;
     out.nlev[2 * b + 1] = maxU;
+    // split L / U column tables (padding -> the zero entry S)
+    for (int i = 0; i < m; i++) {
+      const int di = out.diag[(size_t)b * S + i], ns = out.nslot[(size_t)b * S + i];
+      if (di > kLU || ns - di - 1 > kLU) return "more than 13 lower or upper in-block entries in a row";
+      for (int p = 0; p < kLU; p++) {
+        out.cL[(size_t)b * kLU * S + (size_t)p * S + i] =
+            p < di ? out.lcol[(size_t)b * kW * S + (size_t)p * S + i] : (uint16_t)S;
+        const int q = di + 1 + p;
+        out.cU[(size_t)b * kLU * S + (size_t)p * S + i] =
+            q < ns ? out.lcol[(size_t)b * kW * S + (size_t)q * S + i] : (uint16_t)S;
+      }
+    }
     for (int i = 0; i < m; i++) local[mem[i]] = -1;
   }
   return std::string();
 }
 
 struct SolverWS {
-  int n = 0, restart = 0, fp32 = -1, bs = 0;
+  int n = 0, restart = 0, fp32 = -1, bs = 0, part_version = -1;
   int64_t nnz = 0;
   BJLayout bj;
   std::vector<void *> allocs;
@@ -128,6 +161,8 @@ struct SolverWS {
   void *V = nullptr, *w = nullptr, *z = nullptr, *b = nullptr, *x = nullptr, *Jk = nullptr, *fac = nullptr,
        *partial = nullptr;
   double *dpartial = nullptr, *h = nullptr;
+  double *hs = nullptr;  // Hessenberg, Givens cs/sn, g, y of the device GMRES cycle
+  GmresDev *gdev = nullptr;
   int *status = nullptr;
   int nchunk = 0;
 };
@@ -154,13 +189,15 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if (!ctx->ws) ctx->ws = new SolverWS();
   SolverWS &ws = *ctx->ws;
   const int fp32 = o.krylov_fp32 ? 1 : 0;
-  if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.fp32 == fp32 && ws.bs == o.pc_block) return 0;
+  if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.fp32 == fp32 && ws.bs == o.pc_block &&
+      ws.part_version == ctx->pc_part_version)
+    return 0;
   ws_free(ws);
   HostBJ hb;
-  std::string msg = build_bj(ctx->h_jrp, ctx->h_jcols, ctx->nc, o.pc_block, hb);
+  std::string msg = build_bj(ctx->h_jrp, ctx->h_jcols, ctx->nc, o.pc_block, ctx->pc_part, hb);
   if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
   const int n = ctx->nc;
-  const int64_t nnz = ctx->jnnz;
+  const int64_t nnz = (int64_t)ctx->jell_w * n;  // slot-major ELL Jacobian
   ws.n = n; ws.restart = o.ksp_restart; ws.fp32 = fp32; ws.bs = o.pc_block; ws.nnz = nnz;
   ws.nchunk = (int)((n + kChunk - 1) / kChunk);
   int rc = 0;
@@ -175,9 +212,12 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
     return 0;
   };
   ws.bj.nblocks = hb.nblocks;
+  ws.bj.bs = hb.bs;
+  ws.part_version = ctx->pc_part_version;
   if ((rc = up(&ws.bj.d_row, hb.row)) || (rc = up(&ws.bj.d_jpos, hb.jpos)) || (rc = up(&ws.bj.d_lcol, hb.lcol)) ||
       (rc = up(&ws.bj.d_diag, hb.diag)) || (rc = up(&ws.bj.d_nslot, hb.nslot)) || (rc = up(&ws.bj.d_levL, hb.levL)) ||
-      (rc = up(&ws.bj.d_levU, hb.levU)) || (rc = up(&ws.bj.d_nlev, hb.nlev)))
+      (rc = up(&ws.bj.d_levU, hb.levU)) || (rc = up(&ws.bj.d_nlev, hb.nlev)) || (rc = up(&ws.bj.d_cL, hb.cL)) ||
+      (rc = up(&ws.bj.d_cU, hb.cU)))
     return rc;
   for (double **p : {&ws.F, &ws.Y, &ws.W, &ws.G, &ws.JY, &ws.Udot, &ws.Uprev_ts, &ws.Usave})
     if ((rc = ws_alloc(ws, p, n))) return rc;
@@ -185,7 +225,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   char **cp[] = {(char **)&ws.V, (char **)&ws.w, (char **)&ws.z, (char **)&ws.b, (char **)&ws.x, (char **)&ws.fac,
                  (char **)&ws.partial};
   size_t cnt[] = {(size_t)(o.ksp_restart + 1) * n, (size_t)n, (size_t)n, (size_t)n, (size_t)n,
-                  (size_t)hb.nblocks * kW * kBlk, (size_t)(o.ksp_restart + 2) * ws.nchunk};
+                  (size_t)hb.nblocks * kW * hb.bs, (size_t)(o.ksp_restart + 2) * ws.nchunk};
   for (int i = 0; i < 7; i++)
     if ((rc = ws_alloc(ws, cp[i], cnt[i] * tv))) return rc;
   if (fp32) {
@@ -195,6 +235,8 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   }
   if ((rc = ws_alloc(ws, &ws.dpartial, ws.nchunk))) return rc;
   if ((rc = ws_alloc(ws, &ws.h, o.ksp_restart + 2))) return rc;
+  if ((rc = ws_alloc(ws, &ws.hs, (size_t)(o.ksp_restart + 1) * o.ksp_restart + 4 * (o.ksp_restart + 1)))) return rc;
+  if ((rc = ws_alloc(ws, &ws.gdev, 1))) return rc;
   if ((rc = ws_alloc(ws, &ws.status, 1))) return rc;
   return 0;
 }
@@ -224,7 +266,10 @@ static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t s
   return rc;
 }
 
-// Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with GMRES.
+// Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
+// GMRES(m), left preconditioned, classical Gram-Schmidt.  Each restart cycle is
+// launched as one batch of kernels; Givens rotations and the stopping test run
+// on the device, and the host synchronises once per cycle.
 template <class TV>
 static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec, int *its_out,
                  hipStream_t st) {
@@ -233,96 +278,63 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   const int64_t N = n;
   TV *V = (TV *)ws.V, *w = (TV *)ws.w, *z = (TV *)ws.z, *b = (TV *)ws.b, *x = (TV *)ws.x, *fac = (TV *)ws.fac;
   TV *Jk = (TV *)ws.Jk, *part = (TV *)ws.partial;
+  GmresDev *gd = ws.gdev;
+  double *dH = ws.hs, *dcs = dH + (size_t)(m + 1) * m, *dsn = dcs + m, *dg = dsn + m, *dy = dg + (m + 1);
+  const int *stop = &gd->done;
   const unsigned g256 = (unsigned)((N + 255) / 256);
-  if (!std::is_same<TV, double>::value) k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
-  SCHK(hipMemsetAsync(ws.status, 0, sizeof(int), st));
-  k_bj_factor<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, ws.J, fac, ws.status);
-  k_convert<double, TV><<<g256, 256, 0, st>>>(N, bvec, b);
-  k_fill<<<g256, 256, 0, st>>>(N, yvec, 0.0);
-  SCHK(hipGetLastError());
-  int status = 0;
-  SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
-
-  auto apply_op = [&](const TV *in, TV *out) {  // out = M^-1 A in
-    k_spmv<TV><<<g256, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, Jk, in, z);
-    k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, z, out);
+  if (!std::is_same<TV, double>::value)
+    k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
+  k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
+  if (ws.bj.bs == 64) k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status);
+  else k_bj_factor<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, ws.J, fac, ws.status);
+  auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
+    if (ws.bj.bs == 64) k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp);
+    else k_bj_apply<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, fac, in, out, stp);
   };
-  auto norm_of = [&](const TV *v, double *out) -> int {
+  auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
     k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
-    SCHK(hipMemcpyAsync(out, ws.h, sizeof(double), hipMemcpyDeviceToHost, st));
-    SCHK(hipStreamSynchronize(st));
-    *out = std::sqrt(*out);
-    return 0;
   };
+  k_convert<double, TV><<<g256, 256, 0, st>>>(N, bvec, b);
+  if (std::is_same<TV, double>::value) x = (TV *)yvec;  // x accumulates the solution in place
+  k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)0, b, x);
   // r0 = M^-1 b  (x0 = 0)
-  k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, b, w);
-  double beta;
-  if (norm_of(w, &beta)) return TDYHIP_ERR_HIP;
-  if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
-  const double tol = std::max(o.ksp_rtol * beta, o.ksp_atol);
-  int its = 0;
-  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hcol(m + 2), yk(m);
-  bool conv = !(beta > tol);
-  // x (TV) accumulates the solution
-  if (std::is_same<TV, double>::value) {
-    x = (TV *)yvec;
-  } else {
-    k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)0, b, x);
-  }
-  while (!conv && its < o.ksp_max_it) {
-    k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)(1.0 / beta), w, V);
-    std::fill(gv.begin(), gv.end(), 0.0);
-    gv[0] = beta;
-    int j = 0;
-    for (; j < m; j++) {
-      TV *vj = V + (int64_t)j * N;
-      apply_op(vj, w);
-      k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk);
-      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h);
-      k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part);
-      k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1);
-      SCHK(hipMemcpyAsync(hcol.data(), ws.h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
-      SCHK(hipStreamSynchronize(st));
-      const double hn = std::sqrt(hcol[j + 1]);
-      for (int i = 0; i <= j; i++) H[i + (size_t)j * (m + 1)] = hcol[i];
-      H[j + 1 + (size_t)j * (m + 1)] = hn;
-      if (hn > 0) k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)(1.0 / hn), w, V + (int64_t)(j + 1) * N);
-      for (int i = 0; i < j; i++) {
-        const double a = H[i + (size_t)j * (m + 1)], c = H[i + 1 + (size_t)j * (m + 1)];
-        H[i + (size_t)j * (m + 1)] = cs[i] * a + sn[i] * c;
-        H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * a + cs[i] * c;
-      }
-      const double a = H[j + (size_t)j * (m + 1)], c = H[j + 1 + (size_t)j * (m + 1)];
-      const double d = std::sqrt(a * a + c * c);
-      cs[j] = d > 0 ? a / d : 1.0;
-      sn[j] = d > 0 ? c / d : 0.0;
-      H[j + (size_t)j * (m + 1)] = d;
-      H[j + 1 + (size_t)j * (m + 1)] = 0.0;
-      gv[j + 1] = -sn[j] * gv[j];
-      gv[j] = cs[j] * gv[j];
-      its++;
-      if (std::fabs(gv[j + 1]) <= tol || its >= o.ksp_max_it || hn == 0.0) { j++; conv = std::fabs(gv[j]) <= tol || hn == 0.0; break; }
+  pc_apply(b, w, nullptr);
+  norm2_to_h0(w);
+  k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 1);
+  SCHK(hipGetLastError());
+  GmresDev hg;
+  int status = 0;
+  for (int cycle = 0;; cycle++) {
+    k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V, gd);
+    for (int j = 0; j < m; j++) {
+      k_spmv_ell_stop<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, V + (int64_t)j * N, z, gd);
+      pc_apply(z, w, stop);
+      k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk, stop);
+      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
+      k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part, stop);
+      k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1, stop);
+      k_arnoldi_step<<<1, 1, 0, st>>>(gd, j, m, ws.h, dH, dcs, dsn, dg);
+      k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V + (int64_t)(j + 1) * N, gd);
     }
-    if (j > m) j = m;
-    for (int i = j - 1; i >= 0; i--) {
-      double s = gv[i];
-      for (int k = i + 1; k < j; k++) s -= H[i + (size_t)k * (m + 1)] * yk[k];
-      yk[i] = s / H[i + (size_t)i * (m + 1)];
-    }
-    SCHK(hipMemcpyAsync(ws.h, yk.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
-    k_gemv_update<TV, TV><<<g256, 256, 0, st>>>(N, j, V, N, ws.h, x);
-    if (conv || its >= o.ksp_max_it) break;
+    k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
+    k_gemv_update_dev<TV, TV><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, x);
+    SCHK(hipGetLastError());
+    SCHK(hipMemcpyAsync(&hg, gd, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+    if (cycle == 0) SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
+    if (hg.converged || hg.its >= o.ksp_max_it) break;
     // restart: w = M^-1 (b - A x)
-    k_spmv<TV><<<g256, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, Jk, x, z);
+    k_spmv_ell<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, x, z);
     k_sub<TV><<<g256, 256, 0, st>>>(N, b, z, z);
-    k_bj_apply<TV><<<ws.bj.nblocks, kBlk, 0, st>>>(ws.bj, fac, z, w);
-    if (norm_of(w, &beta)) return TDYHIP_ERR_HIP;
-    conv = !(beta > tol);
+    pc_apply(z, w, nullptr);
+    norm2_to_h0(w);
+    k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 0);
   }
   if (!std::is_same<TV, double>::value) k_convert<TV, double><<<g256, 256, 0, st>>>(N, x, yvec);
   SCHK(hipGetLastError());
-  *its_out = its;
+  *its_out = hg.its;
   return 0;
 }
 
@@ -359,9 +371,9 @@ static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, h
   SolverWS &ws = *ctx->ws;
   if (fm.type == 0) {
     k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
-    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st);
+    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st, true);
   }
-  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st);
+  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true);
 }
 
 static int linsolve(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *b, double *y, int *its,
@@ -400,7 +412,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
       k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
       ynorm = maxstep;
     }
-    k_spmv<double><<<g, 256, 0, st>>>(n, ctx->d_jrp, ctx->d_jcols, ws.J, ws.Y, ws.JY);
+    k_spmv_ell<double><<<g, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, ws.Y, ws.JY);
     double initslope;
     if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
     if (initslope > 0.0) initslope = -initslope;
@@ -474,11 +486,25 @@ void tdyhip_solver_defaults(tdyhip_solver_opts *o) {
 
 static int check_opts(const tdyhip_solver_opts *o) {
   if (!o) { tdyhip::set_error("null solver options"); return TDYHIP_ERR_ARG; }
-  if (o->ksp_restart < 1 || o->ksp_restart > 200 || o->pc_block < 1 || o->pc_block > tdyhip::kBlk ||
+  if (o->ksp_restart < 1 || o->ksp_restart > 200 || o->pc_block < 1 || o->pc_block > tdyhip::kBlkMax ||
       o->snes_max_it < 0 || o->ksp_max_it < 1) {
     tdyhip::set_error("bad solver options");
     return TDYHIP_ERR_ARG;
   }
+  return 0;
+}
+
+int tdyhip_set_pc_partition(tdyhip_ctx *ctx, const int32_t *block_of_cell) {
+  using namespace tdyhip;
+  if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
+  if (block_of_cell) {
+    for (int c = 0; c < ctx->nc; c++)
+      if (block_of_cell[c] < 0 || block_of_cell[c] >= ctx->nc) { set_error("block id out of range"); return TDYHIP_ERR_ARG; }
+    ctx->pc_part.assign(block_of_cell, block_of_cell + ctx->nc);
+  } else {
+    ctx->pc_part.clear();
+  }
+  ctx->pc_part_version++;
   return 0;
 }
 

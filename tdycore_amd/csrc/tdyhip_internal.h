@@ -85,7 +85,10 @@ struct tdyhip_ctx {
   double *d_G = nullptr;            // 4nf gravity term
   uint8_t *d_row_ul = nullptr;      // 4nf*2
   uint8_t *d_jslot = nullptr;       // nv*64
-  int32_t *d_jrp = nullptr, *d_jcols = nullptr, *d_jdiag = nullptr;
+  int32_t *d_jrp = nullptr, *d_jcols = nullptr;
+  uint8_t *d_jdslot = nullptr;      // nc: slot of the diagonal within its row
+  int jell_w = 0;                   // ELL width (longest Jacobian row)
+  int32_t *d_ecols = nullptr;       // jell_w*nc slot-major ELL columns (padding = own row)
 
   // device: state
   double *d_rho = nullptr, *d_ukvr = nullptr, *d_drho = nullptr, *d_dukvr = nullptr;
@@ -101,4 +104,6 @@ struct tdyhip_ctx {
   std::vector<uint8_t> h_face_flags;
   std::vector<void *> allocations;
   tdyhip::SolverWS *ws = nullptr;  // Newton/Krylov workspace, allocated on first solve
+  std::vector<int32_t> pc_part;    // caller-given block-Jacobi partition (empty: breadth-first)
+  int pc_part_version = 0;
 };

@@ -319,3 +319,11 @@ def boundary_values(mesh: Mesh, fn) -> np.ndarray:
     (``ConditionsComputeBoundaryPressure`` at ``faces->centroid``,
     src/fv/mpfao/tdympfao_utils.c:745-750).  Returns (nbnd,)."""
     return np.asarray(fn(mesh.face_cen[mesh.bnd_face]), dtype=np.float64)
+
+
+def brick_partition(mesh: Mesh, bx: int, by: int, bz: int) -> np.ndarray:
+    """Block id per cell for bx*by*bz bricks of a box mesh (block-Jacobi blocks)."""
+    c = np.arange(mesh.nc)
+    i, j, k = c % mesh.nx, (c // mesh.nx) % mesh.ny, c // (mesh.nx * mesh.ny)
+    gx, gy = -(-mesh.nx // bx), -(-mesh.ny // by)
+    return ((i // bx) + gx * ((j // by) + gy * (k // bz))).astype(np.int32)

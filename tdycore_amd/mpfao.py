@@ -140,6 +140,16 @@ class MPFAO:
         _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
 
     # ---- Newton / time loops on the device ----
+    def set_pc_partition(self, block_of_cell=None):
+        """Block-Jacobi blocks (block id per cell, each block <= pc_block cells);
+        None restores the breadth-first default."""
+        L = _lib.load()
+        if block_of_cell is None:
+            _lib.check(L.tdyhip_set_pc_partition(self._h, None))
+        else:
+            a = np.ascontiguousarray(block_of_cell, dtype=np.int32)
+            _lib.check(L.tdyhip_set_pc_partition(self._h, a.ctypes.data))
+
     @staticmethod
     def solver_options(**kw):
         """PETSc-default solver options (SNES rtol 1e-8, GMRES(30), KSP rtol 1e-5,

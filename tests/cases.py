@@ -49,3 +49,22 @@ def cfg0_problem(n=16, seed=20261202, jitter=0.1):
     U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], 0.5)
     kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
     return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+
+
+def cfg1_problem(nx=128, ny=128, nz=64, seed=20261203, lx=1000.0, ly=1000.0, lz=100.0):
+    """Plain method at scale: nx*ny*nz hex on lx*ly*lz m, layered log10 k ~ N(-12,1)
+    smoothed by a 5x5 x-y moving average per layer (rescaled back to sigma 1),
+    K = diag(k, k, k/10); phi ~ U(0.2,0.4), Sr 0.1, VG n=1.5 (m=1/3), alpha 1e-4;
+    hydrostatic IC with the water table at mid-height; Dirichlet top/bottom at
+    hydrostatic values, no-flow sides."""
+    rng = np.random.default_rng(seed)
+    m = mg.box_mesh(nx, ny, nz, upper=(lx, ly, lz))
+    K = mg.layered_K(nx, ny, nz, rng)
+    phi = rng.uniform(0.2, 0.4, size=m.nc)
+    Sr = np.full(m.nc, 0.1)
+    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-", "x+", "y-", "y+"])
+    zt = 0.5 * lz
+    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], zt))
+    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
+    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
