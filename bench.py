@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""MPFA-O Richards Newton throughput on one MI355X.
+
+Workload: the 128x128x64 layered, jittered hex box (1,048,576 cells,
+1000x1000x100 m), backward-Euler steps of dt = 1 day driven by the device
+Newton solver (SNESNEWTONLS + backtracking line search, snes_rtol 1e-10) with
+GMRES(30) + block-Jacobi ILU(0) (8x8x4 cell bricks) as the inner solve.
+
+A step is one backward-Euler time step: SNES pre-solve (accumulation of the
+previous state) and the Newton solve to convergence, every residual,
+Jacobian, factorisation and Krylov iteration included.  W warmup steps run
+from the initial state, the state is reset, then exactly K steps are timed.
+
+Printed value: Newton cells/s = cells x Newton iterations / wall time of the
+K timed steps (FP64 residual, Jacobian, Krylov).  The JSON line also carries:
+  roofline      the kernel with the largest share of device time in the timed
+                region: algorithmic bytes per launch / mean launch duration
+                (HIP events on the library's stream) against 8 TB/s;
+  kernels       the same figures for every kernel of the path;
+  fp32_krylov   the same K steps with FP32 Jacobian storage and FP32 GMRES /
+                ILU(0) (FP64 residual and Newton update), its speedup and its
+                max relative pressure difference from the FP64 run;
+  cpu_baseline  the oracle (single-threaded C restatement of the reference
+                algorithm with serial GMRES(30)+ILU(0)) timed on this host on
+                one Newton iteration of the first step of the same mesh.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed backward-Euler steps (K)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed steps before the timed region (W)")
+    ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 64], metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--dt", type=float, default=86400.0)
+    ap.add_argument("--brick", type=int, nargs=3, default=[8, 8, 4], help="block-Jacobi brick (cells)")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-Krylov variant")
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
+    ap.add_argument("--no-timers", action="store_true", help="time without per-kernel HIP events")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_table(rep):
+    out = []
+    for r in rep:
+        if r["launches"] == 0 or r["ms"] <= 0:
+            continue
+        avg_ms = r["ms"] / r["launches"]
+        avg_b = r["bytes"] / r["launches"]
+        out.append({"kernel": r["kernel"], "launches": r["launches"], "total_ms": round(r["ms"], 3),
+                    "avg_us": round(1e3 * avg_ms, 2), "bytes_per_launch": avg_b,
+                    "GB_s": round(avg_b / (avg_ms * 1e-3) / 1e9, 1)})
+    out.sort(key=lambda r: -r["total_ms"])
+    return out
+
+
+def run_steps(g, U, U0, dt, K, W, opts, timers):
+    import torch
+    U.copy_(U0)
+    if W > 0:
+        g.run_snes_steps(U, dt, W, opts)
+    U.copy_(U0)
+    torch.cuda.synchronize()
+    if timers:
+        g.kernel_timers(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = g.run_snes_steps(U, dt, K, opts)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rep = None
+    if timers:
+        rep = g.kernel_timer_report()
+        g.kernel_timers(False)
+    return el, st, rep
+
+
+def cpu_baseline(P, dt):
+    """One Newton iteration of the first step on the full mesh with the oracle."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    from pyoracle import Oracle
+    t = time.perf_counter()
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    log(f"cpu oracle setup {time.perf_counter() - t:.1f} s")
+    U = P["U0"].copy()
+    o.set_dtime(dt)
+    o.snes_presolve(U)
+    o.set_newton_maxit(1)
+    t = time.perf_counter()
+    _, st, _ = o.newton_snes(U, lin="gmres", rtol=1e-10)
+    el = time.perf_counter() - t
+    nc = P["mesh"].nc
+    return {"value": nc * int(st[0]) / el, "unit": "Newton cells/s", "cores": 1, "kind": "port",
+            "sample": f"1 Newton iteration (residual, Jacobian, ILU(0), GMRES(30) {int(st[1])} its, "
+                      f"line search) of step 1 on the same {nc}-cell mesh; {el:.2f} s"}
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE {world}); "
+              "the path is measured on one GPU's full mesh", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from tdycore_amd import workloads
+    from tdycore_amd.meshgen import brick_partition
+    from tdycore_amd.mpfao import MPFAO
+
+    nx, ny, nz = a.mesh
+    t = time.perf_counter()
+    P = workloads.layered_box_problem(nx, ny, nz)
+    nc = P["mesh"].nc
+    log(f"problem {nx}x{ny}x{nz} ({nc} cells) built in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    log(f"device setup {setup_s:.2f} s")
+    g.set_pc_partition(brick_partition(P["mesh"], *a.brick))
+    bs = a.brick[0] * a.brick[1] * a.brick[2]
+    U0 = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
+    U = U0.clone()
+    timers = not a.no_timers
+
+    o64 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=0)
+    el, st, rep = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o64, timers)
+    if st["status"] != 0:
+        raise RuntimeError(f"FP64 Newton stepping failed: {st}")
+    log(f"fp64: {a.steps} steps {el:.3f} s newton {st['newton_its']} linear {st['linear_its']}")
+    value = nc * st["newton_its"] / el
+    kernels = kernel_table(rep) if rep else []
+    roof = None
+    if kernels:
+        k0 = kernels[0]
+        ach = k0["GB_s"]
+        roof = {"bound": "hbm", "kernel": k0["kernel"], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": k0["bytes_per_launch"], "avg_us": k0["avg_us"]}
+    U64 = U.clone()
+
+    fp32 = None
+    if not a.no_fp32:
+        o32 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=1)
+        el32, st32, rep32 = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o32, timers)
+        if st32["status"] != 0:
+            raise RuntimeError(f"FP32-Krylov Newton stepping failed: {st32}")
+        rel = ((U - U64).abs() / U64.abs().clamp_min(1e-300)).max().item()
+        v32 = nc * st32["newton_its"] / el32
+        fp32 = {"value": v32, "ms_per_step": 1e3 * el32 / a.steps, "speedup_vs_fp64": round(v32 / value, 3),
+                "wall_speedup_vs_fp64": round(el / el32, 3),
+                "newton_its": st32["newton_its"], "linear_its": st32["linear_its"], "max_rel_p_diff_vs_fp64": rel,
+                "kernels": kernel_table(rep32)[:6] if rep32 else []}
+        log(f"fp32 krylov: {el32:.3f} s newton {st32['newton_its']} linear {st32['linear_its']} rel {rel:.3e}")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(P, a.dt)
+        log(f"cpu baseline {cpu['value']:.3e} Newton cells/s")
+
+    line = {
+        "metric": "MPFA-O Newton cells/s at 1M hex (FP64)", "value": value, "unit": "Newton cells/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded layered permeability, jittered hex mesh)",
+        "config": {"workload": f"layered jittered hex box {nx}x{ny}x{nz} ({nc} cells), 1000x1000x100 m, "
+                               f"backward Euler dt={a.dt:g} s, snes_rtol 1e-10, GMRES(30)+bjacobi ILU(0) "
+                               f"{a.brick[0]}x{a.brick[1]}x{a.brick[2]} bricks",
+                   "cells": nc, "dt": a.dt, "newton_its": st["newton_its"], "linear_its": st["linear_its"],
+                   "setup_s": round(setup_s, 3), "kernel_timers": timers},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "fp32_krylov": fp32,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

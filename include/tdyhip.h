@@ -143,6 +143,13 @@ int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, cons
 int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max,
                      double final_time, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
 
+/* Kernel timers: HIP events around every launch of the path on its stream.
+ * enable=1 resets and starts, 0 stops.  The report is a JSON array of
+ * {kernel, launches, ms, bytes} (bytes: algorithmic bytes, each array counted
+ * once per launch). */
+int tdyhip_kernel_timers(tdyhip_ctx *ctx, int32_t enable);
+int tdyhip_kernel_timer_report(tdyhip_ctx *ctx, char *buf, int64_t buflen);
+
 /* Diagnostics: transmissibility (rows face*4+subface, columns cell | nc+bnd) and the
  * gravity term, copied to HOST arrays, for inspection and parity tests. */
 int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz);

@@ -42,6 +42,7 @@ def lib():
         L.ora_snes_jacobian.argtypes = [P, P, P]
         L.ora_run_ts_be.argtypes = [P, P, C.c_double, C.c_int, C.c_int, C.c_double, P]
         L.ora_run_snes_steps.argtypes = [P, P, C.c_double, C.c_int, C.c_int, C.c_double, P]
+        L.ora_set_newton_maxit.argtypes = [P, C.c_int]
         L.ora_run_tdyti.argtypes = [P, P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, P]
         L.ora_newton_snes.argtypes = [P, P, C.c_int, C.c_double, P]
@@ -199,6 +200,9 @@ class Oracle:
         lib().ora_run_tdyti(self.h, _p(U), float(dt_init), float(growth), float(dt_max), float(final_time),
                             0 if lin == "lu" else 1, float(rtol), _p(st))
         return U, st
+
+    def set_newton_maxit(self, maxit):
+        lib().ora_set_newton_maxit(self.h, int(maxit))
 
     def newton_snes(self, U, lin="lu", rtol=1e-8):
         U = np.array(U, dtype=np.float64, copy=True)

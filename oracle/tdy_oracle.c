@@ -72,6 +72,7 @@ struct ora_ctx {
   int density_type, gmatrix;
   double coeffs[4]; /* Mualem cubic */
   double *p_bnd_fn;
+  int newton_maxit; /* SNES max_it (PETSc default 50) */
 
   /* IR */
   vtx_ir *ir;
@@ -719,6 +720,7 @@ ora_ctx *ora_create(int nc, int nf, int nv, int nbnd, const int *cell_verts, con
   ctx->gravity = params[ORA_P_GRAVITY];
   ctx->density_type = (int)params[ORA_P_DENSITY];
   ctx->gmatrix = (int)params[ORA_P_GMATRIX];
+  ctx->newton_maxit = 50;
   ctx->pref = params[ORA_P_PREF];
   ctx->p_bnd_fn = p_bnd_fn ? dupd(p_bnd_fn, nbnd) : NULL;
   if (mualem_coeffs(ctx->vg_m, ctx->x0, ctx->x1, ctx->x2, ctx->dx, ctx->coeffs)) { set_err("cubic setup failed"); ora_destroy(ctx); return NULL; }
@@ -1181,7 +1183,7 @@ static int newton(ora_ctx *ctx, double *X, fun_t fun, jac_t jac, void *aux, int 
   double *W = malloc(n * sizeof(double)), *Gv = malloc(n * sizeof(double));
   double *Jv = malloc(nnz * sizeof(double)), *JY = malloc(n * sizeof(double));
   const double atol = 1e-50, stol = 1e-8, divtol = 1e4, alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8;
-  const int maxit = 50;
+  const int maxit = ctx->newton_maxit;
   fun(ctx, X, F, aux);
   double fnorm = nrm2(n, F), fnorm0 = fnorm, ttol = rtol * fnorm;
   int status = 0, it = 0, linits = 0;
@@ -1259,6 +1261,8 @@ out:
   free(F); free(Y); free(W); free(Gv); free(Jv); free(JY);
   return status;
 }
+
+void ora_set_newton_maxit(ora_ctx *ctx, int maxit) { ctx->newton_maxit = maxit > 0 ? maxit : 50; }
 
 int ora_newton_snes(ora_ctx *ctx, double *U, int lin, double rtol, int *stats) {
   return newton(ctx, U, sn_fun, sn_jac, NULL, lin, rtol, stats);

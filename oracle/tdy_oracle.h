@@ -68,6 +68,8 @@ int ora_run_tdyti(ora_ctx *ctx, double *U, double dt_init, double growth,
 /* one Newton solve of the SNES form at the current dt/accum_prev, starting
  * from U (in/out): returns status, fills stats */
 int ora_newton_snes(ora_ctx *ctx, double *U, int lin, double snes_rtol, int *stats);
+/* SNES max_it for every driver (default 50; at the cap status is -5) */
+void ora_set_newton_maxit(ora_ctx *ctx, int maxit);
 
 void ora_liquid_mass(ora_ctx *ctx, const double *U, double *out);
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "tdyhip_ifunction_ijacobian", "tdyhip_set_dtime", "tdyhip_snes_presolve", "tdyhip_snes_function",
     "tdyhip_snes_jacobian", "tdyhip_trans_nnz", "tdyhip_trans_csr", "tdyhip_gravdis",
     "tdyhip_solver_defaults", "tdyhip_newton_snes", "tdyhip_run_snes_steps", "tdyhip_run_ts_be",
-    "tdyhip_run_tdyti", "tdyhip_set_pc_partition",
+    "tdyhip_run_tdyti", "tdyhip_set_pc_partition", "tdyhip_kernel_timers", "tdyhip_kernel_timer_report",
 ]
 
 
@@ -86,7 +86,8 @@ def load():
         "tdyhip_trans_csr": [P, P, P, P], "tdyhip_gravdis": [P, P],
         "tdyhip_newton_snes": [P, P, P, P], "tdyhip_run_snes_steps": [P, P, D, C.c_int32, P, P],
         "tdyhip_run_ts_be": [P, P, D, C.c_int32, P, P], "tdyhip_run_tdyti": [P, P, D, D, D, D, P, P],
-        "tdyhip_set_pc_partition": [P, P],
+        "tdyhip_set_pc_partition": [P, P], "tdyhip_kernel_timers": [P, C.c_int32],
+        "tdyhip_kernel_timer_report": [P, C.c_char_p, C.c_int64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -53,6 +53,70 @@ static int dupload(tdyhip_ctx *ctx, T **p, const T *h, size_t n) {
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ---- kernel timers (events on the launching stream) ----
+static hipEvent_t kt_event(tdyhip_ctx *ctx) {
+  hipEvent_t e = nullptr;
+  if (!ctx->kt.pool.empty()) {
+    e = ctx->kt.pool.back();
+    ctx->kt.pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+static hipEvent_t kt_begin(tdyhip_ctx *ctx, hipStream_t st) {
+  if (!ctx->kt.on) return nullptr;
+  hipEvent_t e = kt_event(ctx);
+  if (e) (void)hipEventRecord(e, st);
+  return e;
+}
+static void kt_end(tdyhip_ctx *ctx, hipStream_t st, const char *name, double bytes, hipEvent_t e0) {
+  if (!e0) return;
+  hipEvent_t e1 = kt_event(ctx);
+  if (!e1) return;
+  (void)hipEventRecord(e1, st);
+  KernelTimers::Rec &r = ctx->kt.get(name);
+  r.launches++;
+  r.bytes += bytes;
+  r.pending.emplace_back(e0, e1);
+}
+static void kt_collect(tdyhip_ctx *ctx) {
+  for (auto &nr : ctx->kt.recs) {
+    for (auto &pe : nr.second.pending) {
+      float ms = 0.f;
+      if (hipEventSynchronize(pe.second) == hipSuccess && hipEventElapsedTime(&ms, pe.first, pe.second) == hipSuccess)
+        nr.second.ms += ms;
+      ctx->kt.pool.push_back(pe.first);
+      ctx->kt.pool.push_back(pe.second);
+    }
+    nr.second.pending.clear();
+  }
+}
+#define KT(ctx, st, name, bytes, ...)              \
+  do {                                             \
+    hipEvent_t kt_e0_ = kt_begin(ctx, st);         \
+    __VA_ARGS__;                                   \
+    kt_end(ctx, st, name, (double)(bytes), kt_e0_); \
+  } while (0)
+
+// algorithmic bytes per launch (each array counted once)
+static double bytes_state(const tdyhip_ctx *ctx) {
+  return (double)ctx->nc * (8 * 5 + 8 * 7) + (double)ctx->nbnd * (4 + 8 + 8 + 8 + 8);
+}
+static double bytes_rows(const tdyhip_ctx *ctx, bool R, bool J) {
+  double b = (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (8 + 4) + (double)ctx->nf * (8 + 1) +
+             (double)ctx->nv * kMaxCols * 4 + (double)ctx->nc * 8 * 3 + (double)ctx->nbnd * 8 * 3;
+  if (R) b += (double)ctx->active_rows * 8;
+  if (J) b += (double)ctx->jnnz * 8 + (double)ctx->active_rows * 2 + (double)ctx->nv * 64 + (double)ctx->nc * 8 * 2;
+  return b;
+}
+static double bytes_gather(const tdyhip_ctx *ctx, bool R, bool J) {
+  double b = 0;
+  if (R) b += (double)ctx->nc * (24 + 8 + 8) + (double)ctx->nf * (32 + 8);
+  if (J) b += (double)ctx->nc * (8 * 4 + 1 + 16);
+  return b;
+}
+
 static StateParams state_params(const tdyhip_ctx *ctx, int form, double shift) {
   StateParams sp;
   sp.vg_m = ctx->vg_m; sp.vg_alpha = ctx->vg_alpha; sp.x0 = ctx->x0; sp.x1 = ctx->x1; sp.dx = ctx->dx;
@@ -117,9 +181,10 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U, double *J, bool ell) {
 // state (+boundary) at U
 static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double *Udot, hipStream_t st) {
   StateParams sp = state_params(ctx, form, 0.0);
-  k_cell_state<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, Udot, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol,
-                                                   ctx->d_accum_prev, ctx->d_rho, ctx->d_ukvr, ctx->d_drho,
-                                                   ctx->d_dukvr, ctx->d_dM, ctx->d_d2M, ctx->d_acc);
+  KT(ctx, st, "k_cell_state", bytes_state(ctx),
+     k_cell_state<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, Udot, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol,
+                                                      ctx->d_accum_prev, ctx->d_rho, ctx->d_ukvr, ctx->d_drho,
+                                                      ctx->d_dukvr, ctx->d_dM, ctx->d_d2M, ctx->d_acc));
   if (ctx->nbnd > 0)
     k_bnd_state<<<nblk(ctx->nbnd, 256), 256, 0, st>>>(ctx->nbnd, sp, U, ctx->d_bnd_cell, ctx->d_Sr,
                                                       ctx->d_pbnd_fn, ctx->d_Pbnd, ctx->d_ukvr_bnd);
@@ -136,22 +201,25 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
   const int64_t jlen = ell ? (int64_t)ctx->jell_w * ctx->nc : ctx->jnnz;
   const int32_t *jrp = ell ? nullptr : ctx->d_jrp;
   if (J) k_fill<<<nblk(jlen, 256), 256, 0, st>>>(jlen, J, 0.0);
-  if (R && J) k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
-  else if (R) k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a);
-  else k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a);
+  if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
+  else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a));
+  else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
   const unsigned g = nblk(ctx->nc, 256);
   if (R && J)
-    k_cell_gather<true, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                 ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                 Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
+    KT(ctx, st, "k_cell_gather_RJ", bytes_gather(ctx, true, true),
+       k_cell_gather<true, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                    ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                    Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
   else if (R)
-    k_cell_gather<true, false><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                  ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                  Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
+    KT(ctx, st, "k_cell_gather_R", bytes_gather(ctx, true, false),
+       k_cell_gather<true, false><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                     ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                     Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
   else
-    k_cell_gather<false, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                  ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                  Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J);
+    KT(ctx, st, "k_cell_gather_J", bytes_gather(ctx, false, true),
+       k_cell_gather<false, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
+                                                     ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
+                                                     Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -194,6 +262,14 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   const int nc = m->nc, nf = m->nf, nv = m->nv, nbnd = m->nbnd;
   const int64_t nrows = (int64_t)4 * nf;
   ctx->jnnz = topo.jrp[nc];
+  for (int f = 0; f < nf; f++) {
+    if (topo.face_flags[f] & kFaceNoFlow) continue;
+    ctx->active_rows += 4;
+    for (int q = 0; q < 4; q++) {
+      const int v = m->face_verts[f * 4 + q];
+      for (int k = 0; k < kMaxCols; k++) ctx->trans_nnz += topo.vcols[(size_t)v * kMaxCols + k] >= 0;
+    }
+  }
 
   if ((rc = dupload(ctx, &ctx->d_face_cells, m->face_cells, (size_t)2 * nf))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_face_verts, m->face_verts, (size_t)4 * nf))) return fail(rc);

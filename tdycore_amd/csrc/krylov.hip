@@ -39,7 +39,13 @@ struct BJLayout {
   int32_t *d_nlev = nullptr;   // nblocks*2 (L, U level counts)
   uint16_t *d_cL = nullptr;    // nblocks*kLU*bs local column of L slot (bs = zero entry)
   uint16_t *d_cU = nullptr;    // nblocks*kLU*bs local column of U slot
+  // ILU(0) update lists: for row i and lower slot p (row k), pairs (d, s) with
+  // a_i[d] -= l_ik * a_k[s]; npair[p] pairs per p, packed d | s << 8.
+  uint8_t *d_npair = nullptr;  // nblocks*kLU*bs
+  uint16_t *d_pair = nullptr;  // nblocks*kPair*bs
 };
+
+constexpr int kPair = kLU * kLU;  // max update pairs per row
 
 // ---------------------------------------------------------------- ILU(0) factor
 template <class TV, int BS>
@@ -63,21 +69,20 @@ __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__re
   const int lev = L.d_levL[base + t];
   const int nlev = L.d_nlev[2 * b];
   __syncthreads();
+  const int64_t pbase = (int64_t)b * kPair * BS;
+  const int64_t nbase = (int64_t)b * kLU * BS;
   for (int l = 0; l < nlev; l++) {
     if (lev == l && nsl > 0) {
       const int di = dg[t];
+      int e = 0;
       for (int p = 0; p < di; p++) {  // lower entries, increasing column
         const int k = lc[p][t];
-        const TV ukk = a[dg[k]][k];
-        const TV lik = a[p][t] / ukk;
+        const TV lik = a[p][t] / a[dg[k]][k];
         a[p][t] = lik;
-        // row i entries after p, merged with row k entries after its diagonal
-        int pi = p + 1;
-        const int nk = ns[k];
-        for (int q = dg[k] + 1; q < nk; q++) {
-          const int cq = lc[q][k];
-          while (pi < nsl && lc[pi][t] < cq) pi++;
-          if (pi < nsl && lc[pi][t] == cq) a[pi][t] -= lik * a[q][k];
+        const int np = L.d_npair[nbase + (int64_t)p * BS + t];
+        for (int q = 0; q < np; q++, e++) {
+          const int pr = L.d_pair[pbase + (int64_t)e * BS + t];
+          a[pr & 0xff][t] -= lik * a[pr >> 8][k];
         }
       }
       if (a[dg[t]][t] == (TV)0) atomicOr(status, 1);

@@ -16,7 +16,8 @@ struct HostBJ {
   int nblocks = 0, bs = 0;
   std::vector<int32_t> row, jpos;
   std::vector<uint8_t> lcol, diag, nslot;
-  std::vector<uint16_t> levL, levU, cL, cU;
+  std::vector<uint16_t> levL, levU, cL, cU, pair;
+  std::vector<uint8_t> npair;
   std::vector<int32_t> nlev;
 };
 
@@ -87,6 +88,8 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   out.levU.assign((size_t)nb * S, 0);
   out.cL.assign((size_t)nb * kLU * S, (uint16_t)S);
   out.cU.assign((size_t)nb * kLU * S, (uint16_t)S);
+  out.npair.assign((size_t)nb * kLU * S, 0);
+  out.pair.assign((size_t)nb * kPair * S, 0);
   out.nlev.assign((size_t)nb * 2, 0);
   for (int b = 0; b < nb; b++) {
     const auto &mem = members[b];
@@ -136,6 +139,25 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     for (int i = 0; i < m; i++) {
       const int di = out.diag[(size_t)b * S + i], ns = out.nslot[(size_t)b * S + i];
       if (di > kLU || ns - di - 1 > kLU) return "more than 13 lower or upper in-block entries in a row";
+      // ILU(0) update pairs: row i slot d -= l_ik * row k slot s, for the entries of row k
+      // after its diagonal whose column is in row i's pattern
+      int e = 0;
+      for (int p = 0; p < di; p++) {
+        const int k = out.lcol[(size_t)b * kW * S + (size_t)p * S + i];
+        const int dk = out.diag[(size_t)b * S + k], nk = out.nslot[(size_t)b * S + k];
+        int np = 0, pi = p + 1;
+        for (int q = dk + 1; q < nk; q++) {
+          const int cq = out.lcol[(size_t)b * kW * S + (size_t)q * S + k];
+          while (pi < ns && out.lcol[(size_t)b * kW * S + (size_t)pi * S + i] < cq) pi++;
+          if (pi < ns && out.lcol[(size_t)b * kW * S + (size_t)pi * S + i] == cq) {
+            if (e >= kPair) return "too many ILU(0) update pairs in a row";
+            out.pair[(size_t)b * kPair * S + (size_t)e * S + i] = (uint16_t)(pi | (q << 8));
+            e++;
+            np++;
+          }
+        }
+        out.npair[(size_t)b * kLU * S + (size_t)p * S + i] = (uint8_t)np;
+      }
       for (int p = 0; p < kLU; p++) {
         out.cL[(size_t)b * kLU * S + (size_t)p * S + i] =
             p < di ? out.lcol[(size_t)b * kW * S + (size_t)p * S + i] : (uint16_t)S;
@@ -217,7 +239,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if ((rc = up(&ws.bj.d_row, hb.row)) || (rc = up(&ws.bj.d_jpos, hb.jpos)) || (rc = up(&ws.bj.d_lcol, hb.lcol)) ||
       (rc = up(&ws.bj.d_diag, hb.diag)) || (rc = up(&ws.bj.d_nslot, hb.nslot)) || (rc = up(&ws.bj.d_levL, hb.levL)) ||
       (rc = up(&ws.bj.d_levU, hb.levU)) || (rc = up(&ws.bj.d_nlev, hb.nlev)) || (rc = up(&ws.bj.d_cL, hb.cL)) ||
-      (rc = up(&ws.bj.d_cU, hb.cU)))
+      (rc = up(&ws.bj.d_cU, hb.cU)) || (rc = up(&ws.bj.d_npair, hb.npair)) || (rc = up(&ws.bj.d_pair, hb.pair)))
     return rc;
   for (double **p : {&ws.F, &ws.Y, &ws.W, &ws.G, &ws.JY, &ws.Udot, &ws.Uprev_ts, &ws.Usave})
     if ((rc = ws_alloc(ws, p, n))) return rc;
@@ -266,6 +288,18 @@ static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t s
   return rc;
 }
 
+static double bytes_spmv(const tdyhip_ctx *ctx, size_t tv) {
+  return (double)ctx->jell_w * ctx->nc * (tv + 4) + 2.0 * ctx->nc * tv;
+}
+static double bytes_apply(const SolverWS &ws, size_t tv) {
+  const double rows = (double)ws.bj.nblocks * ws.bj.bs;
+  return rows * ((2 * kLU + 1) * tv + 2 * kLU * 2 + 4 + 2 + 2) + 2.0 * ws.n * tv;
+}
+static double bytes_factor(const SolverWS &ws, size_t tv) {
+  const double rows = (double)ws.bj.nblocks * ws.bj.bs;
+  return rows * (kW * (4 + 1 + 8) + kLU + kPair * 2 + (2 * kLU + 1) * tv + 1 + 1 + 2);
+}
+
 // Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
 // GMRES(m), left preconditioned, classical Gram-Schmidt.  Each restart cycle is
 // launched as one batch of kernels; Givens rotations and the stopping test run
@@ -285,11 +319,11 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   if (!std::is_same<TV, double>::value)
     k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
   k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
-  if (ws.bj.bs == 64) k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status);
-  else k_bj_factor<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, ws.J, fac, ws.status);
+  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
-    if (ws.bj.bs == 64) k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp);
-    else k_bj_apply<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, fac, in, out, stp);
+    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp));
+    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, fac, in, out, stp));
   };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
@@ -308,11 +342,11 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   for (int cycle = 0;; cycle++) {
     k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V, gd);
     for (int j = 0; j < m; j++) {
-      k_spmv_ell_stop<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, V + (int64_t)j * N, z, gd);
+      KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(TV)), k_spmv_ell_stop<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, V + (int64_t)j * N, z, gd));
       pc_apply(z, w, stop);
-      k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk, stop);
+      KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
-      k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part, stop);
+      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part, stop));
       k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1, stop);
       k_arnoldi_step<<<1, 1, 0, st>>>(gd, j, m, ws.h, dH, dcs, dsn, dg);
       k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V + (int64_t)(j + 1) * N, gd);
@@ -326,7 +360,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.converged || hg.its >= o.ksp_max_it) break;
     // restart: w = M^-1 (b - A x)
-    k_spmv_ell<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, x, z);
+    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(TV)), k_spmv_ell<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, x, z));
     k_sub<TV><<<g256, 256, 0, st>>>(N, b, z, z);
     pc_apply(z, w, nullptr);
     norm2_to_h0(w);
@@ -412,7 +446,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
       k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
       ynorm = maxstep;
     }
-    k_spmv_ell<double><<<g, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, ws.Y, ws.JY);
+    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(double)), k_spmv_ell<double><<<g, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, ws.Y, ws.JY));
     double initslope;
     if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
     if (initslope > 0.0) initslope = -initslope;
@@ -491,6 +525,33 @@ static int check_opts(const tdyhip_solver_opts *o) {
     tdyhip::set_error("bad solver options");
     return TDYHIP_ERR_ARG;
   }
+  return 0;
+}
+
+int tdyhip_kernel_timers(tdyhip_ctx *ctx, int32_t enable) {
+  using namespace tdyhip;
+  if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
+  kt_collect(ctx);
+  if (enable) ctx->kt.recs.clear();
+  ctx->kt.on = enable != 0;
+  return 0;
+}
+
+int tdyhip_kernel_timer_report(tdyhip_ctx *ctx, char *buf, int64_t buflen) {
+  using namespace tdyhip;
+  if (!ctx || !buf || buflen < 3) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  kt_collect(ctx);
+  std::string js = "[";
+  char line[512];
+  for (size_t i = 0; i < ctx->kt.recs.size(); i++) {
+    const auto &r = ctx->kt.recs[i];
+    snprintf(line, sizeof line, "%s{\"kernel\":\"%s\",\"launches\":%lld,\"ms\":%.6f,\"bytes\":%.6e}", i ? "," : "",
+             r.first.c_str(), (long long)r.second.launches, r.second.ms, r.second.bytes);
+    js += line;
+  }
+  js += "]";
+  if ((int64_t)js.size() + 1 > buflen) { set_error("report buffer too small"); return TDYHIP_ERR_ARG; }
+  std::memcpy(buf, js.c_str(), js.size() + 1);
   return 0;
 }
 

@@ -56,6 +56,26 @@ void set_error(const std::string &msg);
 
 struct SolverWS;
 
+// Per-kernel timers: HIP events around each launch on the launching stream,
+// plus the launch's algorithmic bytes (each array counted once).
+struct KernelTimers {
+  bool on = false;
+  struct Rec {
+    int64_t launches = 0;
+    double bytes = 0.0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  };
+  std::vector<std::pair<std::string, Rec>> recs;
+  std::vector<hipEvent_t> pool;
+  Rec &get(const char *name) {
+    for (auto &r : recs)
+      if (r.first == name) return r.second;
+    recs.emplace_back(name, Rec());
+    return recs.back().second;
+  }
+};
+
 }  // namespace tdyhip
 
 struct tdyhip_ctx {
@@ -106,4 +126,8 @@ struct tdyhip_ctx {
   tdyhip::SolverWS *ws = nullptr;  // Newton/Krylov workspace, allocated on first solve
   std::vector<int32_t> pc_part;    // caller-given block-Jacobi partition (empty: breadth-first)
   int pc_part_version = 0;
+  tdyhip::KernelTimers kt;
+  // algorithmic-byte bookkeeping
+  int64_t trans_nnz = 0;           // transmissibility entries (cells + Dirichlet columns)
+  int64_t active_rows = 0;         // flux rows of faces that carry flux
 };

@@ -139,6 +139,18 @@ class MPFAO:
     def SNESJacobian(self, U, J):
         _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
 
+    # ---- per-kernel timing ----
+    def kernel_timers(self, enable: bool):
+        """Start (reset) or stop HIP-event timing of every kernel launch."""
+        _lib.check(_lib.load().tdyhip_kernel_timers(self._h, 1 if enable else 0))
+
+    def kernel_timer_report(self):
+        """List of {kernel, launches, ms, bytes} since timers were enabled."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        _lib.check(_lib.load().tdyhip_kernel_timer_report(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # ---- Newton / time loops on the device ----
     def set_pc_partition(self, block_of_cell=None):
         """Block-Jacobi blocks (block id per cell, each block <= pc_block cells);

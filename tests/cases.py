@@ -5,6 +5,7 @@ import os
 import numpy as np
 
 from tdycore_amd import meshgen as mg
+from tdycore_amd import workloads
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDS = json.load(open(os.path.join(HERE, "golden", "reference_golds.json")))
@@ -36,35 +37,10 @@ def gold_problem(case):
 
 
 def cfg0_problem(n=16, seed=20261202, jitter=0.1):
-    """Plain-method configuration: jittered unit cube, rotated full-tensor K,
-    phi ~ U(0.2,0.4), VG n=1.5 (m=1/3), Sr=0.1, hydrostatic IC with the water
-    table at z=0.5, Dirichlet top/bottom (hydrostatic values), no-flow sides."""
-    rng = np.random.default_rng(seed)
-    m = mg.box_mesh(n, n, n, jitter=jitter, seed=seed + 1)
-    K = mg.random_rotated_K(m.nc, rng, -12.0, 0.5)
-    phi = rng.uniform(0.2, 0.4, size=m.nc)
-    Sr = np.full(m.nc, 0.1)
-    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-", "x+", "y-", "y+"])
-    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 0.5))
-    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], 0.5)
-    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
-    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+    """Plain-method configuration (16^3 jittered unit cube)."""
+    return workloads.jittered_cube_problem(n=n, seed=seed, jitter=jitter)
 
 
-def cfg1_problem(nx=128, ny=128, nz=64, seed=20261203, lx=1000.0, ly=1000.0, lz=100.0):
-    """Plain method at scale: nx*ny*nz hex on lx*ly*lz m, layered log10 k ~ N(-12,1)
-    smoothed by a 5x5 x-y moving average per layer (rescaled back to sigma 1),
-    K = diag(k, k, k/10); phi ~ U(0.2,0.4), Sr 0.1, VG n=1.5 (m=1/3), alpha 1e-4;
-    hydrostatic IC with the water table at mid-height; Dirichlet top/bottom at
-    hydrostatic values, no-flow sides."""
-    rng = np.random.default_rng(seed)
-    m = mg.box_mesh(nx, ny, nz, upper=(lx, ly, lz))
-    K = mg.layered_K(nx, ny, nz, rng)
-    phi = rng.uniform(0.2, 0.4, size=m.nc)
-    Sr = np.full(m.nc, 0.1)
-    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-", "x+", "y-", "y+"])
-    zt = 0.5 * lz
-    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], zt))
-    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
-    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
-    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+def cfg1_problem(nx=128, ny=128, nz=64, **kw):
+    """Plain method at scale (128x128x64 layered, jittered box)."""
+    return workloads.layered_box_problem(nx, ny, nz, **kw)
