@@ -34,6 +34,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (scaled by the counters' reading of a 1 GiB device copy)
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "pmc_traffic_layered_box_128x128x64.json")
 
 
 def parse():
@@ -87,6 +90,24 @@ def run_steps(g, U, U0, dt, K, W, opts, timers):
         rep = g.kernel_timer_report()
         g.kernel_timers(False)
     return el, st, rep
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth (read + write bytes / time) of a
+    1 GiB FP64 buffer, measured on torch's stream with torch events."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    return gbs
 
 
 def cpu_baseline(P, dt):
@@ -145,13 +166,20 @@ def main():
     log(f"fp64: {a.steps} steps {el:.3f} s newton {st['newton_its']} linear {st['linear_its']}")
     value = nc * st["newton_its"] / el
     kernels = kernel_table(rep) if rep else []
+    copy_gbs = stream_copy_gbs()
+    log(f"stream copy {copy_gbs:.0f} GB/s")
     roof = None
     if kernels:
         k0 = kernels[0]
         ach = k0["GB_s"]
+        traffic = None
+        if (nx, ny, nz) == (128, 128, 64) and os.path.exists(TRAFFIC_FILE):
+            pmc = json.load(open(TRAFFIC_FILE))["kernels"].get(k0["kernel"])
+            traffic = pmc["traffic_bytes"] if pmc else None
         roof = {"bound": "hbm", "kernel": k0["kernel"], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": k0["bytes_per_launch"], "avg_us": k0["avg_us"]}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": k0["bytes_per_launch"], "avg_us": k0["avg_us"],
+                "stream_copy_GB_s": round(copy_gbs, 1), "frac_of_copy": round(ach / copy_gbs, 4)}
     U64 = U.clone()
 
     fp32 = None
