@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-Krylov variant")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel HIP events")
+    ap.add_argument("--serve", action="store_true",
+                    help="run the serving ensemble instead (Poisson arrivals of box requests)")
+    ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests on the device")
+    ap.add_argument("--build-workers", type=int, default=12, help="serving: host threads building problems")
     return ap.parse_args()
 
 
@@ -130,6 +136,51 @@ def cpu_baseline(P, dt):
                       f"line search) of step 1 on the same {nc}-cell mesh; {el:.2f} s"}
 
 
+def serve(a):
+    """Serving ensemble: requests arrive as a Poisson stream and are admitted by
+    HBM footprint into a fixed number of device slots."""
+    import numpy as np
+    import torch
+    from tdycore_amd import serving, workloads
+    torch.cuda.init()
+    specs = workloads.ensemble_request_specs(a.requests)
+    srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers)
+    # warm the library and the device once (not timed)
+    srv.submit(serving.Request(rid=-1, nx=16, ny=16, nz=8, seed=1, sigma=1.0, steps=1)).result()
+    rng = np.random.default_rng(20261207)
+    gaps = rng.exponential(1.0 / a.rate, size=len(specs))
+    futs = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for sp, gap in zip(specs, gaps):
+        t_next += gap
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(srv.submit(serving.Request(**sp)))
+        if len(futs) % 64 == 0:
+            log(f"submitted {len(futs)} requests, {sum(f.done() for f in futs)} done")
+    res = [f.result() for f in futs]
+    srv.close()
+    summ = serving.latency_summary(res)
+    bad = [r["rid"] for r in res if r["stats"]["status"] != 0]
+    cells = sum(r["nc"] for r in res)
+    newton = sum(r["stats"]["newton_its"] for r in res)
+    line = {
+        "metric": "ensemble Richards solves/s (p50/p99 latency)", "value": summ["solves_per_s"], "unit": "solves/s",
+        "n_gpus": 1, "steps": len(res), "warmup": 1, "ms_per_step": 1e3 * summ["makespan_s"] / len(res),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered boxes, rotated log-normal permeability)",
+        "config": {"workload": f"{len(res)} requests, Poisson arrivals at {a.rate:g}/s, nx,ny~U{{16..96}}, "
+                               f"nz~U{{8..48}}, sigma~U(0.5,1.5), 5 backward-Euler steps of 1 h, FP64 residual, "
+                               f"FP32 Krylov, {a.slots} device slots",
+                   "total_cells": cells, "newton_its": newton, "failed": bad},
+        "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"], "serving": summ,
+        "peak_inflight": srv.peak_inflight, "peak_admitted_bytes": srv.peak_used, "hbm_budget": srv.hbm_budget,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -137,6 +188,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE {world}); "
               "the path is measured on one GPU's full mesh", file=sys.stderr)
         sys.exit(2)
+    if a.serve:
+        return serve(a)
     import numpy as np
     import torch
     from tdycore_amd import workloads

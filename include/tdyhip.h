@@ -143,6 +143,12 @@ int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, cons
 int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max,
                      double final_time, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
 
+/* Device memory held by the context (setup arrays plus the solver workspace), in
+ * bytes: what an HBM-footprint admission policy accounts per problem.  No
+ * counterpart in the reference (PETSc's PetscMallocGetCurrentUsage plays that
+ * role on the CPU). */
+int tdyhip_device_bytes(const tdyhip_ctx *ctx, int64_t *bytes);
+
 /* Kernel timers: HIP events around every launch of the path on its stream.
  * enable=1 resets and starts, 0 stops.  The report is a JSON array of
  * {kernel, launches, ms, bytes} (bytes: algorithmic bytes, each array counted

@@ -40,6 +40,7 @@ static int dalloc(tdyhip_ctx *ctx, T **p, size_t n) {
     return e == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP;
   }
   ctx->allocations.push_back(*p);
+  ctx->dev_bytes += (int64_t)(n * sizeof(T));
   return 0;
 }
 

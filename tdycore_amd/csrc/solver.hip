@@ -174,6 +174,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
 struct SolverWS {
   int n = 0, restart = 0, fp32 = -1, bs = 0, part_version = -1;
   int64_t nnz = 0;
+  int64_t bytes = 0;  // bytes held in `allocs`
   BJLayout bj;
   std::vector<void *> allocs;
   // Newton vectors (FP64)
@@ -204,6 +205,7 @@ static int ws_alloc(SolverWS &ws, T **p, size_t n) {
     return e == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP;
   }
   ws.allocs.push_back(*p);
+  ws.bytes += (int64_t)(n * sizeof(T));
   return 0;
 }
 
@@ -525,6 +527,13 @@ static int check_opts(const tdyhip_solver_opts *o) {
     tdyhip::set_error("bad solver options");
     return TDYHIP_ERR_ARG;
   }
+  return 0;
+}
+
+int tdyhip_device_bytes(const tdyhip_ctx *ctx, int64_t *bytes) {
+  using namespace tdyhip;
+  if (!ctx || !bytes) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  *bytes = ctx->dev_bytes + (ctx->ws ? ctx->ws->bytes : 0);
   return 0;
 }
 

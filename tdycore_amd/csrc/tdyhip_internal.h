@@ -123,6 +123,7 @@ struct tdyhip_ctx {
   std::vector<int32_t> h_face_verts;
   std::vector<uint8_t> h_face_flags;
   std::vector<void *> allocations;
+  int64_t dev_bytes = 0;           // bytes held in `allocations`
   tdyhip::SolverWS *ws = nullptr;  // Newton/Krylov workspace, allocated on first solve
   std::vector<int32_t> pc_part;    // caller-given block-Jacobi partition (empty: breadth-first)
   int pc_part_version = 0;

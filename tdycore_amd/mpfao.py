@@ -139,6 +139,12 @@ class MPFAO:
     def SNESJacobian(self, U, J):
         _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
 
+    def device_bytes(self) -> int:
+        """Device memory held by this context (setup arrays + solver workspace)."""
+        b = C.c_int64()
+        _lib.check(_lib.load().tdyhip_device_bytes(self._h, C.byref(b)))
+        return b.value
+
     # ---- per-kernel timing ----
     def kernel_timers(self, enable: bool):
         """Start (reset) or stop HIP-event timing of every kernel launch."""

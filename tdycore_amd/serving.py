@@ -1,0 +1,228 @@
+"""Continuous-batching server for ensembles of Richards solves on one GPU.
+
+Requests (a box size, a permeability seed and sigma, a number of
+backward-Euler steps) arrive at any time.  Each one goes through three
+stages:
+
+1. **build** - a host thread pool generates the mesh, the materials and the
+   initial state.  This is the part of the reference that DMPlex and
+   ``InitMaterials`` do (src/fv/mpfao/tdympfao.c:630-763);
+2. **admit** - the request waits, in the order its build finished, until a GPU slot is free
+   and its device footprint fits in the HBM budget beside the requests
+   already on the device;
+3. **solve** - a slot thread sets up the transmissibilities on the device and
+   runs the backward-Euler steps.  Each step is a pre-solve plus a Newton
+   solve: FP64 residual, FP32-Krylov inner solve.  The request's state stays
+   in HBM from setup to its last step.  Each slot drives its own
+   non-blocking HIP stream, so the setup and Newton kernels of different
+   requests overlap on the device, and a request that arrives while others
+   are solving starts as soon as a slot and memory free up.
+
+Results are bit-for-bit those of the same request run alone, up to the
+summation order of the Jacobian's FP64 atomics.  The serving tests bound the
+difference at 1e-8 relative.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import dataclasses
+import queue
+import threading
+import time
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import workloads
+from .meshgen import brick_partition
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    nx: int
+    ny: int
+    nz: int
+    seed: int
+    sigma: float
+    steps: int = 5
+    dt: float = 3600.0
+
+
+def block_rows(nx, ny, nz, brick):
+    """Rows of the block-Jacobi layout: one 64- or 256-wide slab per brick."""
+    bx, by, bz = brick
+    nb = -(-nx // bx) * -(-ny // by) * -(-nz // bz)
+    bs = bx * by * bz
+    return nb * (64 if bs <= 64 else 256)
+
+
+def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
+    """Upper bound of the device bytes a request holds while it solves (the
+    library's setup arrays, its setup temporaries and the Newton/Krylov
+    workspace), from the mesh counts.  Hexahedral meshes: at most 27 Jacobian
+    entries per row."""
+    tv = 4 if krylov_fp32 else 8
+    ctx = 333 * nc + 353 * nf + 96 * nv + 36 * nbnd
+    setup_tmp = 768 * nc + 56 * nf + 200 * nv
+    ws = 64 * nc + 216 * nc + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
+    if krylov_fp32:
+        ws += 108 * nc
+    return int(ctx + max(setup_tmp, ws) + (1 << 20))
+
+
+class _Job:
+    __slots__ = ("req", "future", "t_arrive", "t_built", "t_admit", "t_done", "problem", "need")
+
+    def __init__(self, req, future):
+        self.req = req
+        self.future = future
+        self.t_arrive = time.perf_counter()
+        self.t_built = self.t_admit = self.t_done = None
+        self.problem = None
+        self.need = 0
+
+
+class EnsembleServer:
+    """Serve Richards solve requests with continuous admission on one GPU.
+
+    slots        concurrent requests on the device (one host thread and one
+                 HIP stream each)
+    hbm_budget   bytes of device memory the in-flight requests may hold;
+                 default 80 % of the free HBM at start
+    build_workers  host threads generating problems
+    solve_fn     (problem, request) -> (pressure ndarray, stats dict); the
+                 default runs the device Newton solver
+    """
+
+    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, build_workers: int = 8,
+                 krylov_fp32: bool = True, snes_rtol: float = 1e-10, brick=(8, 8, 4),
+                 solve_fn: Optional[Callable] = None):
+        self.slots = slots
+        self.brick = tuple(brick)
+        self.krylov_fp32 = krylov_fp32
+        self.snes_rtol = snes_rtol
+        if hbm_budget is None:
+            import torch
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self._solve = solve_fn or self._gpu_solve
+        self._build = cf.ThreadPoolExecutor(max_workers=build_workers, thread_name_prefix="build")
+        self._ready: "queue.Queue" = queue.Queue()
+        self._cv = threading.Condition()
+        self._used = 0
+        self._inflight = 0
+        self.peak_used = 0
+        self.peak_inflight = 0
+        self._closed = False
+        self._workers = [threading.Thread(target=self._slot, args=(i,), daemon=True, name=f"slot{i}")
+                         for i in range(slots)]
+        for w in self._workers:
+            w.start()
+
+    # ---- client side ----
+    def submit(self, req: Request) -> cf.Future:
+        """Queue a request; the future resolves to a result dict."""
+        fut: cf.Future = cf.Future()
+        job = _Job(req, fut)
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server is closed")
+        self._build.submit(self._build_job, job)
+        return fut
+
+    def close(self):
+        """Finish the queued requests and stop the slot threads."""
+        self._build.shutdown(wait=True)
+        with self._cv:
+            self._closed = True
+        for _ in self._workers:
+            self._ready.put(None)
+        for w in self._workers:
+            w.join()
+
+    # ---- stages ----
+    def _build_job(self, job: _Job):
+        r = job.req
+        try:
+            P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
+            m = P["mesh"]
+            rows = block_rows(r.nx, r.ny, r.nz, self.brick)
+            job.need = footprint_bytes(m.nc, m.nf, m.nv, m.nbnd, rows, self.krylov_fp32)
+            if job.need > self.hbm_budget:
+                raise MemoryError(f"request {r.rid} needs {job.need} B, above the HBM budget {self.hbm_budget} B")
+            job.problem = P
+            job.t_built = time.perf_counter()
+            self._ready.put(job)
+        except BaseException as e:  # noqa: BLE001 - delivered to the caller's future
+            job.future.set_exception(e)
+
+    def _admit(self, job: _Job):
+        with self._cv:
+            while self._used + job.need > self.hbm_budget:
+                self._cv.wait()
+            self._used += job.need
+            self._inflight += 1
+            self.peak_used = max(self.peak_used, self._used)
+            self.peak_inflight = max(self.peak_inflight, self._inflight)
+        job.t_admit = time.perf_counter()
+
+    def _release(self, job: _Job):
+        with self._cv:
+            self._used -= job.need
+            self._inflight -= 1
+            self._cv.notify_all()
+
+    def _slot(self, wid: int):
+        while True:
+            job = self._ready.get()
+            if job is None:
+                return
+            self._admit(job)
+            try:
+                U, st = self._solve(job.problem, job.req)
+                job.t_done = time.perf_counter()
+                job.future.set_result({
+                    "rid": job.req.rid, "nc": job.problem["mesh"].nc, "U": U, "stats": st,
+                    "latency_s": job.t_done - job.t_arrive, "build_s": job.t_built - job.t_arrive,
+                    "queue_s": job.t_admit - job.t_built, "solve_s": job.t_done - job.t_admit,
+                    "t_arrive": job.t_arrive, "t_done": job.t_done, "slot": wid})
+            except BaseException as e:  # noqa: BLE001
+                job.future.set_exception(e)
+            finally:
+                job.problem = None
+                self._release(job)
+
+    def _gpu_solve(self, P, req: Request):
+        import torch
+        from .mpfao import MPFAO
+        g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        try:
+            g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
+            bs = self.brick[0] * self.brick[1] * self.brick[2]
+            o = g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs)
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
+            s.synchronize()
+            st = g.run_snes_steps(U, req.dt, req.steps, o)
+            with torch.cuda.stream(s):
+                out = U.cpu().numpy()
+            st["device_bytes"] = g.device_bytes()
+            return out, st
+        finally:
+            g.close()
+
+
+def latency_summary(results):
+    """p50/p99 latency and throughput over a list of result dicts."""
+    lat = np.array([r["latency_s"] for r in results])
+    t0 = min(r["t_arrive"] for r in results)
+    t1 = max(r["t_done"] for r in results)
+    return {"requests": len(results), "solves_per_s": len(results) / (t1 - t0),
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "makespan_s": t1 - t0,
+            "mean_solve_s": float(np.mean([r["solve_s"] for r in results])),
+            "mean_queue_s": float(np.mean([r["queue_s"] for r in results])),
+            "mean_build_s": float(np.mean([r["build_s"] for r in results]))}

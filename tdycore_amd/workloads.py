@@ -48,3 +48,34 @@ def layered_box_problem(nx=128, ny=128, nz=64, seed=20261203, lx=1000.0, ly=1000
     U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
     kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
     return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+
+
+def ensemble_box_problem(nx, ny, nz, seed, sigma_log10, jitter=0.1):
+    """One request of the serving ensemble: the unit cube in nx*ny*nz hexes,
+    jittered as the unit-cube problem, with its own seed for the rotated
+    log-normal tensor permeability (log10 k_i ~ N(-12, sigma_log10));
+    phi ~ U(0.2, 0.4), Sr 0.1, van Genuchten n = 1.5, alpha 1e-4; hydrostatic
+    initial state with the water table at z = 0.5, Dirichlet top and bottom
+    at the hydrostatic values, no-flow sides."""
+    rng = np.random.default_rng(seed)
+    m = mg.box_mesh(nx, ny, nz, jitter=jitter, seed=seed + 1)
+    K = mg.random_rotated_K(m.nc, rng, -12.0, sigma_log10)
+    phi = rng.uniform(0.2, 0.4, size=m.nc)
+    Sr = np.full(m.nc, 0.1)
+    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-", "x+", "y-", "y+"])
+    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 0.5))
+    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], 0.5)
+    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+
+
+def ensemble_request_specs(n=512, seed=20261206):
+    """Sizes and seeds of the serving ensemble: nx, ny ~ U{16..96},
+    nz ~ U{8..48}, sigma(log10 k) ~ U(0.5, 1.5), one seed per request."""
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(16, 97, size=n)
+    ny = rng.integers(16, 97, size=n)
+    nz = rng.integers(8, 49, size=n)
+    sig = rng.uniform(0.5, 1.5, size=n)
+    return [dict(rid=i, nx=int(nx[i]), ny=int(ny[i]), nz=int(nz[i]), seed=int(seed + 1000 * (i + 1)),
+                 sigma=float(sig[i])) for i in range(n)]

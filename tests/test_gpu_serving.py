@@ -1,0 +1,52 @@
+"""Ensemble serving on the device: each result is the same alone and in the
+mix (1e-8 relative), matches the oracle, and the admission footprint bounds
+what the library actually holds."""
+import numpy as np
+import pytest
+
+from tdycore_amd import serving, workloads
+
+pytestmark = pytest.mark.gpu
+
+REQS = [serving.Request(rid=i, nx=nx, ny=ny, nz=nz, seed=7000 + 17 * i, sigma=s, steps=2)
+        for i, (nx, ny, nz, s) in enumerate([(16, 16, 8, 0.5), (24, 16, 8, 1.5), (16, 20, 12, 1.0),
+                                             (20, 24, 10, 0.8), (17, 19, 9, 1.2)])]
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def test_alone_equals_mix():
+    alone = {}
+    srv = serving.EnsembleServer(slots=1, build_workers=1)
+    for r in REQS:
+        alone[r.rid] = srv.submit(r).result(timeout=120)
+    srv.close()
+    srv = serving.EnsembleServer(slots=4, build_workers=4)
+    futs = [srv.submit(r) for r in REQS]
+    mix = {f.result(timeout=120)["rid"]: f.result() for f in futs}
+    srv.close()
+    assert srv.peak_inflight >= 2
+    for r in REQS:
+        a, m = alone[r.rid], mix[r.rid]
+        assert a["stats"]["status"] == 0 and m["stats"]["status"] == 0
+        assert _rel(m["U"], a["U"]) <= 1e-8, r
+        P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
+        mesh = P["mesh"]
+        est = serving.footprint_bytes(mesh.nc, mesh.nf, mesh.nv, mesh.nbnd,
+                                      serving.block_rows(r.nx, r.ny, r.nz, (8, 8, 4)))
+        assert a["stats"]["device_bytes"] <= est, (a["stats"]["device_bytes"], est)
+
+
+def test_served_result_matches_oracle():
+    from pyoracle import Oracle
+    r = REQS[0]
+    srv = serving.EnsembleServer(slots=2, build_workers=2)
+    res = srv.submit(r).result(timeout=120)
+    srv.close()
+    P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    U, st = o.run_snes_steps(P["U0"], r.dt, r.steps, lin="gmres", rtol=1e-10)
+    assert st[3] == 0
+    assert _rel(res["U"], U) <= 1e-8
