@@ -150,6 +150,18 @@ def serve(a):
     rng = np.random.default_rng(20261207)
     gaps = rng.exponential(1.0 / a.rate, size=len(specs))
     futs = []
+    ndone = [0]
+
+    def report(f):
+        ndone[0] += 1
+        if f.exception() is None:
+            r = f.result()
+            st = r["stats"]
+            log(f"done {ndone[0]}: rid {r['rid']} cells {r['nc']} newton {st['newton_its']} linear {st['linear_its']} "
+                f"steps {st['steps']} cuts {st['cuts']} solve {r['solve_s']:.2f}s latency {r['latency_s']:.2f}s")
+        else:
+            log(f"done {ndone[0]}: failed {f.exception()}")
+
     t0 = time.perf_counter()
     t_next = t0
     for sp, gap in zip(specs, gaps):
@@ -158,8 +170,7 @@ def serve(a):
         if dt > 0:
             time.sleep(dt)
         futs.append(srv.submit(serving.Request(**sp)))
-        if len(futs) % 64 == 0:
-            log(f"submitted {len(futs)} requests, {sum(f.done() for f in futs)} done")
+        futs[-1].add_done_callback(report)
     res = [f.result() for f in futs]
     srv.close()
     summ = serving.latency_summary(res)

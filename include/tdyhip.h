@@ -114,14 +114,17 @@ typedef struct {
   double ksp_rtol, ksp_atol;              /* 1e-5, 1e-50 (preconditioned residual) */
   int32_t ksp_max_it, ksp_restart;        /* 10000, 30 */
   int32_t krylov_fp32;                    /* 0: FP64 Jacobian + Krylov; 1: FP32 Jacobian copy,
-                                             FP32 ILU(0) and Krylov vectors, FP32 accumulation */
+                                             FP32 ILU(0) and Krylov vectors, FP32 accumulation,
+                                             FP64 repeat of a failed FP32 solve */
   int32_t pc_block;                       /* rows per block-Jacobi ILU(0) block, 1..256 */
 } tdyhip_solver_opts;
 
 typedef struct {
   int64_t newton_its, linear_its, steps, cuts;
   int32_t status;                         /* 0 converged, <0 SNESConvergedReason-like failure */
-  int32_t pad;
+  int32_t krylov_fallbacks;               /* FP32 linear solves repeated in FP64 (krylov_fp32 = 1:
+                                             an FP32 GMRES gets at most 1000 iterations; if it does
+                                             not converge the correction is solved again in FP64) */
   double last_fnorm;
 } tdyhip_solver_stats;
 

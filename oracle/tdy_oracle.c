@@ -1142,7 +1142,9 @@ static int gmres_ilu(const ora_ctx *ctx, const double *J, const double *F, doubl
     }
     for (int i = 0; i < j; i++) { const double *vi = V + (size_t)i * n; for (int q = 0; q < n; q++) Y[q] += yk[i] * vi[q]; }
     free(yk);
-    if (fabs(g[j]) <= tol || its >= 10000) break;
+    if (fabs(g[j]) <= tol) break;
+    /* KSP_DIVERGED_ITS is a linear-solve failure for SNES (SNESCheckKSPSolve) */
+    if (its >= 10000) { status = 2; break; }
   }
 done:
   *its_out = its;

@@ -55,7 +55,7 @@ class SolverOpts(C.Structure):
 
 class SolverStats(C.Structure):
     _fields_ = [("newton_its", C.c_int64), ("linear_its", C.c_int64), ("steps", C.c_int64),
-                ("cuts", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32), ("last_fnorm", C.c_double)]
+                ("cuts", C.c_int64), ("status", C.c_int32), ("krylov_fallbacks", C.c_int32), ("last_fnorm", C.c_double)]
 
 
 def load():

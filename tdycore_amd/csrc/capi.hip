@@ -108,13 +108,15 @@ static double bytes_rows(const tdyhip_ctx *ctx, bool R, bool J) {
   double b = (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (8 + 4) + (double)ctx->nf * (8 + 1) +
              (double)ctx->nv * kMaxCols * 4 + (double)ctx->nc * 8 * 3 + (double)ctx->nbnd * 8 * 3;
   if (R) b += (double)ctx->active_rows * 8;
-  if (J) b += (double)ctx->jnnz * 8 + (double)ctx->active_rows * 2 + (double)ctx->nv * 64 + (double)ctx->nc * 8 * 2;
+  if (J) b += (double)ctx->trans_nnz * 8 + (double)ctx->nc * 8 * 2;  // row derivatives out; drho, dukvr
   return b;
 }
 static double bytes_gather(const tdyhip_ctx *ctx, bool R, bool J) {
-  double b = 0;
-  if (R) b += (double)ctx->nc * (24 + 8 + 8) + (double)ctx->nf * (32 + 8);
-  if (J) b += (double)ctx->nc * (8 * 4 + 1 + 16);
+  double b = (double)ctx->nc * 24 + (double)ctx->nf * 8;  // cell faces, face cells
+  if (R) b += (double)ctx->nc * (8 + 8) + (double)ctx->nf * 32;
+  if (J)
+    b += (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (4 + 1) + (double)ctx->nf +
+         (double)ctx->nv * (kMaxCols * 4 + 64) + (double)ctx->nc * (8 * 4 + 1) + (double)ctx->jnnz * 8;
   return b;
 }
 
@@ -167,15 +169,27 @@ static int check_stream(tdyhip_ctx *ctx, tdyhip_stream s, hipStream_t *out) {
   return 0;
 }
 
-static RowArgs row_args(tdyhip_ctx *ctx, const double *U, double *J, bool ell) {
+static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
   RowArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
   a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts; a.face_flags = ctx->d_face_flags;
   a.vcols = ctx->d_vcols; a.T = ctx->d_T; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd;
   a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
-  a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
-  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? (int64_t)ctx->nc : 1;
-  a.flux = ctx->d_flux; a.J = J;
+  a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
+  a.flux = ctx->d_flux; a.Jrow = ctx->d_jrow;
+  return a;
+}
+
+static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const double *Udot, double *R, double *J,
+                              bool ell) {
+  GatherArgs a;
+  a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
+  a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts;
+  a.vcols = ctx->d_vcols; a.face_flags = ctx->d_face_flags; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
+  a.jdslot = ctx->d_jdslot; a.flux = ctx->d_flux; a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
+  a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
+  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? (int64_t)ctx->nc : 1; a.ell_w = ctx->jell_w;
+  a.R = R; a.J = J;
   return a;
 }
 
@@ -195,32 +209,25 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
 
 // rows + gather.  want_R / want_J select the outputs.
 // ell = true: J is the solver's slot-major ELL array (jell_w*nc), else CSR values.
+// Every entry of J is written (no prior clear is needed).
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
                        double *J, hipStream_t st, bool ell = false) {
   const int64_t nrows = (int64_t)4 * ctx->nf;
-  RowArgs a = row_args(ctx, U, J, ell);
-  const int64_t jlen = ell ? (int64_t)ctx->jell_w * ctx->nc : ctx->jnnz;
-  const int32_t *jrp = ell ? nullptr : ctx->d_jrp;
-  if (J) k_fill<<<nblk(jlen, 256), 256, 0, st>>>(jlen, J, 0.0);
+  RowArgs a = row_args(ctx, U);
   if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
   else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a));
   else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
-  const unsigned g = nblk(ctx->nc, 256);
+  GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell);
+  const unsigned g = nblk(ctx->nc, kGatherBlock);
   if (R && J)
     KT(ctx, st, "k_cell_gather_RJ", bytes_gather(ctx, true, true),
-       k_cell_gather<true, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                    ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                    Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
+       k_cell_gather<true, true><<<g, kGatherBlock, 0, st>>>(ga));
   else if (R)
     KT(ctx, st, "k_cell_gather_R", bytes_gather(ctx, true, false),
-       k_cell_gather<true, false><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                     ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                     Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
+       k_cell_gather<true, false><<<g, kGatherBlock, 0, st>>>(ga));
   else
     KT(ctx, st, "k_cell_gather_J", bytes_gather(ctx, false, true),
-       k_cell_gather<false, true><<<g, 256, 0, st>>>(ctx->nc, form, shift, ctx->dtime, ctx->d_cell_faces,
-                                                     ctx->d_face_cells, ctx->d_flux, ctx->d_acc, ctx->d_dM, ctx->d_d2M,
-                                                     Udot, ctx->d_cell_vol, jrp, ctx->d_jdslot, a.jstride, R, J));
+       k_cell_gather<false, true><<<g, kGatherBlock, 0, st>>>(ga));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -296,6 +303,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
       dslot[c] = (uint8_t)(topo.jdiag[c] - topo.jrp[c]);
       W = std::max(W, topo.jrp[c + 1] - topo.jrp[c]);
     }
+    if (W > kJacSlots) { set_error("more than 27 Jacobian entries in a row"); return fail(TDYHIP_ERR_ARG); }
     ctx->jell_w = W;
     std::vector<int32_t> ecols((size_t)W * nc);
     for (int c = 0; c < nc; c++)
@@ -314,6 +322,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   for (double **p : {&ctx->d_Pbnd, &ctx->d_ukvr_bnd, &ctx->d_rho_bnd})
     if ((rc = dalloc(ctx, p, nbnd))) return fail(rc);
   if ((rc = dalloc(ctx, &ctx->d_flux, (size_t)nrows))) return fail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_jrow, (size_t)kMaxCols * nrows))) return fail(rc);
 
   // setup temporaries
   int32_t *d_scf = nullptr; IRDesc *d_ir = nullptr; double *d_Gsub = nullptr, *d_K = nullptr, *d_cc = nullptr,

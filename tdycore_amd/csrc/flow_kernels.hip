@@ -108,12 +108,8 @@ struct RowArgs {
   const double *Pbnd;
   const double *rho, *ukvr, *drho, *dukvr;
   const double *rho_bnd, *ukvr_bnd;
-  const uint8_t *row_ul;
-  const uint8_t *jslot;
-  const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
-  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or n (ELL)
   double *flux;
-  double *J;
+  double *Jrow;  // kMaxCols * nrows: d(flux row r)/d(p of column k) at Jrow[k*nrows + r]
 };
 
 template <bool WITH_R, bool WITH_J>
@@ -174,14 +170,11 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     double dukvr_up = 0.0, dukvr_dn = 0.0;
     if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
     else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
-    const uint8_t ul = a.row_ul[2 * r], dl = a.row_ul[2 * r + 1];
-    const uint8_t *slot = a.jslot + (int64_t)v * 64;
-    const int64_t rup = up >= 0 ? (a.jrp ? a.jrp[up] : up) : 0, rdn = dn >= 0 ? (a.jrp ? a.jrp[dn] : dn) : 0;
 #pragma unroll
     for (int k = 0; k < kMaxCols; k++) {
       const int col = ck[k];
-      if (col < 0 || col >= a.nc) continue;
-      double Jac;
+      double Jac = 0.0;
+      if (col < 0 || col >= a.nc) continue;  // padding or boundary-pressure column: never enters J
       if (zero) Jac = 0.0;
       else if (up >= 0 && col == up)
         Jac = dden_up * ukvr * TP + den * dukvr_up * TP + den * ukvr * Tk[k] + 2 * den * dden_up * ukvr * G +
@@ -191,37 +184,80 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
               (den * den) * dukvr_dn * G;
       else
         Jac = den * ukvr * Tk[k];
-      Jac = -Jac;
-      if (up >= 0) unsafeAtomicAdd(a.J + rup + (int64_t)slot[ul * 8 + k] * a.jstride, Jac);
-      if (dn >= 0) unsafeAtomicAdd(a.J + rdn + (int64_t)slot[dl * 8 + k] * a.jstride, -Jac);
+      // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
+      a.Jrow[(int64_t)k * a.nrows + r] = -Jac;
     }
   }
 }
 
+struct GatherArgs {
+  int nc;
+  int64_t nrows;
+  int form;  // 0 TS, 1 SNES
+  double shift, dtime;
+  const int32_t *cell_faces, *face_cells, *face_verts, *vcols;
+  const uint8_t *face_flags, *row_ul, *jslot, *jdslot;
+  const double *flux, *acc, *dM, *d2M, *Udot, *vol, *Jrow;
+  const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
+  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or nc (ELL)
+  int ell_w;           // ELL width (slots written per row in the ELL layout)
+  double *R, *J;
+};
+
+constexpr int kGatherBlock = 128;
+constexpr int kJacSlots = 27;  // vertex-adjacency row length of a hexahedral mesh
+
+// Per-cell gather: R[c] sums the fluxes of the cell's faces (+ accumulation);
+// J's row c sums, in a fixed order (face, sub-face, column), the flux-row
+// derivatives of the cell's faces into the row's slots in LDS, adds the
+// accumulation diagonal and writes the whole row once.  Every cell owns its
+// residual entry and its Jacobian row, so the result does not depend on
+// scheduling.
 template <bool WITH_R, bool WITH_J>
-__global__ void k_cell_gather(int nc, int form, double shift, double dtime, const int32_t *__restrict__ cell_faces,
-                              const int32_t *__restrict__ face_cells, const double *__restrict__ flux,
-                              const double *__restrict__ acc, const double *__restrict__ dM,
-                              const double *__restrict__ d2M, const double *__restrict__ Udot,
-                              const double *__restrict__ vol, const int32_t *__restrict__ jrp,
-                              const uint8_t *__restrict__ jdslot, int64_t jstride, double *__restrict__ R,
-                              double *__restrict__ J) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+__global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
+  __shared__ double acc[WITH_J ? kJacSlots : 1][kGatherBlock];
+  const int t = threadIdx.x;
+  const int c = blockIdx.x * kGatherBlock + t;
+  if (c >= a.nc) return;
   if (WITH_R) {
     double s = 0.0;
 #pragma unroll
     for (int lf = 0; lf < 6; lf++) {
-      const int f = cell_faces[c * 6 + lf];
-      const double4 q = *reinterpret_cast<const double4 *>(flux + (int64_t)f * 4);
+      const int f = a.cell_faces[c * 6 + lf];
+      const double4 q = *reinterpret_cast<const double4 *>(a.flux + (int64_t)f * 4);
       const double fs = ((q.x + q.y) + q.z) + q.w;
-      s += (face_cells[2 * f] == c) ? fs : -fs;
+      s += (a.face_cells[2 * f] == c) ? fs : -fs;
     }
-    R[c] = s + acc[c];
+    a.R[c] = s + a.acc[c];
   }
   if (WITH_J) {
-    const double v = (form == 0) ? (shift * dM[c] + d2M[c] * Udot[c]) * vol[c] : dM[c] * vol[c] * (1.0 / dtime);
-    J[(jrp ? jrp[c] : (int64_t)c) + (int64_t)jdslot[c] * jstride] += v;
+#pragma unroll
+    for (int sl = 0; sl < kJacSlots; sl++) acc[sl][t] = 0.0;
+    for (int lf = 0; lf < 6; lf++) {
+      const int f = a.cell_faces[c * 6 + lf];
+      if (a.face_flags[f] & kFaceNoFlow) continue;
+      const bool is_up = a.face_cells[2 * f] == c;
+      for (int q = 0; q < 4; q++) {
+        const int64_t r = (int64_t)f * 4 + q;
+        const int v = a.face_verts[r];
+        const int li = a.row_ul[2 * r + (is_up ? 0 : 1)];
+        const int32_t *cols = a.vcols + (int64_t)v * kMaxCols;
+        const uint8_t *slot = a.jslot + (int64_t)v * 64 + li * 8;
+#pragma unroll
+        for (int k = 0; k < kMaxCols; k++) {
+          const int col = cols[k];
+          if (col < 0 || col >= a.nc) continue;
+          const double d = a.Jrow[(int64_t)k * a.nrows + r];
+          acc[slot[k]][t] += is_up ? d : -d;
+        }
+      }
+    }
+    const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
+                                    : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
+    acc[a.jdslot[c]][t] += dv;
+    const int64_t base = a.jrp ? a.jrp[c] : c;
+    const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
+    for (int sl = 0; sl < len; sl++) a.J[base + (int64_t)sl * a.jstride] = acc[sl][t];
   }
 }
 

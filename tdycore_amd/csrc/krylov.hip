@@ -391,6 +391,13 @@ __global__ void k_gemv_update_dev(int64_t n, int mmax, const TV *__restrict__ V,
   x[i] += (TX)s;
 }
 
+// out = (TV)(a - b), the difference taken in FP64
+template <class TV>
+__global__ void k_sub_to(int64_t n, const double *__restrict__ a, const double *__restrict__ b, TV *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TV)(a[i] - b[i]);
+}
+
 // w = x - lambda*y
 __global__ void k_waxpy(int64_t n, double lambda, const double *__restrict__ x, const double *__restrict__ y,
                         double *__restrict__ w) {

@@ -225,7 +225,9 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   ws.n = n; ws.restart = o.ksp_restart; ws.fp32 = fp32; ws.bs = o.pc_block; ws.nnz = nnz;
   ws.nchunk = (int)((n + kChunk - 1) / kChunk);
   int rc = 0;
-  const size_t tv = fp32 ? sizeof(float) : sizeof(double);
+  // Krylov buffers are sized for FP64 in both modes: an FP32 solve that fails
+  // is repeated in FP64 on the same buffers.
+  const size_t tv = sizeof(double);
   auto up = [&](auto **p, const auto &vec) -> int {
     int r = ws_alloc(ws, p, vec.size());
     if (r) return r;
@@ -313,7 +315,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   const int n = ws.n, m = o.ksp_restart;
   const int64_t N = n;
   TV *V = (TV *)ws.V, *w = (TV *)ws.w, *z = (TV *)ws.z, *b = (TV *)ws.b, *x = (TV *)ws.x, *fac = (TV *)ws.fac;
-  TV *Jk = (TV *)ws.Jk, *part = (TV *)ws.partial;
+  TV *Jk = std::is_same<TV, double>::value ? (TV *)ws.J : (TV *)ws.Jk, *part = (TV *)ws.partial;
   GmresDev *gd = ws.gdev;
   double *dH = ws.hs, *dcs = dH + (size_t)(m + 1) * m, *dsn = dcs + m, *dg = dsn + m, *dy = dg + (m + 1);
   const int *stop = &gd->done;
@@ -332,8 +334,13 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
   };
   k_convert<double, TV><<<g256, 256, 0, st>>>(N, bvec, b);
-  if (std::is_same<TV, double>::value) x = (TV *)yvec;  // x accumulates the solution in place
-  k_scale_copy<TV><<<g256, 256, 0, st>>>(N, (TV)0, b, x);
+  // The solution accumulates in FP64 in place (yvec) in both precisions, and
+  // every restart recomputes the true residual b - J x with the FP64 Jacobian
+  // (mixed-precision restarted GMRES): the FP32 Arnoldi cycles only ever
+  // solve for a correction, so the FP32 form stops on the same residual as
+  // the FP64 one.
+  (void)x;
+  k_scale_copy<double><<<g256, 256, 0, st>>>(N, 0.0, bvec, yvec);
   // r0 = M^-1 b  (x0 = 0)
   pc_apply(b, w, nullptr);
   norm2_to_h0(w);
@@ -354,23 +361,25 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V + (int64_t)(j + 1) * N, gd);
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
-    k_gemv_update_dev<TV, TV><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, x);
+    k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
     SCHK(hipGetLastError());
     SCHK(hipMemcpyAsync(&hg, gd, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
     if (cycle == 0) SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.converged || hg.its >= o.ksp_max_it) break;
-    // restart: w = M^-1 (b - A x)
-    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(TV)), k_spmv_ell<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, x, z));
-    k_sub<TV><<<g256, 256, 0, st>>>(N, b, z, z);
+    // restart: w = M^-1 (b - A x), the residual in FP64
+    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(double)), k_spmv_ell<double><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, yvec, ws.JY));
+    k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z);
     pc_apply(z, w, nullptr);
     norm2_to_h0(w);
     k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 0);
   }
-  if (!std::is_same<TV, double>::value) k_convert<TV, double><<<g256, 256, 0, st>>>(N, x, yvec);
   SCHK(hipGetLastError());
   *its_out = hg.its;
+  // KSP_DIVERGED_ITS: PETSc's SNES counts it as a linear-solve failure
+  // (SNESCheckKSPSolve, maxLinearSolveFailures 1 -> SNES_DIVERGED_LINEAR_SOLVE)
+  if (!hg.converged) { set_error("GMRES reached ksp_max_it"); return TDYHIP_ERR_NOCONV; }
   return 0;
 }
 
@@ -412,9 +421,23 @@ static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, h
   return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true);
 }
 
+// FP32 Krylov: the FP32 attempt gets at most kFp32MaxIt iterations; if it
+// does not converge (or its ILU(0) meets a zero pivot), the same Newton
+// correction is solved again in FP64 from scratch.
+constexpr int kFp32MaxIt = 1000;
 static int linsolve(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *b, double *y, int *its,
-                    hipStream_t st) {
-  return o.krylov_fp32 ? gmres<float>(ctx, o, b, y, its, st) : gmres<double>(ctx, o, b, y, its, st);
+                    int *fallbacks, hipStream_t st) {
+  if (!o.krylov_fp32) return gmres<double>(ctx, o, b, y, its, st);
+  tdyhip_solver_opts o32 = o;
+  o32.ksp_max_it = std::min(o.ksp_max_it, kFp32MaxIt);
+  int its32 = 0;
+  int rc = gmres<float>(ctx, o32, b, y, &its32, st);
+  if (rc != TDYHIP_ERR_NOCONV && rc != TDYHIP_ERR_SINGULAR) { *its = its32; return rc; }
+  (*fallbacks)++;
+  int its64 = 0;
+  rc = gmres<double>(ctx, o, b, y, &its64, st);
+  *its = its32 + its64;
+  return rc;
 }
 
 // SNESSolve_NEWTONLS with SNESLINESEARCHBT and SNESConvergedDefault
@@ -437,9 +460,9 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
   for (it = 1; it <= o.snes_max_it; it++) {
     if ((rc = eval_J(ctx, fm, X, ws.J, st))) return rc;
     int lits = 0;
-    rc = linsolve(ctx, o, ws.F, ws.Y, &lits, st);
+    rc = linsolve(ctx, o, ws.F, ws.Y, &lits, &stats.krylov_fallbacks, st);
     stats.linear_its += lits;
-    if (rc == TDYHIP_ERR_SINGULAR) { status = -3; break; }
+    if (rc == TDYHIP_ERR_SINGULAR || rc == TDYHIP_ERR_NOCONV) { status = -3; break; }
     if (rc) return rc;
     double ynorm;
     if ((rc = dev_norm(ctx, ws.Y, &ynorm, st))) return rc;

@@ -183,7 +183,8 @@ class MPFAO:
     @staticmethod
     def _stats_dict(st):
         return {"newton_its": st.newton_its, "linear_its": st.linear_its, "steps": st.steps,
-                "cuts": st.cuts, "status": st.status, "last_fnorm": st.last_fnorm}
+                "cuts": st.cuts, "status": st.status, "krylov_fallbacks": st.krylov_fallbacks,
+                "last_fnorm": st.last_fnorm}
 
     def newton_snes(self, U, opts=None):
         o = opts or self.solver_options()

@@ -11,8 +11,10 @@ stages:
    and its device footprint fits in the HBM budget beside the requests
    already on the device;
 3. **solve** - a slot thread sets up the transmissibilities on the device and
-   runs the backward-Euler steps.  Each step is a pre-solve plus a Newton
-   solve: FP64 residual, FP32-Krylov inner solve.  The request's state stays
+   runs the backward-Euler steps to the request's final time with the
+   time-integrator rules of ``TDyTimeIntegratorRunToTime`` (src/tdyti.c:71-196):
+   a Newton failure cuts dt and retries.  Each step is a pre-solve plus a
+   Newton solve: FP64 residual, FP32-Krylov inner solve.  The request's state stays
    in HBM from setup to its last step.  Each slot drives its own
    non-blocking HIP stream, so the setup and Newton kernels of different
    requests overlap on the device, and a request that arrives while others
@@ -62,8 +64,8 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
     library's setup arrays, its setup temporaries and the Newton/Krylov
     workspace), from the mesh counts.  Hexahedral meshes: at most 27 Jacobian
     entries per row."""
-    tv = 4 if krylov_fp32 else 8
-    ctx = 333 * nc + 353 * nf + 96 * nv + 36 * nbnd
+    tv = 8  # Krylov buffers are sized for FP64 (the fallback of a failed FP32 solve)
+    ctx = 333 * nc + 609 * nf + 96 * nv + 36 * nbnd
     setup_tmp = 768 * nc + 56 * nf + 200 * nv
     ws = 64 * nc + 216 * nc + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
     if krylov_fp32:
@@ -206,7 +208,9 @@ class EnsembleServer:
             with torch.cuda.stream(s):
                 U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
             s.synchronize()
-            st = g.run_snes_steps(U, req.dt, req.steps, o)
+            # TDyTimeIntegratorRunToTime semantics: steps of req.dt, a failed
+            # Newton solve cuts dt by 4 and regrows by 1.25 up to req.dt
+            st = g.run_tdyti(U, req.dt, 1.25, req.dt, req.steps * req.dt, o)
             with torch.cuda.stream(s):
                 out = U.cpu().numpy()
             st["device_bytes"] = g.device_bytes()

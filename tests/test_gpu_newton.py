@@ -50,3 +50,21 @@ def test_device_newton_matches_oracle_cfg0(fp32):
     err = np.abs(U.cpu().numpy() - Uo).max() / np.abs(Uo).max()
     print("max rel pressure err", err, st)
     assert err < 1e-8
+
+
+def test_device_newton_is_deterministic():
+    """Residual gather, Jacobian gather and every Krylov reduction run in a
+    fixed order: two solves from the same state agree bit for bit."""
+    print("[case] determinism cfg0 n=12", flush=True)
+    import torch
+    from tdycore_amd.mpfao import MPFAO
+    P = cfg0_problem(n=12)
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    for fp32 in (0, 1):
+        outs = []
+        for _ in range(2):
+            U = torch.tensor(P["U0"], device="cuda")
+            st = g.run_snes_steps(U, 3600.0, 2, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32))
+            assert st["status"] == 0
+            outs.append(U.cpu().numpy())
+        assert np.array_equal(outs[0], outs[1]), fp32

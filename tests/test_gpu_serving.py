@@ -9,7 +9,7 @@ from tdycore_amd import serving, workloads
 pytestmark = pytest.mark.gpu
 
 REQS = [serving.Request(rid=i, nx=nx, ny=ny, nz=nz, seed=7000 + 17 * i, sigma=s, steps=2)
-        for i, (nx, ny, nz, s) in enumerate([(16, 16, 8, 0.5), (24, 16, 8, 1.5), (16, 20, 12, 1.0),
+        for i, (nx, ny, nz, s) in enumerate([(16, 16, 8, 0.5), (24, 16, 8, 0.6), (16, 20, 12, 1.0),
                                              (20, 24, 10, 0.8), (17, 19, 9, 1.2)])]
 
 
@@ -32,6 +32,8 @@ def test_alone_equals_mix():
         a, m = alone[r.rid], mix[r.rid]
         assert a["stats"]["status"] == 0 and m["stats"]["status"] == 0
         assert _rel(m["U"], a["U"]) <= 1e-8, r
+        # every kernel reduces in a fixed order: the mix is bit-for-bit the same
+        assert np.array_equal(m["U"], a["U"]), r
         P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
         mesh = P["mesh"]
         est = serving.footprint_bytes(mesh.nc, mesh.nf, mesh.nv, mesh.nbnd,
@@ -47,6 +49,6 @@ def test_served_result_matches_oracle():
     srv.close()
     P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
     o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
-    U, st = o.run_snes_steps(P["U0"], r.dt, r.steps, lin="gmres", rtol=1e-10)
+    U, st = o.run_tdyti(P["U0"], r.dt, 1.25, r.dt, r.steps * r.dt, lin="gmres", rtol=1e-10)
     assert st[3] == 0
     assert _rel(res["U"], U) <= 1e-8
