@@ -116,7 +116,9 @@ typedef struct {
   int32_t krylov_fp32;                    /* 0: FP64 Jacobian + Krylov; 1: FP32 Jacobian copy,
                                              FP32 ILU(0) and Krylov vectors, FP32 accumulation,
                                              FP64 repeat of a failed FP32 solve */
-  int32_t pc_block;                       /* rows per block-Jacobi ILU(0) block, 1..256 */
+  int32_t pc_block;                       /* rows per block-Jacobi ILU(0) block, 1..512 */
+  int32_t hip_graphs;                     /* 1: replay each GMRES restart cycle as a captured
+                                             hipGraph (not while kernel timers run); 0: launch */
 } tdyhip_solver_opts;
 
 typedef struct {

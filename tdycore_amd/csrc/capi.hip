@@ -30,6 +30,15 @@ struct HipError {};
     }                                                                                     \
   } while (0)
 
+// Host<->device copies are ordered on the context's own (non-blocking) stream
+// and waited for there; the legacy null stream is never used, so a copy in one
+// context cannot interfere with a graph capture running in another thread.
+static hipError_t copy_sync(const tdyhip_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+
 template <class T>
 static int dalloc(tdyhip_ctx *ctx, T **p, size_t n) {
   *p = nullptr;
@@ -48,7 +57,7 @@ template <class T>
 static int dupload(tdyhip_ctx *ctx, T **p, const T *h, size_t n) {
   int rc = dalloc(ctx, p, n);
   if (rc) return rc;
-  if (n) HIPCHK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  if (n) HIPCHK(copy_sync(ctx, *p, h, n * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -338,7 +347,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
     hipError_t e_ = hipMalloc((void **)&ptr, (size_t)(n) * sizeof(*ptr));                           \
     if (e_ != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e_));          \
       return tmp_fail(e_ == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP); }               \
-    if (src) { e_ = hipMemcpy(ptr, src, (size_t)(n) * sizeof(*ptr), hipMemcpyHostToDevice);        \
+    if (src) { e_ = copy_sync(ctx, ptr, src, (size_t)(n) * sizeof(*ptr), hipMemcpyHostToDevice);   \
       if (e_ != hipSuccess) { set_error(std::string("hipMemcpy: ") + hipGetErrorString(e_)); return tmp_fail(TDYHIP_ERR_HIP); } } \
   } while (0)
   TMPALLOC(d_scf, topo.sc_faces.size(), topo.sc_faces.data());
@@ -370,7 +379,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
     return tmp_fail(TDYHIP_ERR_HIP);
   }
   int status = 0;
-  if (hipMemcpy(&status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
+  if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
   for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc, (void *)d_fa,
                   (void *)d_fn, (void *)d_status})
     (void)hipFree(p);
@@ -507,7 +516,7 @@ int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, doub
   const int64_t nrows = (int64_t)4 * ctx->nf;
   std::vector<double> T((size_t)kMaxCols * nrows);
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  HIPCHK(hipMemcpy(T.data(), ctx->d_T, T.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(ctx, T.data(), ctx->d_T, T.size() * sizeof(double), hipMemcpyDeviceToHost));
   int64_t n = 0;
   rowptr[0] = 0;
   for (int64_t r = 0; r < nrows; r++) {
@@ -533,7 +542,7 @@ int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, doub
 int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out) {
   if (!ctx || !out) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  HIPCHK(hipMemcpy(out, ctx->d_G, sizeof(double) * 4 * ctx->nf, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(ctx, out, ctx->d_G, sizeof(double) * 4 * ctx->nf, hipMemcpyDeviceToHost));
   return 0;
 }
 

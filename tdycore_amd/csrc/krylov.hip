@@ -7,8 +7,8 @@
 // parallel; driven from src/tdyti.c:135,188).
 //
 // Block-Jacobi ILU(0): cells are partitioned (host) into compact blocks of at
-// most BS cells (BS = 64: one wavefront per block; BS = 256: one 4-wave
-// workgroup per block).  Inside a block rows keep their global order;
+// most BS cells (BS = 64: one wavefront per block; BS = 256 or 512: one 4- or
+// 8-wave workgroup per block; at 512 the FP64 factor holds 137 KB of LDS).  Inside a block rows keep their global order;
 // entries coupling different blocks are dropped.  The factorization (IKJ)
 // and the two triangular solves advance level by level (levels of the
 // in-block dependency DAG, computed on the host) with the block's rows in
@@ -18,20 +18,20 @@
 
 namespace tdyhip {
 
-constexpr int kBlkMax = 256;  // max rows per block
+constexpr int kBlkMax = 512;  // max rows per block
 constexpr int kW = 27;        // max in-block entries per row (hexahedral vertex adjacency)
 constexpr int kLU = 13;       // max strictly lower (upper) entries per row
 // Factored block layout per row (slot-major, stride BS): slots 0..12 L
 // multipliers, 13..25 U entries, 26 the inverse diagonal; padded L/U slots hold
 // 0 and point at the zero entry x[BS] of the apply kernel's LDS vector.
 
-// Per-block arrays are strided by the block width BS (64 or 256).
+// Per-block arrays are strided by the block width BS (64, 256 or 512).
 struct BJLayout {
   int nblocks = 0;
-  int bs = 0;                  // stride / launch width (64 or 256)
+  int bs = 0;                  // stride / launch width (64, 256 or 512)
   int32_t *d_row = nullptr;    // nblocks*bs global row (-1 pad)
   int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian (-1 pad), slot-major
-  uint8_t *d_lcol = nullptr;   // nblocks*kW*bs local column (255 pad)
+  uint16_t *d_lcol = nullptr;  // nblocks*kW*bs local column (0xffff pad)
   uint8_t *d_diag = nullptr;   // nblocks*bs slot of the diagonal
   uint8_t *d_nslot = nullptr;  // nblocks*bs number of slots
   uint16_t *d_levL = nullptr;  // nblocks*bs
@@ -52,7 +52,7 @@ template <class TV, int BS>
 __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__restrict__ J, TV *__restrict__ fac,
                                                   int *__restrict__ status) {
   __shared__ TV a[kW][BS];
-  __shared__ uint8_t lc[kW][BS];
+  __shared__ uint16_t lc[kW][BS];
   __shared__ uint8_t dg[BS];
   __shared__ uint8_t ns[BS];
   const int b = blockIdx.x, t = threadIdx.x;

@@ -15,8 +15,8 @@ namespace tdyhip {
 struct HostBJ {
   int nblocks = 0, bs = 0;
   std::vector<int32_t> row, jpos;
-  std::vector<uint8_t> lcol, diag, nslot;
-  std::vector<uint16_t> levL, levU, cL, cU, pair;
+  std::vector<uint8_t> diag, nslot;
+  std::vector<uint16_t> lcol, levL, levU, cL, cU, pair;
   std::vector<uint8_t> npair;
   std::vector<int32_t> nlev;
 };
@@ -26,8 +26,8 @@ struct HostBJ {
 // level schedules.
 static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols, int nc, int bs,
                             const std::vector<int32_t> &part, HostBJ &out) {
-  if (bs < 1 || bs > kBlkMax) return "block size must be in [1,256]";
-  const int S = bs <= 64 ? 64 : kBlkMax;  // per-block stride = launch width
+  if (bs < 1 || bs > kBlkMax) return "block size must be in [1,512]";
+  const int S = bs <= 64 ? 64 : bs <= 256 ? 256 : 512;  // per-block stride = launch width
   out.bs = S;
   std::vector<int32_t> blk(nc, -1), local(nc, -1);
   std::vector<std::vector<int32_t>> members;
@@ -81,7 +81,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   out.nblocks = nb;
   out.row.assign((size_t)nb * S, -1);
   out.jpos.assign((size_t)nb * kW * S, -1);
-  out.lcol.assign((size_t)nb * kW * S, 255);
+  out.lcol.assign((size_t)nb * kW * S, 0xffff);
   out.diag.assign((size_t)nb * S, 0);
   out.nslot.assign((size_t)nb * S, 0);
   out.levL.assign((size_t)nb * S, 0);
@@ -106,7 +106,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
         if (ns >= kW) return "more than 27 in-block entries in a row";
         // position of (r, slot k-jrp[r]) in the slot-major ELL Jacobian
         out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (k - jrp[r]) * nc + r;
-        out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint8_t)local[c];
+        out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint16_t)local[c];
         if (c == r) out.diag[(size_t)b * S + i] = (uint8_t)ns;
         ns++;
       }
@@ -188,9 +188,14 @@ struct SolverWS {
   GmresDev *gdev = nullptr;
   int *status = nullptr;
   int nchunk = 0;
+  // one GMRES restart cycle captured as a graph, per Krylov precision
+  // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
+  hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
 };
 
 static void ws_free(SolverWS &ws) {
+  for (hipGraphExec_t &e : ws.cycle_exec)
+    if (e) (void)hipGraphExecDestroy(e);
   for (void *p : ws.allocs) (void)hipFree(p);
   ws.allocs.clear();
   ws = SolverWS();
@@ -231,7 +236,9 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   auto up = [&](auto **p, const auto &vec) -> int {
     int r = ws_alloc(ws, p, vec.size());
     if (r) return r;
-    if (hipMemcpy(*p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(*p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
       set_error("hipMemcpy (block-Jacobi layout)");
       return TDYHIP_ERR_HIP;
     }
@@ -301,7 +308,7 @@ static double bytes_apply(const SolverWS &ws, size_t tv) {
 }
 static double bytes_factor(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
-  return rows * (kW * (4 + 1 + 8) + kLU + kPair * 2 + (2 * kLU + 1) * tv + 1 + 1 + 2);
+  return rows * (kW * (4 + 2 + 8) + kLU + kPair * 2 + (2 * kLU + 1) * tv + 1 + 1 + 2);
 }
 
 // Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
@@ -324,10 +331,12 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
   k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
   if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status));
-  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
     if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp));
-    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, kBlkMax><<<ws.bj.nblocks, kBlkMax, 0, st>>>(ws.bj, fac, in, out, stp));
+    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp));
+    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp));
   };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
@@ -348,7 +357,9 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   SCHK(hipGetLastError());
   GmresDev hg;
   int status = 0;
-  for (int cycle = 0;; cycle++) {
+  // One restart cycle: m Arnoldi steps (each stops early on the device once
+  // the cycle has converged), the Hessenberg solve and the update of x.
+  auto cycle_body = [&]() {
     k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V, gd);
     for (int j = 0; j < m; j++) {
       KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(TV)), k_spmv_ell_stop<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, V + (int64_t)j * N, z, gd));
@@ -362,6 +373,24 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
     k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
+  };
+  // Replay the cycle as a hipGraph (captured on first use) unless the kernel
+  // timers are on (they time every launch) or the vectors are not the
+  // workspace's own.
+  const bool graph = o.hip_graphs && !ctx->kt.on && bvec == ws.F && yvec == ws.Y;
+  hipGraphExec_t &cyc = ws.cycle_exec[std::is_same<TV, double>::value ? 0 : 1];
+  if (graph && !cyc) {
+    hipGraph_t gr = nullptr;
+    SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    cycle_body();
+    SCHK(hipStreamEndCapture(st, &gr));
+    const hipError_t ie = hipGraphInstantiate(&cyc, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie != hipSuccess) { cyc = nullptr; SCHK(ie); }
+  }
+  for (int cycle = 0;; cycle++) {
+    if (graph) SCHK(hipGraphLaunch(cyc, st));
+    else cycle_body();
     SCHK(hipGetLastError());
     SCHK(hipMemcpyAsync(&hg, gd, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
     if (cycle == 0) SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -540,7 +569,7 @@ void tdyhip_solver_defaults(tdyhip_solver_opts *o) {
   if (!o) return;
   o->snes_rtol = 1e-8; o->snes_atol = 1e-50; o->snes_stol = 1e-8; o->snes_max_it = 50;
   o->ksp_rtol = 1e-5; o->ksp_atol = 1e-50; o->ksp_max_it = 10000; o->ksp_restart = 30;
-  o->krylov_fp32 = 0; o->pc_block = 256;
+  o->krylov_fp32 = 0; o->pc_block = 256; o->hip_graphs = 1;
 }
 
 static int check_opts(const tdyhip_solver_opts *o) {

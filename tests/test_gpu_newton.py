@@ -31,21 +31,22 @@ def test_device_newton_reproduces_reference_gold(name):
     _check(U.cpu().numpy(), case["expected"])
 
 
-@pytest.mark.parametrize("fp32", [0, 1])
-def test_device_newton_matches_oracle_cfg0(fp32):
+@pytest.mark.parametrize("fp32,pc_block", [(0, 256), (1, 256), (0, 512), (1, 512), (1, 64)])
+def test_device_newton_matches_oracle_cfg0(fp32, pc_block):
     """Plain-method configuration on a reduced mesh: device Newton (FP64 or
     FP32 Krylov) reaches the oracle's solution to 1e-8 relative in pressure."""
-    print(f"[case] cfg0 Newton fp32={fp32}", flush=True)
+    print(f"[case] cfg0 Newton fp32={fp32} pc_block={pc_block}", flush=True)
     import torch
     from pyoracle import Oracle
     from tdycore_amd.mpfao import MPFAO
-    P = cfg0_problem(n=8)
+    P = cfg0_problem(n=10)
     o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     Uo, st_o = o.run_snes_steps(P["U0"], 3600.0, 3, lin="lu", rtol=1e-10)
     assert st_o[3] == 0
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     U = torch.tensor(P["U0"], device="cuda")
-    st = g.run_snes_steps(U, 3600.0, 3, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32, ksp_rtol=1e-8))
+    st = g.run_snes_steps(U, 3600.0, 3, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32, ksp_rtol=1e-8,
+                                                         pc_block=pc_block))
     assert st["status"] == 0
     err = np.abs(U.cpu().numpy() - Uo).max() / np.abs(Uo).max()
     print("max rel pressure err", err, st)
@@ -62,9 +63,13 @@ def test_device_newton_is_deterministic():
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     for fp32 in (0, 1):
         outs = []
-        for _ in range(2):
+        # graph replays (first two runs: capture, then replay across steps and
+        # Newton iterations with eager work between) against direct launches
+        for graphs in (1, 1, 0):
             U = torch.tensor(P["U0"], device="cuda")
-            st = g.run_snes_steps(U, 3600.0, 2, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32))
+            st = g.run_snes_steps(U, 3600.0, 2, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32,
+                                                                 hip_graphs=graphs))
             assert st["status"] == 0
             outs.append(U.cpu().numpy())
         assert np.array_equal(outs[0], outs[1]), fp32
+        assert np.array_equal(outs[0], outs[2]), fp32
