@@ -165,29 +165,74 @@ __device__ inline TV block_sum(TV v, TV *sh) {
   return s;
 }
 
-// partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1
+// Sum of 32 per-lane values over a wavefront by recursive halving: at each
+// step a lane keeps one half of its values, adds the partner's copy of that
+// half and hands over the other half (16 + 8 + 4 + 2 + 1 exchanges), then one
+// more exchange completes the pair.  Afterwards lanes 2j and 2j+1 both hold
+// the wavefront sum of value j.
+template <int HALF, class TV>
+__device__ inline void halve_step(TV (&v)[32], int lane) {
+  const bool upper = (lane & (2 * HALF)) != 0;
+#pragma unroll
+  for (int k = 0; k < HALF; k++) {
+    const TV send = upper ? v[k] : v[HALF + k];
+    const TV keep = upper ? v[HALF + k] : v[k];
+    v[k] = keep + __shfl_xor(send, 2 * HALF, 64);
+  }
+}
+
+template <class TV>
+__device__ inline TV wave_reduce_scatter32(TV (&v)[32]) {
+  const int lane = threadIdx.x & 63;
+  halve_step<16>(v, lane);
+  halve_step<8>(v, lane);
+  halve_step<4>(v, lane);
+  halve_step<2>(v, lane);
+  halve_step<1>(v, lane);
+  return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1.  Each
+// thread accumulates up to 32 dots in registers; one reduce-scatter per
+// wavefront and one barrier per group of 32 vectors finish the chunk.
 template <class TV>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                     const TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
                                                     const int *__restrict__ stop = nullptr) {
-  __shared__ TV sh[kRedBlock / 64];
+  __shared__ TV red[kRedBlock / 64][32];
   if (stop && *stop) return;
-  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + t;
   TV wv[kRedPerThread];
 #pragma unroll
   for (int e = 0; e < kRedPerThread; e++) {
     const int64_t i = i0 + (int64_t)e * kRedBlock;
     wv[e] = i < n ? w[i] : (TV)0;
   }
-  for (int j = 0; j < nv; j++) {
-    TV s = 0;
+  for (int j0 = 0; j0 < nv; j0 += 32) {
+    TV acc[32];
 #pragma unroll
-    for (int e = 0; e < kRedPerThread; e++) {
-      const int64_t i = i0 + (int64_t)e * kRedBlock;
-      if (i < n) s += wv[e] * V[j * ldv + i];
+    for (int j = 0; j < 32; j++) {
+      acc[j] = 0;
+      if (j0 + j < nv) {
+        const TV *vj = V + (int64_t)(j0 + j) * ldv;
+#pragma unroll
+        for (int e = 0; e < kRedPerThread; e++) {
+          const int64_t i = i0 + (int64_t)e * kRedBlock;
+          if (i < n) acc[j] += wv[e] * vj[i];
+        }
+      }
     }
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[(int64_t)j * nchunk + blockIdx.x] = s;
+    const TV r = wave_reduce_scatter32(acc);
+    if ((lane & 1) == 0) red[wv_id][lane >> 1] = r;
+    __syncthreads();
+    if (t < 32 && j0 + t < nv) {
+      TV s = 0;
+#pragma unroll
+      for (int q = 0; q < kRedBlock / 64; q++) s += red[q][t];
+      partial[(int64_t)(j0 + t) * nchunk + blockIdx.x] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -297,12 +342,7 @@ __global__ void __launch_bounds__(256) k_spmv_ell_stop(int n, int W, const int32
   if (gd->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  TV s = 0;
-  for (int k = 0; k < W; k++) {
-    const int64_t p = (int64_t)k * n + i;
-    s += a[p] * x[ecol[p]];
-  }
-  y[i] = s;
+  y[i] = ell_row<TV>(n, W, i, ecol, a, x);
 }
 
 // y = s * x with s read on the device (1/beta or 1/h(j+1,j))
