@@ -365,7 +365,9 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   if (hipMemsetAsync(d_status, 0, sizeof(int), st) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
   k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa,
                                                         d_fn, d_K, d_Gsub);
-  k_local_systems<<<nblk(nv, 64), 64, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows, d_status);
+  KT(ctx, st, "k_local_systems", (double)nv * sizeof(IRDesc) + (double)nc * 72 * 8 + (double)ctx->trans_nnz * 8,
+     k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
+                                                                                 d_status));
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
                                            d_fc, d_fa, d_fn, d_K, ctx->d_G);
   // boundary rho, mu frozen at (Pref, Tref) (AllocateMemoryForBoundaryValues, tdympfao.c:297-336)

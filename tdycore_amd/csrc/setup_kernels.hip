@@ -2,7 +2,7 @@
 //
 //  k_gmatrix        one thread per subcell: the 3x3 sub-cell G matrix
 //                   (ComputeGMatrix_MPFAO / _TPF, src/fv/mpfao/tdympfao.c:39-126, 128-274)
-//  k_local_systems  one thread per vertex: assemble the interaction-region system
+//  k_local_systems  one 16-lane team per vertex: assemble the interaction-region system
 //                   A x = B [p_cells; p_dir] from the G rows, LU with partial pivoting,
 //                   T = C A^-1 B - F (+ C_dir)   (tdympfao.c:1126-1513, 1515-1756)
 //  k_gravdis        one thread per face: A_sub (n . K_face u)(|g| u_z)
@@ -92,77 +92,159 @@ __device__ inline double crow(const IRDesc &d, const double *__restrict__ Gsub, 
   return -G[0] - G[1] - G[2];
 }
 
-__global__ void __launch_bounds__(64) k_local_systems(int nv, const IRDesc *__restrict__ irs,
-                                                      const double *__restrict__ Gsub, double *__restrict__ T,
-                                                      int64_t nrows, int *__restrict__ status) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const IRDesc d = irs[v];
-  const int nU = d.nint + d.nneu, ncell = d.ncell, ndir = d.ndir, nB = ncell + ndir;
-  double A[kMaxIRFaces * kMaxIRFaces];
-  double X[kMaxIRFaces * kMaxCols];
-  double Cu[kMaxIRFaces + kMaxCols], Cd[kMaxIRFaces + kMaxCols];
-  int piv[kMaxIRFaces];
-  if (nU > 0) {
-    for (int q = 0; q < nU * nU; q++) A[q] = 0.0;
-    for (int q = 0; q < nU * nB; q++) X[q] = 0.0;
-    for (int k = 0; k < d.nface; k++) {
-      if (d.role[k] == 0) {
-        const int row = d.fidx[k];
-        const double Fu = crow(d, Gsub, nU, d.fup_i[k], d.fup_l[k], Cu);
-        const double Fd = crow(d, Gsub, nU, d.fdn_i[k], d.fdn_l[k], Cd);
-        for (int j = 0; j < nU; j++) A[row + j * nU] = -Cu[j] + Cd[j];
-        X[row + d.fup_i[k] * nU] += -Fu;
-        X[row + d.fdn_i[k] * nU] += Fd;
-        for (int j = 0; j < ndir; j++) X[row + (ncell + j) * nU] = Cu[nU + j] - Cd[nU + j];
-      } else if (d.role[k] == 1) {
-        const int row = d.fidx[k];
-        const double Fu = crow(d, Gsub, nU, d.fup_i[k], d.fup_l[k], Cu);
-        for (int j = 0; j < nU; j++) A[row + j * nU] = -Cu[j];
-        X[row + d.fup_i[k] * nU] += -Fu;
-        for (int j = 0; j < ndir; j++) X[row + (ncell + j) * nU] = Cu[nU + j];
-      }
+// One interaction region per 16-lane team (4 per wavefront, 16 per block).
+// Lane k owns the equation of the vertex's face k (a row of A and of the
+// right-hand side B, in registers); partial-pivoting elimination picks the
+// pivot lane by a team max-reduction and broadcasts the pivot row with team
+// shuffles, so rows are never moved.  Back substitution leaves X = A^-1 B in
+// LDS, and each face lane forms its flux row T = C X - F (+ C_dir) from the
+// three nonzeros of its C row.
+constexpr int kTeam = 16;
+constexpr int kTeamsPerBlock = 16;
+
+__device__ inline double team_bcast(double x, int src) { return __shfl(x, src, kTeam); }
+
+__global__ void __launch_bounds__(kTeam * kTeamsPerBlock)
+    k_local_systems(int nv, const IRDesc *__restrict__ irs, const double *__restrict__ Gsub, double *__restrict__ T,
+                    int64_t nrows, int *__restrict__ status) {
+  __shared__ double Xs[kTeamsPerBlock][kMaxIRFaces][kMaxCols];
+  const int team = threadIdx.x / kTeam, lane = threadIdx.x % kTeam;
+  const int v = blockIdx.x * kTeamsPerBlock + team;
+  if (v >= nv) return;  // whole teams leave together: shuffles stay inside a team
+  const IRDesc *d = irs + v;
+  const int nface = d->nface, ncell = d->ncell, ndir = d->ndir;
+  const int nU = d->nint + d->nneu;
+  const int nB = ncell + ndir;
+  const bool has_face = lane < nface;
+  const int role = has_face ? d->role[lane] : 2;
+  // up- and down-side subcell C rows of this face: three (column, -G) pairs each
+  int cu[3] = {-1, -1, -1}, cd[3] = {-1, -1, -1};
+  double gu[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
+  int fup_i = 0, fdn_i = 0;
+  if (has_face) {
+    fup_i = d->fup_i[lane];
+    const int lu = d->fup_l[lane];
+    const double *G = Gsub + (size_t)d->subcell[fup_i] * 9 + lu * 3;
+    for (int m = 0; m < 3; m++) {
+      const int km = d->sck[fup_i][m];
+      cu[m] = (d->role[km] == 2) ? nU + d->fidx[km] : d->fidx[km];
+      gu[m] = G[m];
     }
-    // LU with partial pivoting (getrf), then X <- A^-1 X (getrs)
-    for (int k = 0; k < nU; k++) {
-      int p = k;
-      double mx = fabs(A[k + k * nU]);
-      for (int i = k + 1; i < nU; i++) {
-        const double a = fabs(A[i + k * nU]);
-        if (a > mx) { mx = a; p = i; }
-      }
-      piv[k] = p;
-      if (mx == 0.0) { atomicOr(status, 1); return; }
-      if (p != k)
-        for (int j = 0; j < nU; j++) { const double t = A[k + j * nU]; A[k + j * nU] = A[p + j * nU]; A[p + j * nU] = t; }
-      const double inv = 1.0 / A[k + k * nU];
-      for (int i = k + 1; i < nU; i++) A[i + k * nU] *= inv;
-      for (int j = k + 1; j < nU; j++) {
-        const double akj = A[k + j * nU];
-        for (int i = k + 1; i < nU; i++) A[i + j * nU] -= A[i + k * nU] * akj;
-      }
-    }
-    for (int r = 0; r < nB; r++) {
-      double *b = X + r * nU;
-      for (int k = 0; k < nU; k++)
-        if (piv[k] != k) { const double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
-      for (int k = 0; k < nU; k++) { const double s = b[k]; for (int i = k + 1; i < nU; i++) b[i] -= A[i + k * nU] * s; }
-      for (int k = nU - 1; k >= 0; k--) {
-        b[k] /= A[k + k * nU];
-        const double s = b[k];
-        for (int i = 0; i < k; i++) b[i] -= A[i + k * nU] * s;
+    if (role == 0) {
+      fdn_i = d->fdn_i[lane];
+      const int ld = d->fdn_l[lane];
+      const double *Gd = Gsub + (size_t)d->subcell[fdn_i] * 9 + ld * 3;
+      for (int m = 0; m < 3; m++) {
+        const int km = d->sck[fdn_i][m];
+        cd[m] = (d->role[km] == 2) ? nU + d->fidx[km] : d->fidx[km];
+        gd[m] = Gd[m];
       }
     }
   }
-  // flux rows: internal faces and Dirichlet faces, flux from the up-side subcell
-  for (int k = 0; k < d.nface; k++) {
-    if (d.role[k] == 1) continue;
-    const double F = crow(d, Gsub, nU, d.fup_i[k], d.fup_l[k], Cu);
-    const int64_t r = d.row[k];
+  const double gus = gu[0] + gu[1] + gu[2], gds = gd[0] + gd[1] + gd[2];
+  // equation of an internal (role 0) or no-flow (role 1) face: A row and B row
+  const bool eq = has_face && role != 2;
+  double a[kMaxIRFaces], b[kMaxCols];
+#pragma unroll
+  for (int j = 0; j < kMaxIRFaces; j++) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      s += (cu[m] == j) ? gu[m] : 0.0;   // -C_up
+      s -= (cd[m] == j) ? gd[m] : 0.0;   // +C_dn
+    }
+    a[j] = (eq && j < nU) ? s : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxCols; j++) {
+    double s = 0.0;
+    if (j < ncell) {
+      s += (j == fup_i) ? gus : 0.0;                  // -F_up
+      if (role == 0) s -= (j == fdn_i) ? gds : 0.0;   // +F_dn
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        s -= (cu[m] == nU + j - ncell) ? gu[m] : 0.0;  // C_up,dir
+        s += (cd[m] == nU + j - ncell) ? gd[m] : 0.0;  // -C_dn,dir
+      }
+    }
+    b[j] = (eq && j < nB) ? s : 0.0;
+  }
+  // forward elimination with partial pivoting over the team's lanes
+  int kpiv = -1;
+  int pl[kMaxIRFaces];
+#pragma unroll
+  for (int k = 0; k < kMaxIRFaces; k++) {
+    pl[k] = 0;
+    if (k < nU) {
+      double mag = (eq && kpiv < 0) ? fabs(a[k]) : -1.0;
+      int who = lane;
+#pragma unroll
+      for (int off = kTeam / 2; off > 0; off >>= 1) {
+        const double om = __shfl_xor(mag, off, kTeam);
+        const int ow = __shfl_xor(who, off, kTeam);
+        if (om > mag || (om == mag && ow < who)) { mag = om; who = ow; }
+      }
+      pl[k] = who;
+      if (!(mag > 0.0)) {
+        if (lane == 0) atomicOr(status, 1);
+        return;  // uniform over the team
+      }
+      const double pk = team_bcast(a[k], who);
+      if (lane == who) kpiv = k;
+      const bool elim = eq && kpiv < 0;
+      const double l = elim ? a[k] / pk : 0.0;
+#pragma unroll
+      for (int j = k + 1; j < kMaxIRFaces; j++) {
+        const double pj = team_bcast(a[j], who);
+        if (elim) a[j] -= l * pj;
+      }
+#pragma unroll
+      for (int j = 0; j < kMaxCols; j++) {
+        const double pj = team_bcast(b[j], who);
+        if (elim) b[j] -= l * pj;
+      }
+      if (elim) a[k] = 0.0;
+    }
+  }
+  // back substitution: unknown k from the lane that eliminated column k
+#pragma unroll
+  for (int k = kMaxIRFaces - 1; k >= 0; k--) {
+    if (k < nU) {
+      const int who = pl[k];
+      double y[kMaxCols];
+#pragma unroll
+      for (int j = 0; j < kMaxCols; j++) {
+        y[j] = team_bcast(b[j] / a[k], who);
+        if (kpiv >= 0 && kpiv < k) b[j] -= a[k] * y[j];
+      }
+      if (lane == who) {
+#pragma unroll
+        for (int j = 0; j < kMaxCols; j++) Xs[team][k][j] = y[j];
+      }
+    }
+  }
+  // the team lives in one wavefront: a wavefront-scope fence orders the LDS
+  // writes above before the reads below
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // flux rows of internal and Dirichlet faces: T = C_up X - F_up (+ C_up,dir)
+  if (has_face && role != 1) {
+    const int64_t r = d->row[lane];
     for (int j = 0; j < nB; j++) {
       double s = 0.0;
-      for (int q = 0; q < nU; q++) s += Cu[q] * X[q + j * nU];
-      const double t = (j < ncell) ? s + ((j == d.fup_i[k]) ? -F : 0.0) : s + Cu[nU + (j - ncell)];
+#pragma unroll
+      for (int m = 0; m < 3; m++)
+        if (cu[m] >= 0 && cu[m] < nU) s -= gu[m] * Xs[team][cu[m]][j];
+      double t;
+      if (j < ncell) {
+        t = s + ((j == fup_i) ? gus : 0.0);
+      } else {
+        double cdir = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) cdir -= (cu[m] == nU + j - ncell) ? gu[m] : 0.0;
+        t = s + cdir;
+      }
       T[(int64_t)j * nrows + r] = t;
     }
   }
