@@ -190,14 +190,15 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
 }
 
 static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const double *Udot, double *R, double *J,
-                              bool ell) {
+                              bool ell, const int32_t *perm, int64_t ell_stride) {
   GatherArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
   a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts;
   a.vcols = ctx->d_vcols; a.face_flags = ctx->d_face_flags; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
   a.jdslot = ctx->d_jdslot; a.flux = ctx->d_flux; a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
   a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
-  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? (int64_t)ctx->nc : 1; a.ell_w = ctx->jell_w;
+  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
+  a.perm = ell ? perm : nullptr;
   a.R = R; a.J = J;
   return a;
 }
@@ -217,16 +218,18 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
 }
 
 // rows + gather.  want_R / want_J select the outputs.
-// ell = true: J is the solver's slot-major ELL array (jell_w*nc), else CSR values.
+// ell = true: J is the solver's block-ordered slot-major ELL array (row c at
+// perm[c], slot stride ell_stride), else CSR values in the pattern order.
 // Every entry of J is written (no prior clear is needed).
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
-                       double *J, hipStream_t st, bool ell = false) {
+                       double *J, hipStream_t st, bool ell = false, const int32_t *perm = nullptr,
+                       int64_t ell_stride = 0) {
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
   if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
   else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a));
   else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
-  GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell);
+  GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell, perm, ell_stride);
   const unsigned g = nblk(ctx->nc, kGatherBlock);
   if (R && J)
     KT(ctx, st, "k_cell_gather_RJ", bytes_gather(ctx, true, true),

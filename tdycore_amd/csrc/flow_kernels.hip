@@ -199,7 +199,8 @@ struct GatherArgs {
   const uint8_t *face_flags, *row_ul, *jslot, *jdslot;
   const double *flux, *acc, *dM, *d2M, *Udot, *vol, *Jrow;
   const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
-  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or nc (ELL)
+  const int32_t *perm; // ELL: position of row c (nullptr: c)
+  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or the ELL row count
   int ell_w;           // ELL width (slots written per row in the ELL layout)
   double *R, *J;
 };
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
     const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
                                     : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
     acc[a.jdslot[c]][t] += dv;
-    const int64_t base = a.jrp ? a.jrp[c] : c;
+    const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
     const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
     for (int sl = 0; sl < len; sl++) a.J[base + (int64_t)sl * a.jstride] = acc[sl][t];
   }

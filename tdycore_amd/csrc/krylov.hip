@@ -100,16 +100,15 @@ __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__re
 }
 
 // ---------------------------------------------------------------- ILU(0) apply: y = (LU)^-1 r
+// Forward and backward sweeps over the block's levels with x (BS+1 entries,
+// x[BS] == 0 the target of padded slots) in LDS.  Every thread of the block
+// must call this; x[t] holds the right-hand side on entry and the solution on
+// return.
 template <class TV, int BS>
-__global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restrict__ fac, const TV *__restrict__ r,
-                                                 TV *__restrict__ y, const int *__restrict__ stop = nullptr) {
-  __shared__ TV x[BS + 1];  // x[BS] == 0 is the target of padded slots
-  if (stop && *stop) return;  // uniform over the workgroup
-  const int b = blockIdx.x, t = threadIdx.x;
+__device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, TV *x, int b, int t) {
   const int64_t base = (int64_t)b * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
   const int64_t cbase = (int64_t)b * kLU * BS;
-  const int row = L.d_row[base + t];
   TV vl[kLU], vu[kLU];
   uint16_t cl[kLU], cu[kLU];
 #pragma unroll
@@ -120,8 +119,6 @@ __global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restric
     cu[p] = L.d_cU[cbase + (int64_t)p * BS + t];
   }
   const TV dinv = fac[sbase + (int64_t)(2 * kLU) * BS + t];
-  x[t] = row >= 0 ? r[row] : (TV)0;
-  if (t == 0) x[BS] = (TV)0;
   const int levL = L.d_levL[base + t], levU = L.d_levU[base + t];
   const int nL = L.d_nlev[2 * b], nU = L.d_nlev[2 * b + 1];
   __syncthreads();
@@ -144,6 +141,78 @@ __global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restric
     }
     __syncthreads();
   }
+}
+
+template <class TV, int BS>
+__global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restrict__ fac, const TV *__restrict__ r,
+                                                 TV *__restrict__ y, const int *__restrict__ stop = nullptr) {
+  __shared__ TV x[BS + 1];
+  if (stop && *stop) return;  // uniform over the workgroup
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int row = L.d_row[(int64_t)b * BS + t];
+  x[t] = row >= 0 ? r[row] : (TV)0;
+  if (t == 0) x[BS] = (TV)0;
+  bj_sweeps<TV, BS>(L, fac, x, b, t);
+  if (row >= 0) y[row] = x[t];
+}
+
+// ---------------------------------------------------------------- block-ordered SpMV
+// The solver's Jacobian is stored slot-major over the block layout: slot k of
+// block position p at a[k*NB + p], its global column at ecol[k*NB + p]; the
+// global row of p is L.d_row[p] (-1 for padding positions).
+
+// UNROLL: issue a hexahedral row's 27 loads at once (more registers, more
+// memory-level parallelism); worth it where occupancy is set by other phases.
+template <class TV, bool UNROLL = (sizeof(TV) == 4)>
+__device__ inline TV blk_row(int64_t NB, int W, int64_t p, const int32_t *__restrict__ ecol,
+                             const TV *__restrict__ a, const TV *__restrict__ x) {
+  TV s = 0;
+  if (UNROLL && W == kW) {
+    int c[kW];
+    TV v[kW];
+#pragma unroll
+    for (int k = 0; k < kW; k++) {
+      c[k] = ecol[k * NB + p];
+      v[k] = a[k * NB + p];
+    }
+#pragma unroll
+    for (int k = 0; k < kW; k++) s += v[k] * x[c[k]];
+  } else {
+    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[ecol[k * NB + p]];
+  }
+  return s;
+}
+
+// y[row(p)] = (A x)[row(p)]
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32_t *__restrict__ rowof,
+                                                  const int32_t *__restrict__ ecol, const TV *__restrict__ a,
+                                                  const TV *__restrict__ x, TV *__restrict__ y) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NB) return;
+  const int row = rowof[p];
+  if (row < 0) return;
+  y[row] = blk_row<TV>(NB, W, p, ecol, a, x);
+}
+
+// ---------------------------------------------------------------- fused SpMV + ILU(0) apply: y = (LU)^-1 A x
+// The block's rows of A x go straight into LDS and the triangular sweeps of
+// k_bj_apply follow, so A x never travels through HBM; while some workgroups
+// sweep their levels, others stream their rows of A.
+template <class TV, int BS>
+__global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, int W, const int32_t *__restrict__ ecol,
+                                                      const TV *__restrict__ a, const TV *__restrict__ xin,
+                                                      const TV *__restrict__ fac, TV *__restrict__ y,
+                                                      const int *__restrict__ stop) {
+  __shared__ TV x[BS + 1];
+  if (stop && *stop) return;  // uniform over the workgroup
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int64_t p = (int64_t)b * BS + t;
+  const int row = L.d_row[p];
+  // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
+  x[t] = row >= 0 ? blk_row<TV, true>(NB, W, p, ecol, a, xin) : (TV)0;
+  if (t == 0) x[BS] = (TV)0;
+  bj_sweeps<TV, BS>(L, fac, x, b, t);
   if (row >= 0) y[row] = x[t];
 }
 

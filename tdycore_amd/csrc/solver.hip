@@ -19,12 +19,14 @@ struct HostBJ {
   std::vector<uint16_t> lcol, levL, levU, cL, cU, pair;
   std::vector<uint8_t> npair;
   std::vector<int32_t> nlev;
+  std::vector<int32_t> perm;  // nc: block-layout position of each cell
+  std::vector<int32_t> ecol;  // W*nblocks*bs: global column of slot k at block position p (k*NB + p)
 };
 
 // Breadth-first partition of the cells into blocks of at most `bs` cells over
 // the Jacobian's vertex-adjacency graph, then the in-block ILU(0) slots and
 // level schedules.
-static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols, int nc, int bs,
+static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols, int nc, int W, int bs,
                             const std::vector<int32_t> &part, HostBJ &out) {
   if (bs < 1 || bs > kBlkMax) return "block size must be in [1,512]";
   const int S = bs <= 64 ? 64 : bs <= 256 ? 256 : 512;  // per-block stride = launch width
@@ -79,6 +81,8 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   }
   const int nb = (int)members.size();
   out.nblocks = nb;
+  const int64_t NB = (int64_t)nb * S;  // rows of the block layout
+  if (NB * W > INT32_MAX) return "mesh too large for the block-Jacobi layout";
   out.row.assign((size_t)nb * S, -1);
   out.jpos.assign((size_t)nb * kW * S, -1);
   out.lcol.assign((size_t)nb * kW * S, 0xffff);
@@ -104,8 +108,8 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
         const int c = jcols[k];
         if (blk[c] != b) continue;
         if (ns >= kW) return "more than 27 in-block entries in a row";
-        // position of (r, slot k-jrp[r]) in the slot-major ELL Jacobian
-        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (k - jrp[r]) * nc + r;
+        // position of (r, slot k-jrp[r]) in the block-ordered, slot-major Jacobian
+        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (int32_t)((k - jrp[r]) * NB + (int64_t)b * S + i);
         out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint16_t)local[c];
         if (c == r) out.diag[(size_t)b * S + i] = (uint8_t)ns;
         ns++;
@@ -168,6 +172,18 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     }
     for (int i = 0; i < m; i++) local[mem[i]] = -1;
   }
+  // block-ordered ELL columns (padding slots point at the row itself with a 0 value)
+  out.perm.assign(nc, -1);
+  out.ecol.assign((size_t)W * NB, 0);
+  for (int64_t p = 0; p < NB; p++) {
+    const int r = out.row[p];
+    if (r < 0) continue;
+    out.perm[r] = (int32_t)p;
+    const int len = jrp[r + 1] - jrp[r];
+    for (int k = 0; k < W; k++) out.ecol[(size_t)k * NB + p] = k < len ? jcols[jrp[r] + k] : r;
+  }
+  for (int c = 0; c < nc; c++)
+    if (out.perm[c] < 0) return "cell missing from the block-Jacobi partition";
   return std::string();
 }
 
@@ -188,6 +204,9 @@ struct SolverWS {
   GmresDev *gdev = nullptr;
   int *status = nullptr;
   int nchunk = 0;
+  int64_t NB = 0;                // rows of the block layout (nblocks * bs)
+  int32_t *d_perm = nullptr;     // nc: block-layout position of each cell
+  int32_t *d_ecol = nullptr;     // W*NB block-ordered ELL columns
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
@@ -223,10 +242,10 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
     return 0;
   ws_free(ws);
   HostBJ hb;
-  std::string msg = build_bj(ctx->h_jrp, ctx->h_jcols, ctx->nc, o.pc_block, ctx->pc_part, hb);
+  std::string msg = build_bj(ctx->h_jrp, ctx->h_jcols, ctx->nc, ctx->jell_w, o.pc_block, ctx->pc_part, hb);
   if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
   const int n = ctx->nc;
-  const int64_t nnz = (int64_t)ctx->jell_w * n;  // slot-major ELL Jacobian
+  const int64_t nnz = (int64_t)ctx->jell_w * hb.nblocks * hb.bs;  // block-ordered slot-major ELL Jacobian
   ws.n = n; ws.restart = o.ksp_restart; ws.fp32 = fp32; ws.bs = o.pc_block; ws.nnz = nnz;
   ws.nchunk = (int)((n + kChunk - 1) / kChunk);
   int rc = 0;
@@ -246,6 +265,8 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   };
   ws.bj.nblocks = hb.nblocks;
   ws.bj.bs = hb.bs;
+  ws.NB = (int64_t)hb.nblocks * hb.bs;
+  if ((rc = up(&ws.d_perm, hb.perm)) || (rc = up(&ws.d_ecol, hb.ecol))) return rc;
   ws.part_version = ctx->pc_part_version;
   if ((rc = up(&ws.bj.d_row, hb.row)) || (rc = up(&ws.bj.d_jpos, hb.jpos)) || (rc = up(&ws.bj.d_lcol, hb.lcol)) ||
       (rc = up(&ws.bj.d_diag, hb.diag)) || (rc = up(&ws.bj.d_nslot, hb.nslot)) || (rc = up(&ws.bj.d_levL, hb.levL)) ||
@@ -338,6 +359,14 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp));
     else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp));
   };
+  // w = M^-1 A v in one pass (block-ordered SpMV fused into the ILU sweeps)
+  auto spmv_apply = [&](const TV *in, TV *out, const int *stp) {
+    const int W = ctx->jell_w;
+    const double by = bytes_spmv(ctx, sizeof(TV)) + bytes_apply(ws, sizeof(TV)) - 2.0 * ws.n * sizeof(TV);
+    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
+    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
+    else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
+  };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
     k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
@@ -362,8 +391,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   auto cycle_body = [&]() {
     k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V, gd);
     for (int j = 0; j < m; j++) {
-      KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(TV)), k_spmv_ell_stop<TV><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, Jk, V + (int64_t)j * N, z, gd));
-      pc_apply(z, w, stop);
+      spmv_apply(V + (int64_t)j * N, w, stop);
       KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part, stop));
@@ -398,7 +426,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.converged || hg.its >= o.ksp_max_it) break;
     // restart: w = M^-1 (b - A x), the residual in FP64
-    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(double)), k_spmv_ell<double><<<g256, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, yvec, ws.JY));
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double)), k_spmv_blk<double><<<(unsigned)((ws.NB + 255) / 256), 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, yvec, ws.JY));
     k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z);
     pc_apply(z, w, nullptr);
     norm2_to_h0(w);
@@ -445,9 +473,9 @@ static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, h
   SolverWS &ws = *ctx->ws;
   if (fm.type == 0) {
     k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
-    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st, true);
+    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB);
   }
-  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true);
+  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB);
 }
 
 // FP32 Krylov: the FP32 attempt gets at most kFp32MaxIt iterations; if it
@@ -500,7 +528,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
       k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
       ynorm = maxstep;
     }
-    KT(ctx, st, "k_spmv_ell", bytes_spmv(ctx, sizeof(double)), k_spmv_ell<double><<<g, 256, 0, st>>>(n, ctx->jell_w, ctx->d_ecols, ws.J, ws.Y, ws.JY));
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double)), k_spmv_blk<double><<<(unsigned)((ws.NB + 255) / 256), 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, ws.Y, ws.JY));
     double initslope;
     if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
     if (initslope > 0.0) initslope = -initslope;

@@ -67,9 +67,11 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
     tv = 8  # Krylov buffers are sized for FP64 (the fallback of a failed FP32 solve)
     ctx = 333 * nc + 609 * nf + 96 * nv + 36 * nbnd
     setup_tmp = 768 * nc + 56 * nf + 200 * nv
-    ws = 64 * nc + 216 * nc + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
+    # Newton vectors, block-ordered Jacobian and its columns, Krylov basis,
+    # block-Jacobi tables and factor (rows = the block layout's row count)
+    ws = 64 * nc + 4 * nc + rows * 27 * (8 + 4) + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
     if krylov_fp32:
-        ws += 108 * nc
+        ws += rows * 27 * 4
     return int(ctx + max(setup_tmp, ws) + (1 << 20))
 
 
