@@ -47,6 +47,22 @@ struct BJLayout {
 
 constexpr int kPair = kLU * kLU;  // max update pairs per row
 
+// ---------------------------------------------------------------- device-resident GMRES cycle
+// State shared by the kernels of one restart cycle; `done` makes every later
+// kernel of the cycle return at once, so the host launches a whole cycle and
+// synchronises once.
+struct GmresDev {
+  double tol;      // stopping tolerance on the preconditioned residual
+  double inv_h;    // 1/h(j+1,j) or 1/beta for the next basis vector
+  double resid;    // current residual estimate |g(j+1)|
+  int done;        // cycle finished (converged, breakdown, or iteration cap)
+  int converged;
+  int its;         // total Arnoldi steps
+  int jcur;        // basis vectors completed in this cycle
+  int maxit;
+  int pad;
+};
+
 // ---------------------------------------------------------------- ILU(0) factor
 template <class TV, int BS>
 __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__restrict__ J, TV *__restrict__ fac,
@@ -161,10 +177,19 @@ __global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restric
 // block position p at a[k*NB + p], its global column at ecol[k*NB + p]; the
 // global row of p is L.d_row[p] (-1 for padding positions).
 
+// Column of slot k at block position p: an int32 global column, or an int16
+// offset from the row (used when every offset of the mesh fits, which halves
+// the index bytes of every SpMV).
+template <class TI>
+__device__ inline int blk_col(const TI *__restrict__ ecol, int64_t q, int row) {
+  if (sizeof(TI) == 2) return row + (int)ecol[q];
+  return (int)ecol[q];
+}
+
 // UNROLL: issue a hexahedral row's 27 loads at once (more registers, more
 // memory-level parallelism); worth it where occupancy is set by other phases.
-template <class TV, bool UNROLL = (sizeof(TV) == 4)>
-__device__ inline TV blk_row(int64_t NB, int W, int64_t p, const int32_t *__restrict__ ecol,
+template <class TV, class TI, bool UNROLL = (sizeof(TV) == 4)>
+__device__ inline TV blk_row(int64_t NB, int W, int64_t p, int row, const TI *__restrict__ ecol,
                              const TV *__restrict__ a, const TV *__restrict__ x) {
   TV s = 0;
   if (UNROLL && W == kW) {
@@ -172,48 +197,53 @@ __device__ inline TV blk_row(int64_t NB, int W, int64_t p, const int32_t *__rest
     TV v[kW];
 #pragma unroll
     for (int k = 0; k < kW; k++) {
-      c[k] = ecol[k * NB + p];
+      c[k] = blk_col(ecol, k * NB + p, row);
       v[k] = a[k * NB + p];
     }
 #pragma unroll
     for (int k = 0; k < kW; k++) s += v[k] * x[c[k]];
   } else {
-    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[ecol[k * NB + p]];
+    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[blk_col(ecol, k * NB + p, row)];
   }
   return s;
 }
 
 // y[row(p)] = (A x)[row(p)]
-template <class TV>
+template <class TV, class TI>
 __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32_t *__restrict__ rowof,
-                                                  const int32_t *__restrict__ ecol, const TV *__restrict__ a,
+                                                  const TI *__restrict__ ecol, const TV *__restrict__ a,
                                                   const TV *__restrict__ x, TV *__restrict__ y) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NB) return;
   const int row = rowof[p];
   if (row < 0) return;
-  y[row] = blk_row<TV>(NB, W, p, ecol, a, x);
+  y[row] = blk_row<TV, TI>(NB, W, p, row, ecol, a, x);
 }
 
-// ---------------------------------------------------------------- fused SpMV + ILU(0) apply: y = (LU)^-1 A x
-// The block's rows of A x go straight into LDS and the triangular sweeps of
-// k_bj_apply follow, so A x never travels through HBM; while some workgroups
-// sweep their levels, others stream their rows of A.
-template <class TV, int BS>
-__global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, int W, const int32_t *__restrict__ ecol,
-                                                      const TV *__restrict__ a, const TV *__restrict__ xin,
-                                                      const TV *__restrict__ fac, TV *__restrict__ y,
+// ---------------------------------------------------------------- fused Arnoldi matvec
+// One GMRES step's  V_j = s u  and  out = M^-1 A V_j = s M^-1 A u  (s = 1/||u||
+// read on the device): the block's rows of A u go straight into LDS and the
+// triangular sweeps follow, so neither A u nor a separate scaling pass
+// touches HBM; while some workgroups sweep their levels, others stream their
+// rows of A.  u and out must be different vectors.
+template <class TV, int BS, class TI>
+__global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, int W, const TI *__restrict__ ecol,
+                                                      const TV *__restrict__ a, const TV *__restrict__ u,
+                                                      const GmresDev *__restrict__ gd, TV *__restrict__ vj,
+                                                      const TV *__restrict__ fac, TV *__restrict__ out,
                                                       const int *__restrict__ stop) {
   __shared__ TV x[BS + 1];
   if (stop && *stop) return;  // uniform over the workgroup
   const int b = blockIdx.x, t = threadIdx.x;
   const int64_t p = (int64_t)b * BS + t;
   const int row = L.d_row[p];
+  const TV sc = (TV)gd->inv_h;
   // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
-  x[t] = row >= 0 ? blk_row<TV, true>(NB, W, p, ecol, a, xin) : (TV)0;
+  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, a, u) : (TV)0;
+  if (row >= 0) vj[row] = sc * u[row];
   if (t == 0) x[BS] = (TV)0;
   bj_sweeps<TV, BS>(L, fac, x, b, t);
-  if (row >= 0) y[row] = x[t];
+  if (row >= 0) out[row] = x[t];
 }
 
 // ---------------------------------------------------------------- BLAS-1 with deterministic reductions
@@ -388,21 +418,7 @@ __global__ void k_sub(int64_t n, const TV *__restrict__ a, const TV *__restrict_
   if (i < n) out[i] = a[i] - b[i];
 }
 
-// ---------------------------------------------------------------- device-resident GMRES cycle
-// State shared by the kernels of one restart cycle; `done` makes every later
-// kernel of the cycle return at once, so the host launches a whole cycle and
-// synchronises once.
-struct GmresDev {
-  double tol;      // stopping tolerance on the preconditioned residual
-  double inv_h;    // 1/h(j+1,j) or 1/beta for the next basis vector
-  double resid;    // current residual estimate |g(j+1)|
-  int done;        // cycle finished (converged, breakdown, or iteration cap)
-  int converged;
-  int its;         // total Arnoldi steps
-  int jcur;        // basis vectors completed in this cycle
-  int maxit;
-  int pad;
-};
+
 
 template <class TV>
 __global__ void __launch_bounds__(256) k_spmv_ell_stop(int n, int W, const int32_t *__restrict__ ecol,

@@ -21,6 +21,7 @@ struct HostBJ {
   std::vector<int32_t> nlev;
   std::vector<int32_t> perm;  // nc: block-layout position of each cell
   std::vector<int32_t> ecol;  // W*nblocks*bs: global column of slot k at block position p (k*NB + p)
+  std::vector<int16_t> ecol16;  // the same as offsets from the row, when every offset fits in int16
 };
 
 // Breadth-first partition of the cells into blocks of at most `bs` cells over
@@ -184,6 +185,23 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   }
   for (int c = 0; c < nc; c++)
     if (out.perm[c] < 0) return "cell missing from the block-Jacobi partition";
+  bool fits = true;
+  for (int64_t p = 0; p < NB && fits; p++) {
+    const int r = out.row[p];
+    for (int k = 0; k < W && fits; k++) {
+      const int64_t off = r < 0 ? 0 : (int64_t)out.ecol[(size_t)k * NB + p] - r;
+      fits = off >= INT16_MIN && off <= INT16_MAX;
+    }
+  }
+  if (fits) {
+    out.ecol16.resize(out.ecol.size());
+    for (int64_t p = 0; p < NB; p++) {
+      const int r = out.row[p];
+      for (int k = 0; k < W; k++)
+        out.ecol16[(size_t)k * NB + p] = r < 0 ? 0 : (int16_t)(out.ecol[(size_t)k * NB + p] - r);
+    }
+    out.ecol.clear();
+  }
   return std::string();
 }
 
@@ -206,7 +224,8 @@ struct SolverWS {
   int nchunk = 0;
   int64_t NB = 0;                // rows of the block layout (nblocks * bs)
   int32_t *d_perm = nullptr;     // nc: block-layout position of each cell
-  int32_t *d_ecol = nullptr;     // W*NB block-ordered ELL columns
+  int32_t *d_ecol = nullptr;     // W*NB block-ordered ELL columns (int32), or
+  int16_t *d_ecol16 = nullptr;   // the same as int16 offsets from the row (one of the two is set)
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
@@ -266,7 +285,12 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   ws.bj.nblocks = hb.nblocks;
   ws.bj.bs = hb.bs;
   ws.NB = (int64_t)hb.nblocks * hb.bs;
-  if ((rc = up(&ws.d_perm, hb.perm)) || (rc = up(&ws.d_ecol, hb.ecol))) return rc;
+  if ((rc = up(&ws.d_perm, hb.perm))) return rc;
+  if (!hb.ecol16.empty()) {
+    if ((rc = up(&ws.d_ecol16, hb.ecol16))) return rc;
+  } else if ((rc = up(&ws.d_ecol, hb.ecol))) {
+    return rc;
+  }
   ws.part_version = ctx->pc_part_version;
   if ((rc = up(&ws.bj.d_row, hb.row)) || (rc = up(&ws.bj.d_jpos, hb.jpos)) || (rc = up(&ws.bj.d_lcol, hb.lcol)) ||
       (rc = up(&ws.bj.d_diag, hb.diag)) || (rc = up(&ws.bj.d_nslot, hb.nslot)) || (rc = up(&ws.bj.d_levL, hb.levL)) ||
@@ -320,8 +344,20 @@ static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t s
   return rc;
 }
 
-static double bytes_spmv(const tdyhip_ctx *ctx, size_t tv) {
-  return (double)ctx->jell_w * ctx->nc * (tv + 4) + 2.0 * ctx->nc * tv;
+static double bytes_spmv(const tdyhip_ctx *ctx, size_t tv, bool idx16) {
+  return (double)ctx->jell_w * ctx->nc * (tv + (idx16 ? 2 : 4)) + 2.0 * ctx->nc * tv;
+}
+
+// y = J x with the FP64 Jacobian in the block layout
+static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, double *y, hipStream_t st) {
+  const unsigned g = (unsigned)((ws.NB + 255) / 256);
+  const bool i16 = ws.d_ecol16 != nullptr;
+  if (i16)
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double), true),
+       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.J, x, y));
+  else
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double), false),
+       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, x, y));
 }
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
@@ -359,13 +395,22 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp));
     else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp));
   };
-  // w = M^-1 A v in one pass (block-ordered SpMV fused into the ILU sweeps)
-  auto spmv_apply = [&](const TV *in, TV *out, const int *stp) {
+  // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused into the ILU sweeps)
+  auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp) {
     const int W = ctx->jell_w;
-    const double by = bytes_spmv(ctx, sizeof(TV)) + bytes_apply(ws, sizeof(TV)) - 2.0 * ws.n * sizeof(TV);
-    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
-    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
-    else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, ws.NB, W, ws.d_ecol, Jk, in, fac, out, stp));
+    const double by = bytes_spmv(ctx, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
+    const unsigned nb = ws.bj.nblocks;
+    if (ws.d_ecol16) {
+      const int16_t *ec = ws.d_ecol16;
+      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int16_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int16_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int16_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+    } else {
+      const int32_t *ec = ws.d_ecol;
+      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int32_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int32_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int32_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+    }
   };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
@@ -388,16 +433,17 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   int status = 0;
   // One restart cycle: m Arnoldi steps (each stops early on the device once
   // the cycle has converged), the Hessenberg solve and the update of x.
+  // The unnormalised Arnoldi vector alternates between w and z: step j reads
+  // it from one and leaves the next in the other.
   auto cycle_body = [&]() {
-    k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V, gd);
     for (int j = 0; j < m; j++) {
-      spmv_apply(V + (int64_t)j * N, w, stop);
-      KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, w, part, ws.nchunk, stop));
+      TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
+      spmv_apply(u, V + (int64_t)j * N, o, stop);
+      KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
-      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, w, part, stop));
+      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
       k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1, stop);
       k_arnoldi_step<<<1, 1, 0, st>>>(gd, j, m, ws.h, dH, dcs, dsn, dg);
-      k_scale_dev<TV><<<g256, 256, 0, st>>>(N, w, V + (int64_t)(j + 1) * N, gd);
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
     k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
@@ -426,7 +472,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.converged || hg.its >= o.ksp_max_it) break;
     // restart: w = M^-1 (b - A x), the residual in FP64
-    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double)), k_spmv_blk<double><<<(unsigned)((ws.NB + 255) / 256), 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, yvec, ws.JY));
+    spmv_blk64(ctx, ws, yvec, ws.JY, st);
     k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z);
     pc_apply(z, w, nullptr);
     norm2_to_h0(w);
@@ -528,7 +574,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
       k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
       ynorm = maxstep;
     }
-    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double)), k_spmv_blk<double><<<(unsigned)((ws.NB + 255) / 256), 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, ws.Y, ws.JY));
+    spmv_blk64(ctx, ws, ws.Y, ws.JY, st);
     double initslope;
     if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
     if (initslope > 0.0) initslope = -initslope;
