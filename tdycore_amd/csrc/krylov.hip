@@ -40,12 +40,14 @@ struct BJLayout {
   uint16_t *d_cL = nullptr;    // nblocks*kLU*bs local column of L slot (bs = zero entry)
   uint16_t *d_cU = nullptr;    // nblocks*kLU*bs local column of U slot
   // ILU(0) update lists: for row i and lower slot p (row k), pairs (d, s) with
-  // a_i[d] -= l_ik * a_k[s]; npair[p] pairs per p, packed d | s << 8.
+  // a_i[d] -= l_ik * a_k[s]; npair[p] pairs per p, each packed d | s << 8 in
+  // 16 bits, pairs q and q+1 of slot p sharing the 32-bit word p*kPairW + q/2.
   uint8_t *d_npair = nullptr;  // nblocks*kLU*bs
-  uint16_t *d_pair = nullptr;  // nblocks*kPair*bs
+  uint32_t *d_pair = nullptr;  // nblocks*kPair*bs (kPair = kLU*kPairW words per row)
 };
 
-constexpr int kPair = kLU * kLU;  // max update pairs per row
+constexpr int kPairW = (kLU + 1) / 2;  // 32-bit words per lower slot (two 16-bit pairs each)
+constexpr int kPair = kLU * kPairW;    // pair words per row
 
 // ---------------------------------------------------------------- device-resident GMRES cycle
 // State shared by the kernels of one restart cycle; `done` makes every later
@@ -84,22 +86,39 @@ __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__re
   }
   const int lev = L.d_levL[base + t];
   const int nlev = L.d_nlev[2 * b];
-  __syncthreads();
+  // The update lists of lower slot p+1 (count and up to 13 packed pairs) are
+  // fetched while slot p is eliminated, and slot 0's before the level loop, so
+  // global-memory latency is off the sequential elimination chain.
   const int64_t pbase = (int64_t)b * kPair * BS;
   const int64_t nbase = (int64_t)b * kLU * BS;
+  const int di0 = L.d_diag[base + t];
+  auto fetch = [&](int p, int &cnt, uint32_t (&w)[kPairW]) {
+    cnt = (p < di0) ? L.d_npair[nbase + (int64_t)p * BS + t] : 0;
+#pragma unroll
+    for (int q = 0; q < kPairW; q++)
+      w[q] = (p < di0) ? L.d_pair[pbase + (int64_t)(p * kPairW + q) * BS + t] : 0u;
+  };
+  int ncur;
+  uint32_t cur[kPairW];
+  fetch(0, ncur, cur);
+  __syncthreads();
   for (int l = 0; l < nlev; l++) {
     if (lev == l && nsl > 0) {
-      const int di = dg[t];
-      int e = 0;
-      for (int p = 0; p < di; p++) {  // lower entries, increasing column
+      for (int p = 0; p < di0; p++) {  // lower entries, increasing column
+        int nnext;
+        uint32_t nxt[kPairW];
+        fetch(p + 1, nnext, nxt);
         const int k = lc[p][t];
         const TV lik = a[p][t] / a[dg[k]][k];
         a[p][t] = lik;
-        const int np = L.d_npair[nbase + (int64_t)p * BS + t];
-        for (int q = 0; q < np; q++, e++) {
-          const int pr = L.d_pair[pbase + (int64_t)e * BS + t];
-          a[pr & 0xff][t] -= lik * a[pr >> 8][k];
+#pragma unroll
+        for (int q = 0; q < kLU; q++) {
+          const uint32_t w = (q & 1) ? (cur[q / 2] >> 16) : (cur[q / 2] & 0xffffu);
+          if (q < ncur) a[w & 0xff][t] -= lik * a[w >> 8][k];
         }
+        ncur = nnext;
+#pragma unroll
+        for (int q = 0; q < kPairW; q++) cur[q] = nxt[q];
       }
       if (a[dg[t]][t] == (TV)0) atomicOr(status, 1);
     }

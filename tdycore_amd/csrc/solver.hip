@@ -16,7 +16,8 @@ struct HostBJ {
   int nblocks = 0, bs = 0;
   std::vector<int32_t> row, jpos;
   std::vector<uint8_t> diag, nslot;
-  std::vector<uint16_t> lcol, levL, levU, cL, cU, pair;
+  std::vector<uint16_t> lcol, levL, levU, cL, cU;
+  std::vector<uint32_t> pair;
   std::vector<uint8_t> npair;
   std::vector<int32_t> nlev;
   std::vector<int32_t> perm;  // nc: block-layout position of each cell
@@ -146,7 +147,6 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
       if (di > kLU || ns - di - 1 > kLU) return "more than 13 lower or upper in-block entries in a row";
       // ILU(0) update pairs: row i slot d -= l_ik * row k slot s, for the entries of row k
       // after its diagonal whose column is in row i's pattern
-      int e = 0;
       for (int p = 0; p < di; p++) {
         const int k = out.lcol[(size_t)b * kW * S + (size_t)p * S + i];
         const int dk = out.diag[(size_t)b * S + k], nk = out.nslot[(size_t)b * S + k];
@@ -155,9 +155,9 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
           const int cq = out.lcol[(size_t)b * kW * S + (size_t)q * S + k];
           while (pi < ns && out.lcol[(size_t)b * kW * S + (size_t)pi * S + i] < cq) pi++;
           if (pi < ns && out.lcol[(size_t)b * kW * S + (size_t)pi * S + i] == cq) {
-            if (e >= kPair) return "too many ILU(0) update pairs in a row";
-            out.pair[(size_t)b * kPair * S + (size_t)e * S + i] = (uint16_t)(pi | (q << 8));
-            e++;
+            if (np >= kLU) return "too many ILU(0) update pairs in a row";
+            out.pair[(size_t)b * kPair * S + (size_t)(p * kPairW + np / 2) * S + i] |=
+                (uint32_t)(pi | (q << 8)) << (16 * (np & 1));
             np++;
           }
         }
@@ -365,7 +365,7 @@ static double bytes_apply(const SolverWS &ws, size_t tv) {
 }
 static double bytes_factor(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
-  return rows * (kW * (4 + 2 + 8) + kLU + kPair * 2 + (2 * kLU + 1) * tv + 1 + 1 + 2);
+  return rows * (kW * (4 + 2 + 8) + kLU + kPair * 4 + (2 * kLU + 1) * tv + 1 + 1 + 2);
 }
 
 // Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
