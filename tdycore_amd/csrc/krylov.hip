@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, in
 
 // ---------------------------------------------------------------- BLAS-1 with deterministic reductions
 constexpr int kRedBlock = 256;
-constexpr int kRedPerThread = 4;
+constexpr int kRedPerThread = 2;
 constexpr int kChunk = kRedBlock * kRedPerThread;
 
 template <class TV>
@@ -299,25 +299,31 @@ __device__ inline void halve_step(TV (&v)[32], int lane) {
   }
 }
 
-template <class TV>
-__device__ inline TV wave_reduce_scatter32(TV (&v)[32]) {
+// NS (8, 16 or 32) values: the halving steps use lane bits log2(NS)..1, so
+// value j ends in lanes with (lane >> 1) % NS == j; the lane bits left over
+// (bit 0, and bits 5 / 4 for fewer values) are summed by plain exchanges.
+template <int NS, class TV>
+__device__ inline TV wave_reduce_scatter(TV (&v)[32]) {
   const int lane = threadIdx.x & 63;
-  halve_step<16>(v, lane);
-  halve_step<8>(v, lane);
+  if (NS >= 32) halve_step<16>(v, lane);
+  if (NS >= 16) halve_step<8>(v, lane);
   halve_step<4>(v, lane);
   halve_step<2>(v, lane);
   halve_step<1>(v, lane);
-  return v[0] + __shfl_xor(v[0], 1, 64);
+  TV r = v[0] + __shfl_xor(v[0], 1, 64);
+  if (NS <= 16) r += __shfl_xor(r, 32, 64);
+  if (NS <= 8) r += __shfl_xor(r, 16, 64);
+  return r;
 }
 
 // partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1.  Each
-// thread accumulates up to 32 dots in registers; one reduce-scatter per
-// wavefront and one barrier per group of 32 vectors finish the chunk.
-template <class TV>
+// thread accumulates up to NS dots in registers; one reduce-scatter per
+// wavefront and one barrier per group of NS vectors finish the chunk.
+template <class TV, int NS>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                     const TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
                                                     const int *__restrict__ stop = nullptr) {
-  __shared__ TV red[kRedBlock / 64][32];
+  __shared__ TV red[kRedBlock / 64][NS];
   if (stop && *stop) return;
   const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + t;
@@ -327,12 +333,12 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
     const int64_t i = i0 + (int64_t)e * kRedBlock;
     wv[e] = i < n ? w[i] : (TV)0;
   }
-  for (int j0 = 0; j0 < nv; j0 += 32) {
+  for (int j0 = 0; j0 < nv; j0 += NS) {
     TV acc[32];
 #pragma unroll
     for (int j = 0; j < 32; j++) {
       acc[j] = 0;
-      if (j0 + j < nv) {
+      if (j < NS && j0 + j < nv) {
         const TV *vj = V + (int64_t)(j0 + j) * ldv;
 #pragma unroll
         for (int e = 0; e < kRedPerThread; e++) {
@@ -341,10 +347,10 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
         }
       }
     }
-    const TV r = wave_reduce_scatter32(acc);
-    if ((lane & 1) == 0) red[wv_id][lane >> 1] = r;
+    const TV r = wave_reduce_scatter<NS>(acc);
+    if ((lane & 1) == 0 && (lane >> 1) < NS) red[wv_id][lane >> 1] = r;
     __syncthreads();
-    if (t < 32 && j0 + t < nv) {
+    if (t < NS && j0 + t < nv) {
       TV s = 0;
 #pragma unroll
       for (int q = 0; q < kRedBlock / 64; q++) s += red[q][t];
