@@ -19,7 +19,10 @@ K timed steps (FP64 residual, Jacobian, Krylov).  The JSON line also carries:
   kernels       the same figures for every kernel of the path;
   fp32_krylov   the same K steps with FP32 Jacobian storage and FP32 GMRES /
                 ILU(0) (FP64 residual and Newton update), its speedup and its
-                max relative pressure difference from the FP64 run;
+                max relative pressure difference from the FP64 run, plus an
+                untimed step-by-step pass: per step the FP32-vs-FP64 pressure
+                difference, both forms' mass-balance defects and iteration
+                counts;
   cpu_baseline  the oracle (single-threaded C restatement of the reference
                 algorithm with serial GMRES(30)+ILU(0)) timed on this host on
                 one Newton iteration of the first step of the same mesh.
@@ -96,6 +99,36 @@ def run_steps(g, U, U0, dt, K, W, opts, timers):
         rep = g.kernel_timer_report()
         g.kernel_timers(False)
     return el, st, rep
+
+
+def accuracy_pass(g, U0, dt, K, o64, o32):
+    """Untimed: the K steps again, one at a time, FP64 and FP32-Krylov side by
+    side.  After each step: max relative pressure difference of the FP32 form
+    from the FP64 form, and each form's mass-balance defect
+    |sum_c R_c(U^n)| dt / sum_c rho phi S V, with R the SNES residual of the
+    step (fluxes + accumulation: internal fluxes cancel, so the sum is the
+    step's global water balance)."""
+    import torch
+    U64, U32 = U0.clone(), U0.clone()
+    R = torch.empty_like(U0)
+    rows = []
+
+    def defect(U):
+        g.SNESFunction(U, R)
+        W = g.Diagnostics(U)[2]
+        return abs(R.sum().item()) * dt / W.sum().item()
+
+    for k in range(K):
+        s64 = g.run_snes_steps(U64, dt, 1, o64)
+        mb64 = defect(U64)
+        s32 = g.run_snes_steps(U32, dt, 1, o32)
+        mb32 = defect(U32)
+        rel = ((U32 - U64).abs() / U64.abs()).max().item()
+        rows.append({"step": k + 1, "max_rel_p_diff": rel, "mass_balance_fp64": mb64, "mass_balance_fp32": mb32,
+                     "newton_fp64": s64["newton_its"], "newton_fp32": s32["newton_its"],
+                     "krylov_fp64": s64["linear_its"], "krylov_fp32": s32["linear_its"],
+                     "fp64_fallbacks": s32["krylov_fallbacks"]})
+    return rows
 
 
 def stream_copy_gbs(nbytes=1 << 30, reps=10):
@@ -254,10 +287,14 @@ def main():
             raise RuntimeError(f"FP32-Krylov Newton stepping failed: {st32}")
         rel = ((U - U64).abs() / U64.abs().clamp_min(1e-300)).max().item()
         v32 = nc * st32["newton_its"] / el32
+        per_step = accuracy_pass(g, U0, a.dt, a.steps, o64, o32)
+        log("per-step FP32 vs FP64: max rel p diff %.2e, mass balance fp64 %.2e fp32 %.2e" % (
+            max(r["max_rel_p_diff"] for r in per_step), max(r["mass_balance_fp64"] for r in per_step),
+            max(r["mass_balance_fp32"] for r in per_step)))
         fp32 = {"value": v32, "ms_per_step": 1e3 * el32 / a.steps, "speedup_vs_fp64": round(v32 / value, 3),
                 "wall_speedup_vs_fp64": round(el / el32, 3),
                 "newton_its": st32["newton_its"], "linear_its": st32["linear_its"], "max_rel_p_diff_vs_fp64": rel,
-                "kernels": kernel_table(rep32)[:6] if rep32 else []}
+                "kernels": kernel_table(rep32)[:6] if rep32 else [], "per_step": per_step}
         log(f"fp32 krylov: {el32:.3f} s newton {st32['newton_its']} linear {st32['linear_its']} rel {rel:.3e}")
 
     cpu = None

@@ -107,6 +107,13 @@ int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream str
 int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream stream);
 int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *Jvals, tdyhip_stream stream);
 
+/* Diagnostics: TDyUpdateDiagnostics_MPFAO (src/fv/mpfao/tdympfao.c:2759-2786) -- liquid
+ * saturation S and the reference's liquid mass rho*V per cell -- plus the water mass
+ * rho*phi*S*V per cell (for mass balances).  Device arrays of nc entries; any output
+ * may be NULL. */
+int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liquid_mass, double *water_mass,
+                       tdyhip_stream stream);
+
 /* ---- Newton / Krylov (PETSc SNESNEWTONLS + KSPGMRES + PCBJACOBI/ILU(0) defaults) ---- */
 typedef struct {
   double snes_rtol, snes_atol, snes_stol; /* 1e-8, 1e-50, 1e-8 */

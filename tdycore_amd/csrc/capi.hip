@@ -496,6 +496,17 @@ int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_str
   return launch_flux(ctx, 1, 0.0, U, nullptr, R, nullptr, st);
 }
 
+int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liquid_mass, double *water_mass,
+                       tdyhip_stream s) {
+  if (!ctx || !U) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  StateParams sp = state_params(ctx, 1, 0.0);
+  k_diagnostics<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol, S,
+                                                    liquid_mass, water_mass);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *J, tdyhip_stream s) {
   if (!ctx || !U || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);

@@ -64,6 +64,24 @@ __global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ 
 }
 
 // rho phi S V / dt at Uprev (TDyMPFAOSNESPreSolve, tdympfao_snes.c:32-64)
+// Diagnostics of TDyUpdateDiagnostics_MPFAO (src/fv/mpfao/tdympfao.c:2759-2786):
+// liquid saturation and the reference's "liquid mass" rho*V per cell, plus the
+// water mass rho*phi*S*V that a mass balance needs.
+__global__ void k_diagnostics(int nc, StateParams sp, const double *__restrict__ U, const double *__restrict__ phi,
+                              const double *__restrict__ Sr, const double *__restrict__ vol, double *__restrict__ S_o,
+                              double *__restrict__ mass_o, double *__restrict__ water_o) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double p = U[c];
+  double S, dS, d2S;
+  vg_saturation(sp.vg_m, sp.vg_alpha, Sr[c], sp.pref - p, S, dS, d2S);
+  double rho, drho, d2rho;
+  water_density(sp.density_type, p, rho, drho, d2rho);
+  if (S_o) S_o[c] = S;
+  if (mass_o) mass_o[c] = rho * vol[c];
+  if (water_o) water_o[c] = rho * phi[c] * S * vol[c];
+}
+
 __global__ void k_snes_presolve(int nc, StateParams sp, const double *__restrict__ U,
                                 const double *__restrict__ phi, const double *__restrict__ Sr,
                                 const double *__restrict__ vol, double *__restrict__ accum_prev) {

@@ -256,7 +256,10 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if (!ctx->ws) ctx->ws = new SolverWS();
   SolverWS &ws = *ctx->ws;
   const int fp32 = o.krylov_fp32 ? 1 : 0;
-  if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.fp32 == fp32 && ws.bs == o.pc_block &&
+  // one workspace serves both Krylov precisions (buffers sized for FP64 plus
+  // the FP32 Jacobian copy), so switching precision costs nothing
+  (void)fp32;
+  if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.bs == o.pc_block &&
       ws.part_version == ctx->pc_part_version)
     return 0;
   ws_free(ws);
@@ -306,11 +309,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
                   (size_t)hb.nblocks * kW * hb.bs, (size_t)(o.ksp_restart + 2) * ws.nchunk};
   for (int i = 0; i < 7; i++)
     if ((rc = ws_alloc(ws, cp[i], cnt[i] * tv))) return rc;
-  if (fp32) {
-    if ((rc = ws_alloc(ws, (float **)&ws.Jk, nnz))) return rc;
-  } else {
-    ws.Jk = ws.J;
-  }
+  if ((rc = ws_alloc(ws, (float **)&ws.Jk, nnz))) return rc;  // FP32 copy of the Jacobian
   if ((rc = ws_alloc(ws, &ws.dpartial, ws.nchunk))) return rc;
   if ((rc = ws_alloc(ws, &ws.h, o.ksp_restart + 2))) return rc;
   if ((rc = ws_alloc(ws, &ws.hs, (size_t)(o.ksp_restart + 1) * o.ksp_restart + 4 * (o.ksp_restart + 1)))) return rc;

@@ -40,9 +40,6 @@ def _ptr(t) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
-def _cur_stream():
-    import torch
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 class MPFAO:
@@ -73,6 +70,12 @@ class MPFAO:
         h = C.c_void_p()
         _lib.check(L.tdyhip_create(C.byref(ms), C.byref(mt), C.byref(h)))
         self._h = h
+        # The context works on a torch side stream of its own; every call is
+        # ordered after the caller's current stream and the caller's stream
+        # after the call, so torch work on either side sees the results.
+        import torch
+        self._stream = torch.cuda.Stream()
+        _lib.check(L.tdyhip_set_stream(self._h, C.c_void_p(self._stream.cuda_stream)))
         sz = np.zeros(4, np.int64)
         _lib.check(L.tdyhip_sizes(self._h, sz.ctypes.data))
         self.jac_nnz = int(sz[1])
@@ -91,6 +94,15 @@ class MPFAO:
     @property
     def handle(self):
         return self._h
+
+    def _run(self, fn, *args):
+        """fn(ctx, *args, stream=NULL -> the context's stream), ordered after
+        and before the caller's current torch stream."""
+        import torch
+        cur = torch.cuda.current_stream()
+        self._stream.wait_stream(cur)
+        _lib.check(fn(self._h, *args))
+        cur.wait_stream(self._stream)
 
     # ---- pattern / diagnostics ----
     def jacobian_pattern(self):
@@ -116,28 +128,35 @@ class MPFAO:
 
     # ---- TS callbacks ----
     def IFunction(self, t, U, U_t, R):
-        _lib.check(_lib.load().tdyhip_ifunction(self._h, float(t), _ptr(U), _ptr(U_t), _ptr(R), _cur_stream()))
+        self._run(_lib.load().tdyhip_ifunction, float(t), _ptr(U), _ptr(U_t), _ptr(R), None)
 
     def IJacobian(self, t, U, U_t, shift, J):
-        _lib.check(_lib.load().tdyhip_ijacobian(self._h, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(J),
-                                                _cur_stream()))
+        self._run(_lib.load().tdyhip_ijacobian, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(J), None)
 
     def IFunctionIJacobian(self, t, U, U_t, shift, R, J):
-        _lib.check(_lib.load().tdyhip_ifunction_ijacobian(self._h, float(t), _ptr(U), _ptr(U_t), float(shift),
-                                                          _ptr(R), _ptr(J), _cur_stream()))
+        self._run(_lib.load().tdyhip_ifunction_ijacobian, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(R),
+                  _ptr(J), None)
 
     # ---- SNES callbacks ----
     def SetDtime(self, dt):
         _lib.check(_lib.load().tdyhip_set_dtime(self._h, float(dt)))
 
     def SNESPreSolve(self, Uprev):
-        _lib.check(_lib.load().tdyhip_snes_presolve(self._h, _ptr(Uprev), _cur_stream()))
+        self._run(_lib.load().tdyhip_snes_presolve, _ptr(Uprev), None)
 
     def SNESFunction(self, U, R):
-        _lib.check(_lib.load().tdyhip_snes_function(self._h, _ptr(U), _ptr(R), _cur_stream()))
+        self._run(_lib.load().tdyhip_snes_function, _ptr(U), _ptr(R), None)
 
     def SNESJacobian(self, U, J):
-        _lib.check(_lib.load().tdyhip_snes_jacobian(self._h, _ptr(U), _ptr(J), _cur_stream()))
+        self._run(_lib.load().tdyhip_snes_jacobian, _ptr(U), _ptr(J), None)
+
+    def Diagnostics(self, U):
+        """(S, liquid mass rho*V, water mass rho*phi*S*V) per cell at U
+        (TDyUpdateDiagnostics_MPFAO, src/fv/mpfao/tdympfao.c:2759-2786)."""
+        import torch
+        S, M, W = (torch.empty_like(U) for _ in range(3))
+        self._run(_lib.load().tdyhip_diagnostics, _ptr(U), _ptr(S), _ptr(M), _ptr(W), None)
+        return S, M, W
 
     def device_bytes(self) -> int:
         """Device memory held by this context (setup arrays + solver workspace)."""
@@ -189,25 +208,24 @@ class MPFAO:
     def newton_snes(self, U, opts=None):
         o = opts or self.solver_options()
         st = _lib.SolverStats()
-        _lib.check(_lib.load().tdyhip_newton_snes(self._h, _ptr(U), C.byref(o), C.byref(st)))
+        self._run(_lib.load().tdyhip_newton_snes, _ptr(U), C.byref(o), C.byref(st))
         return self._stats_dict(st)
 
     def run_snes_steps(self, U, dt, nsteps, opts=None):
         o = opts or self.solver_options()
         st = _lib.SolverStats()
-        _lib.check(_lib.load().tdyhip_run_snes_steps(self._h, _ptr(U), float(dt), int(nsteps), C.byref(o),
-                                                     C.byref(st)))
+        self._run(_lib.load().tdyhip_run_snes_steps, _ptr(U), float(dt), int(nsteps), C.byref(o), C.byref(st))
         return self._stats_dict(st)
 
     def run_ts_be(self, U, dt, nsteps, opts=None):
         o = opts or self.solver_options()
         st = _lib.SolverStats()
-        _lib.check(_lib.load().tdyhip_run_ts_be(self._h, _ptr(U), float(dt), int(nsteps), C.byref(o), C.byref(st)))
+        self._run(_lib.load().tdyhip_run_ts_be, _ptr(U), float(dt), int(nsteps), C.byref(o), C.byref(st))
         return self._stats_dict(st)
 
     def run_tdyti(self, U, dt_init, growth, dt_max, final_time, opts=None):
         o = opts or self.solver_options()
         st = _lib.SolverStats()
-        _lib.check(_lib.load().tdyhip_run_tdyti(self._h, _ptr(U), float(dt_init), float(growth), float(dt_max),
-                                                float(final_time), C.byref(o), C.byref(st)))
+        self._run(_lib.load().tdyhip_run_tdyti, _ptr(U), float(dt_init), float(growth), float(dt_max),
+                  float(final_time), C.byref(o), C.byref(st))
         return self._stats_dict(st)

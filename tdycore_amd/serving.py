@@ -70,8 +70,7 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
     # Newton vectors, block-ordered Jacobian and its columns, Krylov basis,
     # block-Jacobi tables and factor (rows = the block layout's row count)
     ws = 64 * nc + 4 * nc + rows * 27 * (8 + 4) + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
-    if krylov_fp32:
-        ws += rows * 27 * 4
+    ws += rows * 27 * 4  # FP32 copy of the Jacobian (kept in both precisions)
     return int(ctx + max(setup_tmp, ws) + (1 << 20))
 
 
