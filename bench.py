@@ -185,13 +185,18 @@ def serve(a):
     futs = []
     ndone = [0]
 
+    done_res = []
+
     def report(f):
         ndone[0] += 1
         if f.exception() is None:
             r = f.result()
+            done_res.append(r)
             st = r["stats"]
             log(f"done {ndone[0]}: rid {r['rid']} cells {r['nc']} newton {st['newton_its']} linear {st['linear_its']} "
                 f"steps {st['steps']} cuts {st['cuts']} solve {r['solve_s']:.2f}s latency {r['latency_s']:.2f}s")
+            if len(done_res) % 32 == 0:
+                log("partial summary: " + json.dumps(serving.latency_summary(list(done_res))))
         else:
             log(f"done {ndone[0]}: failed {f.exception()}")
 

@@ -79,7 +79,12 @@ const char *tdyhip_last_error(void);
  * (batched LU) and the gravity term on the device. */
 int tdyhip_create(const tdyhip_mesh *mesh, const tdyhip_materials *mat, tdyhip_ctx **out);
 int tdyhip_destroy(tdyhip_ctx *ctx);
+/* The context's stream: by default one it created for itself (non-blocking, default
+ * priority) and never shares with another context.  A stream given to set_stream
+ * must not be shared with another context either: each solver call may capture its
+ * launches on the stream into a hipGraph. */
 int tdyhip_set_stream(tdyhip_ctx *ctx, tdyhip_stream stream);
+int tdyhip_get_stream(const tdyhip_ctx *ctx, tdyhip_stream *stream);
 int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out /* [0]=nc [1]=jac nnz [2]=nf [3]=nbnd */);
 
 /* Jacobian sparsity (vertex adjacency, TDyDiscretizationCreateJacobianMatrix,

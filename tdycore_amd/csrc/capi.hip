@@ -425,6 +425,12 @@ int tdyhip_set_stream(tdyhip_ctx *ctx, tdyhip_stream stream) {
   return 0;
 }
 
+int tdyhip_get_stream(const tdyhip_ctx *ctx, tdyhip_stream *stream) {
+  if (!ctx || !stream) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  *stream = (tdyhip_stream)ctx->stream;
+  return 0;
+}
+
 int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out) {
   if (!ctx || !out) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   out[0] = ctx->nc; out[1] = ctx->jnnz; out[2] = ctx->nf; out[3] = ctx->nbnd;

@@ -489,9 +489,9 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
 
 // Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
 // previous Givens rotations, the new rotation, residual estimate and stop test.
-__global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
-                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
-  if (gd->done) return;
+__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h,
+                                      double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
+                                      double *__restrict__ g) {
   const int ld = m + 1;
   const double hn = sqrt(h[j + 1]);
   for (int i = 0; i <= j; i++) H[i + j * ld] = h[i];
@@ -516,6 +516,12 @@ __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restr
   const int conv = gd->resid <= gd->tol || hn == 0.0;
   gd->converged = conv;
   if (conv || gd->its >= gd->maxit || j + 1 == m) gd->done = 1;
+}
+
+__global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
+                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
+  if (gd->done) return;
+  arnoldi_update(gd, j, m, h, H, cs, sn, g);
 }
 
 // y = H(1:j,1:j)^-1 g(1:j) for the j = jcur columns of the cycle

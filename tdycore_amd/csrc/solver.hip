@@ -438,9 +438,13 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     for (int j = 0; j < m; j++) {
       TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
       spmv_apply(u, V + (int64_t)j * N, o, stop);
-      if (j < 8) KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV, 8><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
-      else if (j < 16) KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV, 16><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
-      else KT(ctx, st, "k_mdot", (double)(j + 2) * N * sizeof(TV), k_mdot<TV, 32><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
+      // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
+      // kernel per reduction with a last-block hand-off is much slower here:
+      // the device-scope fences it needs write back the per-XCD L2s.)
+      const double bmd = (double)(j + 2) * N * sizeof(TV);
+      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
+      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
+      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
       k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1, stop);

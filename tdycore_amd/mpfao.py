@@ -70,12 +70,15 @@ class MPFAO:
         h = C.c_void_p()
         _lib.check(L.tdyhip_create(C.byref(ms), C.byref(mt), C.byref(h)))
         self._h = h
-        # The context works on a torch side stream of its own; every call is
-        # ordered after the caller's current stream and the caller's stream
-        # after the call, so torch work on either side sees the results.
+        # The context works on the stream it created for itself (never a
+        # torch pool stream: torch.cuda.Stream() hands out pooled streams that
+        # another context could be using, and the solver captures its launches
+        # on this stream into graphs).  Every call is ordered after the
+        # caller's current stream and the caller's stream after the call.
         import torch
-        self._stream = torch.cuda.Stream()
-        _lib.check(L.tdyhip_set_stream(self._h, C.c_void_p(self._stream.cuda_stream)))
+        hs = C.c_void_p()
+        _lib.check(L.tdyhip_get_stream(self._h, C.byref(hs)))
+        self._stream = torch.cuda.ExternalStream(hs.value)
         sz = np.zeros(4, np.int64)
         _lib.check(L.tdyhip_sizes(self._h, sz.ctypes.data))
         self.jac_nnz = int(sz[1])
@@ -94,6 +97,11 @@ class MPFAO:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def stream(self):
+        """The context's own stream (a torch.cuda.ExternalStream)."""
+        return self._stream
 
     def _run(self, fn, *args):
         """fn(ctx, *args, stream=NULL -> the context's stream), ordered after

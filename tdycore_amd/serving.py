@@ -15,14 +15,14 @@ stages:
    time-integrator rules of ``TDyTimeIntegratorRunToTime`` (src/tdyti.c:71-196):
    a Newton failure cuts dt and retries.  Each step is a pre-solve plus a
    Newton solve: FP64 residual, FP32-Krylov inner solve.  The request's state stays
-   in HBM from setup to its last step.  Each slot drives its own
-   non-blocking HIP stream, so the setup and Newton kernels of different
+   in HBM from setup to its last step.  Each context drives the
+   non-blocking HIP stream it created for itself, so the setup and Newton kernels of different
    requests overlap on the device, and a request that arrives while others
    are solving starts as soon as a slot and memory free up.
 
-Results are bit-for-bit those of the same request run alone, up to the
-summation order of the Jacobian's FP64 atomics.  The serving tests bound the
-difference at 1e-8 relative.
+Results are bit-for-bit those of the same request run alone: every kernel of
+the path (residual and Jacobian gathers, Krylov reductions) sums in a fixed
+order, whatever else runs on the device.
 """
 from __future__ import annotations
 
@@ -52,11 +52,11 @@ class Request:
 
 
 def block_rows(nx, ny, nz, brick):
-    """Rows of the block-Jacobi layout: one 64- or 256-wide slab per brick."""
+    """Rows of the block-Jacobi layout: one 64-, 256- or 512-wide slab per brick."""
     bx, by, bz = brick
     nb = -(-nx // bx) * -(-ny // by) * -(-nz // bz)
     bs = bx * by * bz
-    return nb * (64 if bs <= 64 else 256)
+    return nb * (64 if bs <= 64 else 256 if bs <= 256 else 512)
 
 
 def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
@@ -205,15 +205,15 @@ class EnsembleServer:
             g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
             bs = self.brick[0] * self.brick[1] * self.brick[2]
             o = g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs)
-            s = torch.cuda.Stream()
-            with torch.cuda.stream(s):
+            # all device work of this request stays on the context's own stream
+            # (never a torch pool stream, which another slot could be using)
+            with torch.cuda.stream(g.stream):
                 U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
-            s.synchronize()
-            # TDyTimeIntegratorRunToTime semantics: steps of req.dt, a failed
-            # Newton solve cuts dt by 4 and regrows by 1.25 up to req.dt
-            st = g.run_tdyti(U, req.dt, 1.25, req.dt, req.steps * req.dt, o)
-            with torch.cuda.stream(s):
+                # TDyTimeIntegratorRunToTime semantics: steps of req.dt, a failed
+                # Newton solve cuts dt by 4 and regrows by 1.25 up to req.dt
+                st = g.run_tdyti(U, req.dt, 1.25, req.dt, req.steps * req.dt, o)
                 out = U.cpu().numpy()
+            del U
             st["device_bytes"] = g.device_bytes()
             return out, st
         finally:

@@ -52,3 +52,35 @@ def test_served_result_matches_oracle():
     U, st = o.run_tdyti(P["U0"], r.dt, 1.25, r.dt, r.steps * r.dt, lin="gmres", rtol=1e-10)
     assert st[3] == 0
     assert _rel(res["U"], U) <= 1e-8
+
+
+def test_live_contexts_have_distinct_streams():
+    """Every context owns its stream (the solver captures graphs on it): 40
+    contexts alive together, more than torch's pool of 32 streams, never share."""
+    print("[case] 40 contexts, distinct streams", flush=True)
+    from tdycore_amd.mpfao import MPFAO
+    P = workloads.ensemble_box_problem(4, 4, 4, 11, 1.0)
+    gs = [MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"]) for _ in range(40)]
+    handles = {g.stream.cuda_stream for g in gs}
+    for g in gs:
+        g.close()
+    assert len(handles) == 40
+
+
+def test_many_small_requests_on_many_slots():
+    """More requests than torch's stream pool, 8 slots: each one equals its
+    alone result bit for bit."""
+    print("[case] 40 requests on 8 slots", flush=True)
+    reqs = [serving.Request(rid=i, nx=12 + (i % 5), ny=12 + (i % 3), nz=8, seed=900 + i, sigma=0.6, steps=1)
+            for i in range(40)]
+    srv = serving.EnsembleServer(slots=8, build_workers=8)
+    futs = [srv.submit(r) for r in reqs]
+    mix = {f.result(timeout=120)["rid"]: f.result() for f in futs}
+    srv.close()
+    assert srv.peak_inflight >= 2
+    srv = serving.EnsembleServer(slots=1, build_workers=1)
+    for r in reqs[:6]:
+        a = srv.submit(r).result(timeout=120)
+        assert np.array_equal(a["U"], mix[r.rid]["U"]), r
+    srv.close()
+    assert all(m["stats"]["status"] == 0 for m in mix.values())
