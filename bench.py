@@ -4,7 +4,10 @@
 Workload: the 128x128x64 layered, jittered hex box (1,048,576 cells,
 1000x1000x100 m), backward-Euler steps of dt = 1 day driven by the device
 Newton solver (SNESNEWTONLS + backtracking line search, snes_rtol 1e-10) with
-GMRES(30) + block-Jacobi ILU(0) (8x8x4 cell bricks) as the inner solve.
+GMRES(30) + block-Jacobi ILU(0) (8x8x4 cell bricks) with a two-level
+coarse correction (one value per brick, bricks aggregated into at most 128
+groups solved exactly; --pc-coarse 0 runs one-level block Jacobi) as the
+inner solve.
 
 A step is one backward-Euler time step: SNES pre-solve (accumulation of the
 previous state) and the Newton solve to convergence, every residual,
@@ -23,6 +26,8 @@ K timed steps (FP64 residual, Jacobian, Krylov).  The JSON line also carries:
                 untimed step-by-step pass: per step the FP32-vs-FP64 pressure
                 difference, both forms' mass-balance defects and iteration
                 counts;
+  one_level_fp64  with the coarse correction on (default), the same K FP64
+                steps with PETSc's default one-level block Jacobi, for reference;
   cpu_baseline  the oracle (single-threaded C restatement of the reference
                 algorithm with serial GMRES(30)+ILU(0)) timed on this host on
                 one Newton iteration of the first step of the same mesh.
@@ -59,6 +64,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=8.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests on the device")
     ap.add_argument("--build-workers", type=int, default=12, help="serving: host threads building problems")
+    ap.add_argument("--pc-coarse", type=int, default=1, help="coarse correction on block Jacobi (1) or not (0)")
     return ap.parse_args()
 
 
@@ -177,7 +183,7 @@ def serve(a):
     from tdycore_amd import serving, workloads
     torch.cuda.init()
     specs = workloads.ensemble_request_specs(a.requests)
-    srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers)
+    srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers, pc_coarse=bool(a.pc_coarse))
     # warm the library and the device once (not timed)
     srv.submit(serving.Request(rid=-1, nx=16, ny=16, nz=8, seed=1, sigma=1.0, steps=1)).result()
     rng = np.random.default_rng(20261207)
@@ -209,10 +215,15 @@ def serve(a):
             time.sleep(dt)
         futs.append(srv.submit(serving.Request(**sp)))
         futs[-1].add_done_callback(report)
-    res = [f.result() for f in futs]
+    res, failed = [], []
+    for sp, f in zip(specs, futs):
+        if f.exception() is None:
+            res.append(f.result())
+        else:  # e.g. the time integrator's 25 dt cuts: reported, not retried
+            failed.append({"rid": sp["rid"], "error": str(f.exception())})
     srv.close()
     summ = serving.latency_summary(res)
-    bad = [r["rid"] for r in res if r["stats"]["status"] != 0]
+    bad = [r["rid"] for r in res if r["stats"]["status"] != 0] + [f["rid"] for f in failed]
     cells = sum(r["nc"] for r in res)
     newton = sum(r["stats"]["newton_its"] for r in res)
     line = {
@@ -223,7 +234,8 @@ def serve(a):
         "config": {"workload": f"{len(res)} requests, Poisson arrivals at {a.rate:g}/s, nx,ny~U{{16..96}}, "
                                f"nz~U{{8..48}}, sigma~U(0.5,1.5), 5 backward-Euler steps of 1 h, FP64 residual, "
                                f"FP32 Krylov, {a.slots} device slots",
-                   "total_cells": cells, "newton_its": newton, "failed": bad},
+                   "total_cells": cells, "newton_its": newton, "failed": bad, "failures": failed[:20],
+                   "pc_coarse": a.pc_coarse},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"], "serving": summ,
         "peak_inflight": srv.peak_inflight, "peak_admitted_bytes": srv.peak_used, "hbm_budget": srv.hbm_budget,
     }
@@ -261,7 +273,8 @@ def main():
     U = U0.clone()
     timers = not a.no_timers
 
-    o64 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=0)
+    pcc = 1 if a.pc_coarse else 0
+    o64 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=0, pc_coarse=pcc)
     el, st, rep = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o64, timers)
     if st["status"] != 0:
         raise RuntimeError(f"FP64 Newton stepping failed: {st}")
@@ -286,7 +299,7 @@ def main():
 
     fp32 = None
     if not a.no_fp32:
-        o32 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=1)
+        o32 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=1, pc_coarse=pcc)
         el32, st32, rep32 = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o32, timers)
         if st32["status"] != 0:
             raise RuntimeError(f"FP32-Krylov Newton stepping failed: {st32}")
@@ -302,6 +315,15 @@ def main():
                 "kernels": kernel_table(rep32)[:6] if rep32 else [], "per_step": per_step}
         log(f"fp32 krylov: {el32:.3f} s newton {st32['newton_its']} linear {st32['linear_its']} rel {rel:.3e}")
 
+    one_level = None
+    if pcc:
+        o1 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=0, pc_coarse=0)
+        el1, st1, _ = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o1, False)
+        one_level = {"value": nc * st1["newton_its"] / el1, "ms_per_step": 1e3 * el1 / a.steps,
+                     "newton_its": st1["newton_its"], "linear_its": st1["linear_its"],
+                     "note": "FP64, PETSc-default one-level block-Jacobi ILU(0), no kernel timers"}
+        log(f"one-level fp64: {el1:.3f} s newton {st1['newton_its']} linear {st1['linear_its']}")
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(P, a.dt)
@@ -314,10 +336,12 @@ def main():
         "data": "synthetic (seeded layered permeability, jittered hex mesh)",
         "config": {"workload": f"layered jittered hex box {nx}x{ny}x{nz} ({nc} cells), 1000x1000x100 m, "
                                f"backward Euler dt={a.dt:g} s, snes_rtol 1e-10, GMRES(30)+bjacobi ILU(0) "
-                               f"{a.brick[0]}x{a.brick[1]}x{a.brick[2]} bricks",
+                               f"{a.brick[0]}x{a.brick[1]}x{a.brick[2]} bricks"
+                               + (" + coarse correction" if pcc else ""),
+                   "pc_coarse": pcc,
                    "cells": nc, "dt": a.dt, "newton_its": st["newton_its"], "linear_its": st["linear_its"],
                    "setup_s": round(setup_s, 3), "kernel_timers": timers},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "fp32_krylov": fp32,
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "fp32_krylov": fp32, "one_level_fp64": one_level,
     }
     print(json.dumps(line), flush=True)
 

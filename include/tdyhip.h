@@ -131,6 +131,12 @@ typedef struct {
   int32_t pc_block;                       /* rows per block-Jacobi ILU(0) block, 1..512 */
   int32_t hip_graphs;                     /* 1: replay each GMRES restart cycle as a captured
                                              hipGraph (not while kernel timers run); 0: launch */
+  int32_t pc_coarse;                      /* 1: add a coarse correction to block Jacobi (one value
+                                             per block, blocks aggregated into at most 128 groups
+                                             solved exactly); not in the reference, whose PETSc
+                                             default is one-level block Jacobi.  Used only when the
+                                             mesh fits its limits (<= 32 neighbour blocks per block,
+                                             <= 8192 blocks).  Default 0. */
 } tdyhip_solver_opts;
 
 typedef struct {

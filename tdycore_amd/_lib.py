@@ -50,7 +50,8 @@ class SolverOpts(C.Structure):
     _fields_ = [("snes_rtol", C.c_double), ("snes_atol", C.c_double), ("snes_stol", C.c_double),
                 ("snes_max_it", C.c_int32), ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double),
                 ("ksp_max_it", C.c_int32), ("ksp_restart", C.c_int32), ("krylov_fp32", C.c_int32),
-                ("pc_block", C.c_int32), ("hip_graphs", C.c_int32)]
+                ("pc_block", C.c_int32), ("hip_graphs", C.c_int32),
+                ("pc_coarse", C.c_int32)]
 
 
 class SolverStats(C.Structure):

@@ -1,0 +1,225 @@
+// Coarse correction for the block-Jacobi ILU(0) preconditioner (optional;
+// PETSc's default that the reference runs with is one-level block Jacobi).
+//
+//   M^-1 r = BJ^-1 r + R1 [ G1^-1 + R2 A2^-1 R2^T ] R1^T r
+//
+// Level 1 has one unknown per ILU block (R1 sums a block's rows), with the
+// Galerkin operator A1 = R1^T J R1; its blocks are aggregated into groups of
+// at most kGMax, each group's diagonal block of A1 is inverted exactly (G1),
+// and level 2 (one unknown per group, A2 = R2^T A1 R2, at most kNG of them) is
+// inverted exactly as a dense matrix.  Everything is additive, linear and
+// fixed for one Newton step, so GMRES sees a constant preconditioner.
+// Restriction to level 1 is fused into the ILU sweep kernels (one block sum
+// per workgroup); the level-1/2 work is a few thousand values; prolongation
+// adds one value per block to the preconditioned vector.
+//
+// All reductions run in a fixed order, so the correction is deterministic.
+#include "tdyhip_internal.h"
+
+namespace tdyhip {
+
+constexpr int kCQ = 32;     // neighbour blocks of a block (itself included)
+constexpr int kGMax = 64;   // blocks per level-1 group
+constexpr int kNG = 128;    // level-2 unknowns (dense inverse in LDS)
+
+struct CoarseLayout {
+  int nb = 0, ng = 0, gsz = 0;    // blocks, groups, max blocks per group
+  int32_t *nbr = nullptr;         // nb*kCQ neighbour block ids, ascending (-1 pad)
+  uint8_t *cq = nullptr;          // W*NB: position in nbr[block(p)] of slot k's column block (255 none)
+  int32_t *grp = nullptr;         // nb: group of each block
+  int32_t *gmem = nullptr;        // ng*gsz: blocks of each group (-1 pad)
+  double *A1 = nullptr;           // nb*kCQ: Galerkin rows (column = neighbour position)
+  double *G1 = nullptr;           // ng*gsz*gsz: inverse of each group's diagonal block
+  double *A2 = nullptr;           // ng*ng: level-2 operator, inverted in place
+  double *rc1 = nullptr;          // nb: restricted vector
+  double *z1 = nullptr;           // nb: level-1 correction
+  double *r2 = nullptr;           // ng
+  int *status = nullptr;          // nonzero: singular coarse block
+};
+
+// A1[b][q] = sum over the block's rows and slots whose column lies in block
+// nbr[b][q] of J (one workgroup per block, one thread per row).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, int W, const double *__restrict__ J) {
+  __shared__ double sh[BS / 64];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int64_t p = (int64_t)b * BS + t;
+  uint8_t q_of[32];
+  double v_of[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    q_of[k] = 255;
+    v_of[k] = 0.0;
+    if (k < W) {
+      q_of[k] = C.cq[k * NB + p];
+      v_of[k] = q_of[k] != 255 ? J[k * NB + p] : 0.0;
+    }
+  }
+  for (int q = 0; q < kCQ; q++) {
+    if (C.nbr[b * kCQ + q] < 0) break;  // uniform over the workgroup
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 32; k++) s += (q_of[k] == q) ? v_of[k] : 0.0;
+    s = block_sum_n<BS>(s, sh);
+    if (t == 0) C.A1[b * kCQ + q] = s;
+  }
+}
+
+// In-place Gauss-Jordan inverse of the n x n matrix M (row-major in LDS,
+// leading dimension ld) with row pivoting; all threads of the workgroup
+// call it.  piv and col are LDS scratch of n entries.  Returns false if a
+// pivot is zero.
+__device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col, int *flag) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (t == 0) *flag = 0;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    if (t == 0) {  // pivot search (n <= 128: one thread)
+      int p = k;
+      double mx = fabs(M[k * ld + k]);
+      for (int i = k + 1; i < n; i++) {
+        const double a = fabs(M[i * ld + k]);
+        if (a > mx) { mx = a; p = i; }
+      }
+      piv[k] = p;
+      if (!(mx > 0.0)) *flag = 1;
+    }
+    __syncthreads();
+    if (*flag) return false;
+    const int p = piv[k];
+    if (p != k)
+      for (int j = t; j < n; j += nt) {
+        const double a = M[k * ld + j];
+        M[k * ld + j] = M[p * ld + j];
+        M[p * ld + j] = a;
+      }
+    __syncthreads();
+    const double inv = 1.0 / M[k * ld + k];
+    __syncthreads();
+    for (int j = t; j < n; j += nt) M[k * ld + j] = (j == k) ? inv : M[k * ld + j] * inv;
+    for (int i = t; i < n; i += nt) col[i] = (i == k) ? 0.0 : M[i * ld + k];
+    __syncthreads();
+    for (int e = t; e < n * n; e += nt) {
+      const int i = e / n, j = e % n;
+      if (i == k) continue;
+      const double a = (j == k) ? 0.0 : M[i * ld + j];
+      M[i * ld + j] = a - col[i] * M[k * ld + j];
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = n - 1; k >= 0; k--) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = t; i < n; i += nt) {
+        const double a = M[i * ld + k];
+        M[i * ld + k] = M[i * ld + p];
+        M[i * ld + p] = a;
+      }
+    __syncthreads();
+  }
+  return true;
+}
+
+// One workgroup (kGMax threads) per group: invert the group's diagonal block
+// of A1 into G1 and form the group's row of A2.
+__global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
+  __shared__ double M[kGMax * kGMax];
+  __shared__ double col[kGMax];
+  __shared__ int piv[kGMax];
+  __shared__ int flag;
+  __shared__ int mem[kGMax];
+  const int g = blockIdx.x, t = threadIdx.x, G = C.gsz;
+  if (t < G) mem[t] = C.gmem[g * G + t];
+  __syncthreads();
+  int n = 0;
+  while (n < G && mem[n] >= 0) n++;
+  // row i of the diagonal block: A1 of member i at the columns of member j
+  for (int e = t; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n, bi = mem[i], bj = mem[j];
+    double a = 0.0;
+    for (int q = 0; q < kCQ; q++)
+      if (C.nbr[bi * kCQ + q] == bj) a = C.A1[bi * kCQ + q];
+    M[i * n + j] = a;
+  }
+  __syncthreads();
+  const bool ok = gj_invert(M, n, n, piv, col, &flag);
+  if (!ok) {
+    if (t == 0) atomicOr(C.status, 1);
+    return;
+  }
+  for (int e = t; e < G * G; e += blockDim.x) {
+    const int i = e / G, j = e % G;
+    C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * n + j] : 0.0;
+  }
+  // A2[g][h] = sum over members and their neighbour entries in group h,
+  // from an LDS list of (group, value) in a fixed order
+  __shared__ int gq[kGMax * kCQ];
+  __shared__ double aq[kGMax * kCQ];
+  for (int e = t; e < n * kCQ; e += blockDim.x) {
+    const int i = e / kCQ, q = e % kCQ;
+    const int nb = C.nbr[mem[i] * kCQ + q];
+    gq[e] = nb >= 0 ? C.grp[nb] : -1;
+    aq[e] = nb >= 0 ? C.A1[mem[i] * kCQ + q] : 0.0;
+  }
+  __syncthreads();
+  for (int h = t; h < C.ng; h += blockDim.x) {
+    double s = 0.0;
+    for (int e = 0; e < n * kCQ; e++) s += (gq[e] == h) ? aq[e] : 0.0;
+    C.A2[(int64_t)g * C.ng + h] = s;
+  }
+}
+
+// Single workgroup: A2 <- A2^-1 (ng <= kNG, held in LDS).
+__global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
+  __shared__ double M[kNG * kNG];
+  __shared__ double col[kNG];
+  __shared__ int piv[kNG];
+  __shared__ int flag;
+  const int n = C.ng;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[e] = C.A2[e];
+  __syncthreads();
+  if (!gj_invert(M, n, n, piv, col, &flag)) {
+    if (threadIdx.x == 0) atomicOr(C.status, 1);
+    return;
+  }
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) C.A2[e] = M[e];
+}
+
+// Level 1 of an application: z1 = G1 rc1 per group, r2 = group sums of rc1.
+__global__ void __launch_bounds__(kGMax) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
+  __shared__ double sh[1];
+  if (stop && *stop) return;
+  const int g = blockIdx.x, t = threadIdx.x, G = C.gsz;
+  const int bt = t < G ? C.gmem[g * G + t] : -1;
+  const double rt = bt >= 0 ? C.rc1[bt] : 0.0;
+  double z = 0.0;
+  for (int j = 0; j < G; j++) {
+    const int bj = C.gmem[g * G + j];
+    if (bj >= 0 && bt >= 0) z += C.G1[(int64_t)g * G * G + t * G + j] * C.rc1[bj];
+  }
+  if (bt >= 0) C.z1[bt] = z;
+  const double s = block_sum_n<kGMax>(rt, sh);
+  if (t == 0) C.r2[g] = s;
+}
+
+// Prolongation: out[row] += z1[b] + (A2^-1 r2)[group(b)] for the rows of block b.
+template <class TV, int BS>
+__global__ void __launch_bounds__(BS) k_coarse_prolong(CoarseLayout C, const int32_t *__restrict__ rowof,
+                                                       TV *__restrict__ out, const int *__restrict__ stop) {
+  __shared__ double corr;
+  if (stop && *stop) return;
+  const int b = blockIdx.x, t = threadIdx.x;
+  if (t < 64) {  // one wavefront: the level-2 row of this block's group
+    const int g = C.grp[b];
+    double s = 0.0;
+    for (int h = t; h < C.ng; h += 64) s += C.A2[(int64_t)g * C.ng + h] * C.r2[h];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (t == 0) corr = C.z1[b] + s;
+  }
+  __syncthreads();
+  const int row = rowof[(int64_t)b * BS + t];
+  if (row >= 0) out[row] += (TV)corr;
+}
+
+}  // namespace tdyhip

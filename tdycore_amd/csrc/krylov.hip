@@ -134,6 +134,29 @@ __global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__re
   fac[sbase + (int64_t)(2 * kLU) * BS + t] = nsl > 0 ? (TV)1 / a[di][t] : (TV)0;
 }
 
+// Sum over an NT-thread workgroup; every thread gets the result.
+template <int NT, class TV>
+__device__ inline TV block_sum_n(TV v, TV *sh) {
+  const int t = threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  TV s = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) s += sh[w];
+  __syncthreads();
+  return s;
+}
+
+// Restriction of the block's right-hand side to one value (the coarse
+// correction's R1^T r), written by thread 0; all threads call it.
+template <class TV, int BS>
+__device__ inline void restrict_block(const TV *x, double *rc1, int b, int t) {
+  __shared__ TV sh[BS / 64];
+  const TV s = block_sum_n<BS>(x[t], sh);
+  if (t == 0) rc1[b] = (double)s;
+}
+
 // ---------------------------------------------------------------- ILU(0) apply: y = (LU)^-1 r
 // Forward and backward sweeps over the block's levels with x (BS+1 entries,
 // x[BS] == 0 the target of padded slots) in LDS.  Every thread of the block
@@ -180,13 +203,15 @@ __device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, 
 
 template <class TV, int BS>
 __global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restrict__ fac, const TV *__restrict__ r,
-                                                 TV *__restrict__ y, const int *__restrict__ stop = nullptr) {
+                                                 TV *__restrict__ y, const int *__restrict__ stop = nullptr,
+                                                 double *__restrict__ rc1 = nullptr) {
   __shared__ TV x[BS + 1];
   if (stop && *stop) return;  // uniform over the workgroup
   const int b = blockIdx.x, t = threadIdx.x;
   const int row = L.d_row[(int64_t)b * BS + t];
   x[t] = row >= 0 ? r[row] : (TV)0;
   if (t == 0) x[BS] = (TV)0;
+  if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform
   bj_sweeps<TV, BS>(L, fac, x, b, t);
   if (row >= 0) y[row] = x[t];
 }
@@ -250,7 +275,7 @@ __global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, in
                                                       const TV *__restrict__ a, const TV *__restrict__ u,
                                                       const GmresDev *__restrict__ gd, TV *__restrict__ vj,
                                                       const TV *__restrict__ fac, TV *__restrict__ out,
-                                                      const int *__restrict__ stop) {
+                                                      const int *__restrict__ stop, double *__restrict__ rc1) {
   __shared__ TV x[BS + 1];
   if (stop && *stop) return;  // uniform over the workgroup
   const int b = blockIdx.x, t = threadIdx.x;
@@ -261,6 +286,7 @@ __global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, in
   x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, a, u) : (TV)0;
   if (row >= 0) vj[row] = sc * u[row];
   if (t == 0) x[BS] = (TV)0;
+  if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform
   bj_sweeps<TV, BS>(L, fac, x, b, t);
   if (row >= 0) out[row] = x[t];
 }

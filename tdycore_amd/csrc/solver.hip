@@ -8,6 +8,7 @@
 #include <deque>
 
 #include "krylov.hip"
+#include "coarse.hip"
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
@@ -205,6 +206,88 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   return std::string();
 }
 
+struct HostCoarse {
+  int nb = 0, ng = 0, gsz = 0;
+  std::vector<int32_t> nbr, grp, gmem;
+  std::vector<uint8_t> cq;
+};
+
+// Level-1 / level-2 structure of the coarse correction: neighbour blocks of
+// each block, the slot -> neighbour map, and groups of blocks built by
+// breadth-first aggregation over block adjacency (group size grown until at
+// most kNG groups remain).  Returns an empty string when the mesh is outside
+// the correction's limits (then it is not used).
+static std::string build_coarse(const HostBJ &hb, const std::vector<int32_t> &jrp, const std::vector<int32_t> &jcols,
+                                int nc, int W, HostCoarse &out) {
+  const int S = hb.bs, nb = hb.nblocks;
+  const int64_t NB = (int64_t)nb * S;
+  std::vector<int32_t> blk(nc);
+  for (int c = 0; c < nc; c++) blk[c] = hb.perm[c] / S;
+  out.nb = nb;
+  out.nbr.assign((size_t)nb * kCQ, -1);
+  std::vector<int32_t> tmp;
+  for (int b = 0; b < nb; b++) {
+    tmp.clear();
+    for (int i = 0; i < S; i++) {
+      const int r = hb.row[(size_t)b * S + i];
+      if (r < 0) continue;
+      for (int k = jrp[r]; k < jrp[r + 1]; k++) tmp.push_back(blk[jcols[k]]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    if ((int)tmp.size() > kCQ) return "a block has more than 32 neighbour blocks";
+    for (size_t q = 0; q < tmp.size(); q++) out.nbr[(size_t)b * kCQ + q] = tmp[q];
+  }
+  out.cq.assign((size_t)W * NB, 255);
+  for (int64_t p = 0; p < NB; p++) {
+    const int r = hb.row[p];
+    if (r < 0) continue;
+    const int b = (int)(p / S);
+    const int32_t *nl = &out.nbr[(size_t)b * kCQ];
+    for (int k = 0; k < jrp[r + 1] - jrp[r]; k++) {
+      const int cb = blk[jcols[jrp[r] + k]];
+      for (int q = 0; q < kCQ && nl[q] >= 0; q++)
+        if (nl[q] == cb) { out.cq[(size_t)k * NB + p] = (uint8_t)q; break; }
+    }
+  }
+  // groups
+  for (int G = std::max(1, (nb + kNG - 1) / kNG); G <= kGMax; G++) {
+    std::vector<int32_t> grp(nb, -1), mem;
+    std::vector<std::vector<int32_t>> groups;
+    std::vector<int32_t> queue;
+    for (int seed = 0; seed < nb; seed++) {
+      if (grp[seed] >= 0) continue;
+      const int g = (int)groups.size();
+      groups.emplace_back();
+      queue.assign(1, seed);
+      grp[seed] = g;
+      size_t head = 0;
+      while (head < queue.size() && (int)groups[g].size() < G) {
+        const int b = queue[head++];
+        groups[g].push_back(b);
+        for (int q = 0; q < kCQ; q++) {
+          const int nbk = out.nbr[(size_t)b * kCQ + q];
+          if (nbk < 0) break;
+          if (grp[nbk] < 0 && (int)queue.size() < G) { grp[nbk] = g; queue.push_back(nbk); }
+        }
+      }
+      for (size_t q = head; q < queue.size(); q++) {  // queued beyond the group size: release
+        if ((int)groups[g].size() < G) groups[g].push_back(queue[q]);
+        else grp[queue[q]] = -1;
+      }
+    }
+    if ((int)groups.size() > kNG) continue;
+    out.ng = (int)groups.size();
+    out.gsz = G;
+    out.grp = grp;
+    out.gmem.assign((size_t)out.ng * G, -1);
+    for (int g = 0; g < out.ng; g++)
+      for (size_t i = 0; i < groups[g].size(); i++) out.gmem[(size_t)g * G + i] = groups[g][i];
+    return std::string();
+  }
+  return "too many blocks for the coarse correction";
+}
+
 struct SolverWS {
   int n = 0, restart = 0, fp32 = -1, bs = 0, part_version = -1;
   int64_t nnz = 0;
@@ -226,6 +309,10 @@ struct SolverWS {
   int32_t *d_perm = nullptr;     // nc: block-layout position of each cell
   int32_t *d_ecol = nullptr;     // W*NB block-ordered ELL columns (int32), or
   int16_t *d_ecol16 = nullptr;   // the same as int16 offsets from the row (one of the two is set)
+  // coarse correction (when requested and the mesh is within its limits)
+  bool coarse = false;
+  int coarse_req = -1;
+  CoarseLayout cl;
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
@@ -260,7 +347,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   // the FP32 Jacobian copy), so switching precision costs nothing
   (void)fp32;
   if (ws.n == ctx->nc && ws.restart == o.ksp_restart && ws.bs == o.pc_block &&
-      ws.part_version == ctx->pc_part_version)
+      ws.part_version == ctx->pc_part_version && ws.coarse_req == (o.pc_coarse ? 1 : 0))
     return 0;
   ws_free(ws);
   HostBJ hb;
@@ -315,6 +402,24 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if ((rc = ws_alloc(ws, &ws.hs, (size_t)(o.ksp_restart + 1) * o.ksp_restart + 4 * (o.ksp_restart + 1)))) return rc;
   if ((rc = ws_alloc(ws, &ws.gdev, 1))) return rc;
   if ((rc = ws_alloc(ws, &ws.status, 1))) return rc;
+  ws.coarse_req = o.pc_coarse ? 1 : 0;
+  if (o.pc_coarse) {
+    HostCoarse hc;
+    if (build_coarse(hb, ctx->h_jrp, ctx->h_jcols, ctx->nc, ctx->jell_w, hc).empty()) {
+      CoarseLayout &C = ws.cl;
+      C.nb = hc.nb; C.ng = hc.ng; C.gsz = hc.gsz;
+      if ((rc = up(&C.nbr, hc.nbr)) || (rc = up(&C.cq, hc.cq)) || (rc = up(&C.grp, hc.grp)) ||
+          (rc = up(&C.gmem, hc.gmem)))
+        return rc;
+      if ((rc = ws_alloc(ws, &C.A1, (size_t)hc.nb * kCQ)) ||
+          (rc = ws_alloc(ws, &C.G1, (size_t)hc.ng * hc.gsz * hc.gsz)) ||
+          (rc = ws_alloc(ws, &C.A2, (size_t)hc.ng * hc.ng)) || (rc = ws_alloc(ws, &C.rc1, (size_t)hc.nb)) ||
+          (rc = ws_alloc(ws, &C.z1, (size_t)hc.nb)) || (rc = ws_alloc(ws, &C.r2, (size_t)hc.ng)))
+        return rc;
+      C.status = ws.status;
+      ws.coarse = true;
+    }
+  }
   return 0;
 }
 
@@ -389,10 +494,32 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  // coarse correction: Galerkin rows, group inverses and the level-2 inverse
+  // for this Jacobian; each application then adds its prolongation to the
+  // block-Jacobi result (the restriction is fused into the ILU sweeps)
+  double *rc1 = ws.coarse ? ws.cl.rc1 : nullptr;
+  if (ws.coarse) {
+    const CoarseLayout &C = ws.cl;
+    const int W = ctx->jell_w;
+    if (ws.bj.bs == 64) k_coarse_rows<64><<<C.nb, 64, 0, st>>>(C, ws.NB, W, ws.J);
+    else if (ws.bj.bs == 256) k_coarse_rows<256><<<C.nb, 256, 0, st>>>(C, ws.NB, W, ws.J);
+    else k_coarse_rows<512><<<C.nb, 512, 0, st>>>(C, ws.NB, W, ws.J);
+    k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C);
+    k_coarse_invert<<<1, 1024, 0, st>>>(C);
+  }
+  auto coarse_add = [&](TV *out, const int *stp) {
+    if (!ws.coarse) return;
+    const CoarseLayout &C = ws.cl;
+    k_coarse_level1<<<C.ng, kGMax, 0, st>>>(C, stp);
+    if (ws.bj.bs == 64) k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp);
+    else if (ws.bj.bs == 256) k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp);
+    else k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp);
+  };
   auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
-    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp));
-    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp));
-    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp));
+    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
+    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
+    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
+    coarse_add(out, stp);
   };
   // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused into the ILU sweeps)
   auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp) {
@@ -401,15 +528,16 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     const unsigned nb = ws.bj.nblocks;
     if (ws.d_ecol16) {
       const int16_t *ec = ws.d_ecol16;
-      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int16_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
-      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int16_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
-      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int16_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int16_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int16_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int16_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
     } else {
       const int32_t *ec = ws.d_ecol;
-      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int32_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
-      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int32_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
-      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int32_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp));
+      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int32_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int32_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int32_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
     }
+    coarse_add(out, stp);
   };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
@@ -648,7 +776,7 @@ void tdyhip_solver_defaults(tdyhip_solver_opts *o) {
   if (!o) return;
   o->snes_rtol = 1e-8; o->snes_atol = 1e-50; o->snes_stol = 1e-8; o->snes_max_it = 50;
   o->ksp_rtol = 1e-5; o->ksp_atol = 1e-50; o->ksp_max_it = 10000; o->ksp_restart = 30;
-  o->krylov_fp32 = 0; o->pc_block = 256; o->hip_graphs = 1;
+  o->krylov_fp32 = 0; o->pc_block = 256; o->hip_graphs = 1; o->pc_coarse = 0;
 }
 
 static int check_opts(const tdyhip_solver_opts *o) {

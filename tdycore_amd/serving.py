@@ -100,8 +100,9 @@ class EnsembleServer:
 
     def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, build_workers: int = 8,
                  krylov_fp32: bool = True, snes_rtol: float = 1e-10, brick=(8, 8, 4),
-                 solve_fn: Optional[Callable] = None):
+                 solve_fn: Optional[Callable] = None, pc_coarse: bool = True):
         self.slots = slots
+        self.pc_coarse = pc_coarse
         self.brick = tuple(brick)
         self.krylov_fp32 = krylov_fp32
         self.snes_rtol = snes_rtol
@@ -204,7 +205,8 @@ class EnsembleServer:
         try:
             g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
             bs = self.brick[0] * self.brick[1] * self.brick[2]
-            o = g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs)
+            o = g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs,
+                                 pc_coarse=1 if self.pc_coarse else 0)
             # all device work of this request stays on the context's own stream
             # (never a torch pool stream, which another slot could be using)
             with torch.cuda.stream(g.stream):

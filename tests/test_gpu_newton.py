@@ -31,11 +31,12 @@ def test_device_newton_reproduces_reference_gold(name):
     _check(U.cpu().numpy(), case["expected"])
 
 
-@pytest.mark.parametrize("fp32,pc_block", [(0, 256), (1, 256), (0, 512), (1, 512), (1, 64)])
-def test_device_newton_matches_oracle_cfg0(fp32, pc_block):
+@pytest.mark.parametrize("fp32,pc_block,coarse", [(0, 256, 0), (1, 256, 0), (0, 512, 0), (1, 512, 0), (1, 64, 0),
+                                                  (0, 64, 1), (1, 64, 1), (0, 256, 1)])
+def test_device_newton_matches_oracle_cfg0(fp32, pc_block, coarse):
     """Plain-method configuration on a reduced mesh: device Newton (FP64 or
     FP32 Krylov) reaches the oracle's solution to 1e-8 relative in pressure."""
-    print(f"[case] cfg0 Newton fp32={fp32} pc_block={pc_block}", flush=True)
+    print(f"[case] cfg0 Newton fp32={fp32} pc_block={pc_block} coarse={coarse}", flush=True)
     import torch
     from pyoracle import Oracle
     from tdycore_amd.mpfao import MPFAO
@@ -46,7 +47,7 @@ def test_device_newton_matches_oracle_cfg0(fp32, pc_block):
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     U = torch.tensor(P["U0"], device="cuda")
     st = g.run_snes_steps(U, 3600.0, 3, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32, ksp_rtol=1e-8,
-                                                         pc_block=pc_block))
+                                                         pc_block=pc_block, pc_coarse=coarse))
     assert st["status"] == 0
     err = np.abs(U.cpu().numpy() - Uo).max() / np.abs(Uo).max()
     print("max rel pressure err", err, st)
@@ -61,14 +62,15 @@ def test_device_newton_is_deterministic():
     from tdycore_amd.mpfao import MPFAO
     P = cfg0_problem(n=12)
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
-    for fp32 in (0, 1):
+    for fp32, coarse in ((0, 0), (1, 0), (0, 1)):
         outs = []
         # graph replays (first two runs: capture, then replay across steps and
         # Newton iterations with eager work between) against direct launches
         for graphs in (1, 1, 0):
             U = torch.tensor(P["U0"], device="cuda")
             st = g.run_snes_steps(U, 3600.0, 2, g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32,
-                                                                 hip_graphs=graphs))
+                                                                 hip_graphs=graphs, pc_block=64,
+                                                                 pc_coarse=coarse))
             assert st["status"] == 0
             outs.append(U.cpu().numpy())
         assert np.array_equal(outs[0], outs[1]), fp32
