@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests on the device")
     ap.add_argument("--build-workers", type=int, default=12, help="serving: host threads building problems")
     ap.add_argument("--pc-coarse", type=int, default=1, help="coarse correction on block Jacobi (1) or not (0)")
+    ap.add_argument("--serve-fp32", type=int, default=0, help="serving: FP32 Krylov (1) or FP64 (0)")
+    ap.add_argument("--max-seconds", type=float, default=0.0,
+                    help="serving: after this long, requests not yet on the device are dropped (0: no limit)")
     return ap.parse_args()
 
 
@@ -183,7 +186,8 @@ def serve(a):
     from tdycore_amd import serving, workloads
     torch.cuda.init()
     specs = workloads.ensemble_request_specs(a.requests)
-    srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers, pc_coarse=bool(a.pc_coarse))
+    srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers, pc_coarse=bool(a.pc_coarse),
+                                 krylov_fp32=bool(a.serve_fp32))
     # warm the library and the device once (not timed)
     srv.submit(serving.Request(rid=-1, nx=16, ny=16, nz=8, seed=1, sigma=1.0, steps=1)).result()
     rng = np.random.default_rng(20261207)
@@ -195,6 +199,8 @@ def serve(a):
 
     def report(f):
         ndone[0] += 1
+        if f.cancelled():
+            return
         if f.exception() is None:
             r = f.result()
             done_res.append(r)
@@ -215,12 +221,21 @@ def serve(a):
             time.sleep(dt)
         futs.append(srv.submit(serving.Request(**sp)))
         futs[-1].add_done_callback(report)
-    res, failed = [], []
+        if a.max_seconds and time.perf_counter() - t0 > a.max_seconds:
+            break
+    if a.max_seconds:
+        while time.perf_counter() - t0 < a.max_seconds and not all(f.done() for f in futs):
+            time.sleep(1.0)
+        srv.drain()
+    res, failed, dropped = [], [], []
     for sp, f in zip(specs, futs):
-        if f.exception() is None:
+        if f.cancelled():
+            dropped.append(sp["rid"])
+        elif f.exception() is None:
             res.append(f.result())
         else:  # e.g. the time integrator's 25 dt cuts: reported, not retried
             failed.append({"rid": sp["rid"], "error": str(f.exception())})
+    dropped += [sp["rid"] for sp in specs[len(futs):]]
     srv.close()
     summ = serving.latency_summary(res)
     bad = [r["rid"] for r in res if r["stats"]["status"] != 0] + [f["rid"] for f in failed]
@@ -233,9 +248,10 @@ def serve(a):
         "data": "synthetic (seeded jittered boxes, rotated log-normal permeability)",
         "config": {"workload": f"{len(res)} requests, Poisson arrivals at {a.rate:g}/s, nx,ny~U{{16..96}}, "
                                f"nz~U{{8..48}}, sigma~U(0.5,1.5), 5 backward-Euler steps of 1 h, FP64 residual, "
-                               f"FP32 Krylov, {a.slots} device slots",
+                               f"{'FP32' if a.serve_fp32 else 'FP64'} Krylov, {a.slots} device slots",
                    "total_cells": cells, "newton_its": newton, "failed": bad, "failures": failed[:20],
-                   "pc_coarse": a.pc_coarse},
+                   "not_served": len(dropped), "served": len(res), "max_seconds": a.max_seconds,
+                   "pc_coarse": a.pc_coarse, "krylov_fp32": a.serve_fp32},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"], "serving": summ,
         "peak_inflight": srv.peak_inflight, "peak_admitted_bytes": srv.peak_used, "hbm_budget": srv.hbm_budget,
     }

@@ -66,72 +66,97 @@ struct GmresDev {
 };
 
 // ---------------------------------------------------------------- ILU(0) factor
+// TPR lanes (adjacent, so in one wavefront) share a row: each computes the
+// row's multiplier for lower slot p and applies its share of the slot's
+// update pairs (the targets of one slot are distinct), which shortens the
+// sequential elimination chain of each level about TPR times.  Lanes of one
+// wavefront issue their LDS operations in program order, so the writes of
+// slot p are seen by the reads of slot p+1 without a barrier.
+template <int BS>
+struct FactorLanes {
+  static constexpr int TPR = BS <= 256 ? 4 : 2;  // lanes per row (workgroup <= 1024 threads)
+  static constexpr int NT = BS * TPR;
+  static constexpr int WPL = (kPairW + TPR - 1) / TPR;  // pair words per lane
+};
+
 template <class TV, int BS>
-__global__ void __launch_bounds__(BS) k_bj_factor(BJLayout L, const double *__restrict__ J, TV *__restrict__ fac,
-                                                  int *__restrict__ status) {
+__global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, const double *__restrict__ J,
+                                                                   TV *__restrict__ fac, int *__restrict__ status) {
+  constexpr int TPR = FactorLanes<BS>::TPR, NT = FactorLanes<BS>::NT, WPL = FactorLanes<BS>::WPL;
   __shared__ TV a[kW][BS];
   __shared__ uint16_t lc[kW][BS];
   __shared__ uint8_t dg[BS];
   __shared__ uint8_t ns[BS];
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int b = blockIdx.x, tid = threadIdx.x;
   const int64_t base = (int64_t)b * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
-  const int nsl = L.d_nslot[base + t];
-  ns[t] = (uint8_t)nsl;
-  dg[t] = L.d_diag[base + t];
-  for (int p = 0; p < kW; p++) {
-    const int32_t pos = L.d_jpos[sbase + (int64_t)p * BS + t];
-    a[p][t] = (pos >= 0) ? (TV)J[pos] : (TV)0;
-    lc[p][t] = L.d_lcol[sbase + (int64_t)p * BS + t];
+  for (int e = tid; e < kW * BS; e += NT) {
+    const int32_t pos = L.d_jpos[sbase + e];
+    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos] : (TV)0;
+    lc[e / BS][e % BS] = L.d_lcol[sbase + e];
   }
+  for (int r = tid; r < BS; r += NT) {
+    ns[r] = L.d_nslot[base + r];
+    dg[r] = L.d_diag[base + r];
+  }
+  const int t = tid / TPR, sub = tid % TPR;  // row of the block, lane within the row
   const int lev = L.d_levL[base + t];
   const int nlev = L.d_nlev[2 * b];
-  // The update lists of lower slot p+1 (count and up to 13 packed pairs) are
-  // fetched while slot p is eliminated, and slot 0's before the level loop, so
-  // global-memory latency is off the sequential elimination chain.
+  const int nsl = L.d_nslot[base + t];
+  const int di0 = L.d_diag[base + t];
+  // The update lists of lower slot p+1 (count and this lane's packed pair
+  // words) are fetched while slot p is eliminated, and slot 0's before the
+  // level loop, so global-memory latency is off the elimination chain.
   const int64_t pbase = (int64_t)b * kPair * BS;
   const int64_t nbase = (int64_t)b * kLU * BS;
-  const int di0 = L.d_diag[base + t];
-  auto fetch = [&](int p, int &cnt, uint32_t (&w)[kPairW]) {
+  auto fetch = [&](int p, int &cnt, uint32_t (&w)[WPL]) {
     cnt = (p < di0) ? L.d_npair[nbase + (int64_t)p * BS + t] : 0;
 #pragma unroll
-    for (int q = 0; q < kPairW; q++)
-      w[q] = (p < di0) ? L.d_pair[pbase + (int64_t)(p * kPairW + q) * BS + t] : 0u;
+    for (int i = 0; i < WPL; i++) {
+      const int wq = sub + TPR * i;
+      w[i] = (p < di0 && wq < kPairW) ? L.d_pair[pbase + (int64_t)(p * kPairW + wq) * BS + t] : 0u;
+    }
   };
   int ncur;
-  uint32_t cur[kPairW];
+  uint32_t cur[WPL];
   fetch(0, ncur, cur);
   __syncthreads();
   for (int l = 0; l < nlev; l++) {
     if (lev == l && nsl > 0) {
       for (int p = 0; p < di0; p++) {  // lower entries, increasing column
         int nnext;
-        uint32_t nxt[kPairW];
+        uint32_t nxt[WPL];
         fetch(p + 1, nnext, nxt);
         const int k = lc[p][t];
         const TV lik = a[p][t] / a[dg[k]][k];
-        a[p][t] = lik;
+        __builtin_amdgcn_wave_barrier();  // every lane has read a[p][t] before it is overwritten
+        if (sub == 0) a[p][t] = lik;
 #pragma unroll
-        for (int q = 0; q < kLU; q++) {
-          const uint32_t w = (q & 1) ? (cur[q / 2] >> 16) : (cur[q / 2] & 0xffffu);
-          if (q < ncur) a[w & 0xff][t] -= lik * a[w >> 8][k];
+        for (int i = 0; i < WPL; i++) {
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int q = 2 * (sub + TPR * i) + h;  // pair index of this lane's half-word
+            const uint32_t w = h ? (cur[i] >> 16) : (cur[i] & 0xffffu);
+            if (q < ncur) a[w & 0xff][t] -= lik * a[w >> 8][k];
+          }
         }
+        __builtin_amdgcn_wave_barrier();
         ncur = nnext;
 #pragma unroll
-        for (int q = 0; q < kPairW; q++) cur[q] = nxt[q];
+        for (int i = 0; i < WPL; i++) cur[i] = nxt[i];
       }
-      if (a[dg[t]][t] == (TV)0) atomicOr(status, 1);
+      if (sub == 0 && a[dg[t]][t] == (TV)0) atomicOr(status, 1);
     }
     __syncthreads();
   }
   // split layout: L multipliers, U entries, inverse diagonal
   const int di = dg[t];
-  for (int p = 0; p < kLU; p++) {
+  for (int p = sub; p < kLU; p += TPR) {
     fac[sbase + (int64_t)p * BS + t] = (p < di) ? a[p][t] : (TV)0;
     const int q = di + 1 + p;
     fac[sbase + (int64_t)(kLU + p) * BS + t] = (q < nsl) ? a[q][t] : (TV)0;
   }
-  fac[sbase + (int64_t)(2 * kLU) * BS + t] = nsl > 0 ? (TV)1 / a[di][t] : (TV)0;
+  if (sub == 0) fac[sbase + (int64_t)(2 * kLU) * BS + t] = nsl > 0 ? (TV)1 / a[di][t] : (TV)0;
 }
 
 // Sum over an NT-thread workgroup; every thread gets the result.

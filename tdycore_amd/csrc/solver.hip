@@ -491,9 +491,9 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   if (!std::is_same<TV, double>::value)
     k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
   k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
-  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, ws.J, fac, ws.status));
-  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, ws.J, fac, ws.status));
-  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, FactorLanes<512>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   // coarse correction: Galerkin rows, group inverses and the level-2 inverse
   // for this Jacobian; each application then adds its prolongation to the
   // block-Jacobi result (the restriction is fused into the ILU sweeps)
@@ -501,19 +501,22 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   if (ws.coarse) {
     const CoarseLayout &C = ws.cl;
     const int W = ctx->jell_w;
-    if (ws.bj.bs == 64) k_coarse_rows<64><<<C.nb, 64, 0, st>>>(C, ws.NB, W, ws.J);
-    else if (ws.bj.bs == 256) k_coarse_rows<256><<<C.nb, 256, 0, st>>>(C, ws.NB, W, ws.J);
-    else k_coarse_rows<512><<<C.nb, 512, 0, st>>>(C, ws.NB, W, ws.J);
-    k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C);
-    k_coarse_invert<<<1, 1024, 0, st>>>(C);
+    const double brows = (double)ws.NB * W * (8 + 1) + (double)C.nb * kCQ * (4 + 8);
+    if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<64><<<C.nb, 64, 0, st>>>(C, ws.NB, W, ws.J));
+    else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<256><<<C.nb, 256, 0, st>>>(C, ws.NB, W, ws.J));
+    else KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<512><<<C.nb, 512, 0, st>>>(C, ws.NB, W, ws.J));
+    KT(ctx, st, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C));
+    KT(ctx, st, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, st>>>(C));
   }
   auto coarse_add = [&](TV *out, const int *stp) {
     if (!ws.coarse) return;
     const CoarseLayout &C = ws.cl;
-    k_coarse_level1<<<C.ng, kGMax, 0, st>>>(C, stp);
-    if (ws.bj.bs == 64) k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp);
-    else if (ws.bj.bs == 256) k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp);
-    else k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp);
+    KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24,
+       k_coarse_level1<<<C.ng, kGMax, 0, st>>>(C, stp));
+    const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * C.ng * 8;
+    if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp));
+    else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp));
+    else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
   };
   auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
     if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp, rc1));

@@ -120,6 +120,7 @@ class EnsembleServer:
         self.peak_used = 0
         self.peak_inflight = 0
         self._closed = False
+        self._draining = False
         self._workers = [threading.Thread(target=self._slot, args=(i,), daemon=True, name=f"slot{i}")
                          for i in range(slots)]
         for w in self._workers:
@@ -135,6 +136,20 @@ class EnsembleServer:
                 raise RuntimeError("server is closed")
         self._build.submit(self._build_job, job)
         return fut
+
+    def drain(self):
+        """Stop serving requests that are not on the device yet: their futures
+        are cancelled; requests already solving finish."""
+        self._draining = True
+        while True:
+            try:
+                job = self._ready.get_nowait()
+            except queue.Empty:
+                break
+            if job is None:
+                self._ready.put(None)
+                break
+            job.future.cancel()
 
     def close(self):
         """Finish the queued requests and stop the slot threads."""
@@ -156,6 +171,9 @@ class EnsembleServer:
             job.need = footprint_bytes(m.nc, m.nf, m.nv, m.nbnd, rows, self.krylov_fp32)
             if job.need > self.hbm_budget:
                 raise MemoryError(f"request {r.rid} needs {job.need} B, above the HBM budget {self.hbm_budget} B")
+            if self._draining:
+                job.future.cancel()
+                return
             job.problem = P
             job.t_built = time.perf_counter()
             self._ready.put(job)
@@ -183,6 +201,9 @@ class EnsembleServer:
             job = self._ready.get()
             if job is None:
                 return
+            if self._draining:
+                job.future.cancel()
+                continue
             self._admit(job)
             try:
                 U, st = self._solve(job.problem, job.req)

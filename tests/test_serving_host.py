@@ -71,3 +71,17 @@ def test_request_above_budget_is_rejected():
     with pytest.raises(MemoryError):
         f.result(timeout=60)
     srv.close()
+
+
+def test_drain_cancels_requests_not_yet_on_the_device():
+    def slow(P, r):
+        time.sleep(0.2)
+        return P["U0"], {"status": 0}
+    srv = serving.EnsembleServer(slots=2, hbm_budget=1 << 40, build_workers=2, solve_fn=slow)
+    futs = [srv.submit(serving.Request(rid=i, nx=16, ny=16, nz=8, seed=i, sigma=1.0)) for i in range(16)]
+    time.sleep(0.6)
+    srv.drain()
+    srv.close()
+    done = [f for f in futs if f.done() and not f.cancelled()]
+    assert all(f.done() for f in futs)
+    assert 0 < len(done) < 16 and all(f.result()["stats"]["status"] == 0 for f in done)
