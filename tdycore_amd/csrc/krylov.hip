@@ -575,6 +575,24 @@ __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restr
   arnoldi_update(gd, j, m, h, H, cs, sn, g);
 }
 
+// h[j+1] = sum of the nchunk norm partials (fixed order, as k_reduce_partials)
+// and the Arnoldi/Givens update of column j, in one workgroup.
+template <class TV>
+__global__ void __launch_bounds__(kRedBlock) k_norm_arnoldi(int nchunk, const TV *__restrict__ partial, GmresDev *gd,
+                                                            int j, int m, double *__restrict__ h,
+                                                            double *__restrict__ H, double *__restrict__ cs,
+                                                            double *__restrict__ sn, double *__restrict__ g) {
+  __shared__ TV sh[kRedBlock / 64];
+  if (gd->done) return;
+  TV s = 0;
+  for (int c = threadIdx.x; c < nchunk; c += kRedBlock) s += partial[c];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    h[j + 1] = (double)s;
+    arnoldi_update(gd, j, m, h, H, cs, sn, g);
+  }
+}
+
 // y = H(1:j,1:j)^-1 g(1:j) for the j = jcur columns of the cycle
 __global__ void k_hess_solve(const GmresDev *gd, int m, const double *__restrict__ H, const double *__restrict__ g,
                              double *__restrict__ y) {

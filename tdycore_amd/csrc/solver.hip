@@ -578,8 +578,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
-      k_reduce_partials<TV><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, part, ws.h + j + 1, stop);
-      k_arnoldi_step<<<1, 1, 0, st>>>(gd, j, m, ws.h, dH, dcs, dsn, dg);
+      k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(ws.nchunk, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
     k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
