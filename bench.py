@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--build-workers", type=int, default=12, help="serving: host threads building problems")
     ap.add_argument("--pc-coarse", type=int, default=1, help="coarse correction on block Jacobi (1) or not (0)")
     ap.add_argument("--serve-fp32", type=int, default=0, help="serving: FP32 Krylov (1) or FP64 (0)")
+    ap.add_argument("--deadline", type=float, default=0.0,
+                    help="serving: seconds a request may solve before it is cancelled (0: no limit)")
     ap.add_argument("--max-seconds", type=float, default=0.0,
                     help="serving: after this long, requests not yet on the device are dropped (0: no limit)")
     return ap.parse_args()
@@ -187,7 +189,7 @@ def serve(a):
     torch.cuda.init()
     specs = workloads.ensemble_request_specs(a.requests)
     srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers, pc_coarse=bool(a.pc_coarse),
-                                 krylov_fp32=bool(a.serve_fp32))
+                                 krylov_fp32=bool(a.serve_fp32), deadline_s=a.deadline or None)
     # warm the library and the device once (not timed)
     srv.submit(serving.Request(rid=-1, nx=16, ny=16, nz=8, seed=1, sigma=1.0, steps=1)).result()
     rng = np.random.default_rng(20261207)
@@ -210,7 +212,9 @@ def serve(a):
             if len(done_res) % 32 == 0:
                 log("partial summary: " + json.dumps(serving.latency_summary(list(done_res))))
         else:
-            log(f"done {ndone[0]}: failed {f.exception()}")
+            e = f.exception()
+            kind = "timed out" if isinstance(e, serving.DeadlineExceeded) else "failed"
+            log(f"done {ndone[0]}: {kind} {e}")
 
     t0 = time.perf_counter()
     t_next = t0
@@ -226,18 +230,27 @@ def serve(a):
     if a.max_seconds:
         while time.perf_counter() - t0 < a.max_seconds and not all(f.done() for f in futs):
             time.sleep(1.0)
-        srv.drain()
-    res, failed, dropped = [], [], []
+        srv.drain(cancel_running=True)
+    res, failed, timed_out, dropped, responses = [], [], [], [], []
     for sp, f in zip(specs, futs):
         if f.cancelled():
             dropped.append(sp["rid"])
-        elif f.exception() is None:
+            continue
+        e = f.exception()
+        if e is None:
             res.append(f.result())
+            responses.append(res[-1])
+            continue
+        if hasattr(e, "latency_s"):
+            responses.append({"latency_s": e.latency_s, "t_arrive": e.t_arrive, "t_done": e.t_done})
+        if isinstance(e, serving.DeadlineExceeded):
+            timed_out.append(sp["rid"])
         else:  # e.g. the time integrator's 25 dt cuts: reported, not retried
-            failed.append({"rid": sp["rid"], "error": str(f.exception())})
+            failed.append({"rid": sp["rid"], "error": str(e)})
     dropped += [sp["rid"] for sp in specs[len(futs):]]
     srv.close()
     summ = serving.latency_summary(res)
+    lat_all = np.array([r["latency_s"] for r in responses])
     bad = [r["rid"] for r in res if r["stats"]["status"] != 0] + [f["rid"] for f in failed]
     cells = sum(r["nc"] for r in res)
     newton = sum(r["stats"]["newton_its"] for r in res)
@@ -250,8 +263,11 @@ def serve(a):
                                f"nz~U{{8..48}}, sigma~U(0.5,1.5), 5 backward-Euler steps of 1 h, FP64 residual, "
                                f"{'FP32' if a.serve_fp32 else 'FP64'} Krylov, {a.slots} device slots",
                    "total_cells": cells, "newton_its": newton, "failed": bad, "failures": failed[:20],
+                   "timed_out": timed_out, "deadline_s": a.deadline or None,
                    "not_served": len(dropped), "served": len(res), "max_seconds": a.max_seconds,
                    "pc_coarse": a.pc_coarse, "krylov_fp32": a.serve_fp32},
+        "responses": {"count": len(responses), "p50_latency_s": float(np.percentile(lat_all, 50)),
+                      "p99_latency_s": float(np.percentile(lat_all, 99))},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"], "serving": summ,
         "peak_inflight": srv.peak_inflight, "peak_admitted_bytes": srv.peak_used, "hbm_budget": srv.hbm_budget,
     }

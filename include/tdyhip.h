@@ -28,7 +28,8 @@ enum {
   TDYHIP_ERR_HIP = 2,      /* HIP runtime failure */
   TDYHIP_ERR_SINGULAR = 3, /* singular interaction-region system (PETSC_ERR_MAT_LU_ZRPVT) */
   TDYHIP_ERR_NOCONV = 4,   /* Newton / Krylov did not converge */
-  TDYHIP_ERR_MEM = 5       /* device memory exhausted */
+  TDYHIP_ERR_MEM = 5,      /* device memory exhausted */
+  TDYHIP_ERR_CANCELLED = 6 /* the solve was stopped by tdyhip_cancel */
 };
 
 /* boundary types per face (face_bc): -1 internal, 0 Dirichlet, 1 Neumann
@@ -165,6 +166,13 @@ int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, cons
  * failed Newton solve cuts dt by 4 and retries from the previous solution (at most 25 cuts). */
 int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max,
                      double final_time, const tdyhip_solver_opts *o, tdyhip_solver_stats *st);
+/* Cancellation (on = 1) of the solve running on ctx: tdyhip_newton_snes, run_snes_steps,
+ * run_ts_be and run_tdyti return TDYHIP_ERR_CANCELLED at their next Newton iteration or
+ * GMRES restart, with U part-way.  Callable from any thread while the solve runs; the
+ * request stays set until tdyhip_cancel(ctx, 0), so a solve started while it is set stops
+ * at its first check.  No counterpart in the reference (PETSc solves run to their own
+ * criteria): it is for servers that bound the time a request may hold the device. */
+int tdyhip_cancel(tdyhip_ctx *ctx, int32_t on);
 
 /* Device memory held by the context (setup arrays plus the solver workspace), in
  * bytes: what an HBM-footprint admission policy accounts per problem.  No

@@ -22,12 +22,20 @@ EXPORTS = [
     "tdyhip_snes_jacobian", "tdyhip_trans_nnz", "tdyhip_trans_csr", "tdyhip_gravdis",
     "tdyhip_solver_defaults", "tdyhip_newton_snes", "tdyhip_run_snes_steps", "tdyhip_run_ts_be",
     "tdyhip_run_tdyti", "tdyhip_set_pc_partition", "tdyhip_kernel_timers", "tdyhip_kernel_timer_report",
-    "tdyhip_device_bytes", "tdyhip_diagnostics", "tdyhip_get_stream",
+    "tdyhip_device_bytes", "tdyhip_diagnostics", "tdyhip_get_stream", "tdyhip_cancel",
 ]
 
 
+ERR_NOCONV = 4
+ERR_CANCELLED = 6
+
+
 class TdyHipError(RuntimeError):
-    pass
+    """A nonzero TDYHIP_ERR_* return; ``code`` holds it."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 class Mesh(C.Structure):
@@ -90,7 +98,7 @@ def load():
         "tdyhip_run_ts_be": [P, P, D, C.c_int32, P, P], "tdyhip_run_tdyti": [P, P, D, D, D, D, P, P],
         "tdyhip_set_pc_partition": [P, P], "tdyhip_kernel_timers": [P, C.c_int32],
         "tdyhip_kernel_timer_report": [P, C.c_char_p, C.c_int64], "tdyhip_device_bytes": [P, P],
-        "tdyhip_diagnostics": [P, P, P, P, P, P], "tdyhip_get_stream": [P, P],
+        "tdyhip_diagnostics": [P, P, P, P, P, P], "tdyhip_get_stream": [P, P], "tdyhip_cancel": [P, C.c_int32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -105,4 +113,4 @@ def load():
 def check(rc: int):
     if rc != 0:
         msg = load().tdyhip_last_error().decode(errors="replace")
-        raise TdyHipError(f"tdyhip error {rc}: {msg}")
+        raise TdyHipError(f"tdyhip error {rc}: {msg}", rc)

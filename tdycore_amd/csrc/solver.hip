@@ -606,6 +606,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     SCHK(hipStreamSynchronize(st));
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.converged || hg.its >= o.ksp_max_it) break;
+    if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
     // restart: w = M^-1 (b - A x), the residual in FP64
     spmv_blk64(ctx, ws, yvec, ws.JY, st);
     k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z);
@@ -696,6 +697,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
   if (!(fnorm == fnorm)) { stats.status = -4; return 0; }
   if (fnorm < o.snes_atol) { stats.status = 0; stats.last_fnorm = fnorm; return 0; }
   for (it = 1; it <= o.snes_max_it; it++) {
+    if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
     if ((rc = eval_J(ctx, fm, X, ws.J, st))) return rc;
     int lits = 0;
     rc = linsolve(ctx, o, ws.F, ws.Y, &lits, &stats.krylov_fallbacks, st);
@@ -886,6 +888,12 @@ int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, cons
     if (st->status) { set_error("Newton did not converge"); return TDYHIP_ERR_NOCONV; }
   }
   SCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int tdyhip_cancel(tdyhip_ctx *ctx, int32_t on) {
+  if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
+  ctx->cancel.store(on ? 1 : 0, std::memory_order_relaxed);
   return 0;
 }
 

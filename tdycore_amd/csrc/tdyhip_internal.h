@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -83,6 +84,7 @@ struct tdyhip_ctx {
   int64_t jnnz = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  std::atomic<int> cancel{0};  // set by tdyhip_cancel from any thread
 
   double vg_m, vg_alpha, x0, x1, x2, dx, gravity, pref;
   int density_type, gmatrix;

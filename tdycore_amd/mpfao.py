@@ -237,3 +237,10 @@ class MPFAO:
         self._run(_lib.load().tdyhip_run_tdyti, _ptr(U), float(dt_init), float(growth), float(dt_max),
                   float(final_time), C.byref(o), C.byref(st))
         return self._stats_dict(st)
+
+    def cancel(self, on: bool = True):
+        """Ask the solve running on this context (called from another thread)
+        to stop: it raises TdyHipError with code ERR_CANCELLED at its next
+        Newton iteration or GMRES restart.  Stays set until cancel(False)."""
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.check(_lib.load().tdyhip_cancel(self._h, 1 if on else 0))

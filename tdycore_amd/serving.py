@@ -14,7 +14,7 @@ stages:
    runs the backward-Euler steps to the request's final time with the
    time-integrator rules of ``TDyTimeIntegratorRunToTime`` (src/tdyti.c:71-196):
    a Newton failure cuts dt and retries.  Each step is a pre-solve plus a
-   Newton solve: FP64 residual, FP32-Krylov inner solve.  The request's state stays
+   Newton solve: FP64 residual, FP32 or FP64 Krylov inner solve.  The request's state stays
    in HBM from setup to its last step.  Each context drives the
    non-blocking HIP stream it created for itself, so the setup and Newton kernels of different
    requests overlap on the device, and a request that arrives while others
@@ -23,6 +23,13 @@ stages:
 Results are bit-for-bit those of the same request run alone: every kernel of
 the path (residual and Jacobian gathers, Krylov reductions) sums in a fixed
 order, whatever else runs on the device.
+
+A request may hold a slot for at most ``deadline_s`` seconds of solving: past
+that a watchdog cancels its context (the solver stops at its next Newton
+iteration or GMRES restart) and the request's future fails with
+``DeadlineExceeded``.  Every failed future's exception carries the request's
+``rid``, ``t_arrive``, ``t_done`` and ``latency_s``, so failures count in the
+latency of responses.
 """
 from __future__ import annotations
 
@@ -37,6 +44,32 @@ import numpy as np
 
 from . import workloads
 from .meshgen import brick_partition
+
+
+class DeadlineExceeded(TimeoutError):
+    """The request's solve ran past the server's deadline and was cancelled."""
+
+
+class SolveHandle:
+    """Given to a solve function, which attaches the call that stops its solve
+    (``MPFAO.cancel``) while the solve runs; the watchdog calls ``cancel``."""
+
+    def __init__(self):
+        self._lock = threading.Lock()
+        self._stop = None
+        self.cancelled = False
+
+    def attach(self, stop: Optional[Callable]):
+        with self._lock:
+            self._stop = stop
+            if stop is not None and self.cancelled:
+                stop()
+
+    def cancel(self):
+        with self._lock:
+            self.cancelled = True
+            if self._stop is not None:
+                self._stop()
 
 
 @dataclasses.dataclass
@@ -75,7 +108,7 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
 
 
 class _Job:
-    __slots__ = ("req", "future", "t_arrive", "t_built", "t_admit", "t_done", "problem", "need")
+    __slots__ = ("req", "future", "t_arrive", "t_built", "t_admit", "t_done", "problem", "need", "handle")
 
     def __init__(self, req, future):
         self.req = req
@@ -84,6 +117,7 @@ class _Job:
         self.t_built = self.t_admit = self.t_done = None
         self.problem = None
         self.need = 0
+        self.handle = SolveHandle()
 
 
 class EnsembleServer:
@@ -94,14 +128,19 @@ class EnsembleServer:
     hbm_budget   bytes of device memory the in-flight requests may hold;
                  default 80 % of the free HBM at start
     build_workers  host threads generating problems
-    solve_fn     (problem, request) -> (pressure ndarray, stats dict); the
-                 default runs the device Newton solver
+    deadline_s   longest a request may solve before it is cancelled (None:
+                 no limit)
+    solve_fn     (problem, request, handle) -> (pressure ndarray, stats
+                 dict); the default runs the device Newton solver.  It
+                 attaches its stop call to the SolveHandle while it runs.
     """
 
     def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, build_workers: int = 8,
                  krylov_fp32: bool = True, snes_rtol: float = 1e-10, brick=(8, 8, 4),
-                 solve_fn: Optional[Callable] = None, pc_coarse: bool = True):
+                 solve_fn: Optional[Callable] = None, pc_coarse: bool = True,
+                 deadline_s: Optional[float] = None):
         self.slots = slots
+        self.deadline_s = deadline_s
         self.pc_coarse = pc_coarse
         self.brick = tuple(brick)
         self.krylov_fp32 = krylov_fp32
@@ -121,10 +160,16 @@ class EnsembleServer:
         self.peak_inflight = 0
         self._closed = False
         self._draining = False
+        self._running = {}  # slot -> job on the device
         self._workers = [threading.Thread(target=self._slot, args=(i,), daemon=True, name=f"slot{i}")
                          for i in range(slots)]
         for w in self._workers:
             w.start()
+        self._watch_stop = threading.Event()
+        self._watch = None
+        if deadline_s is not None:
+            self._watch = threading.Thread(target=self._watchdog, daemon=True, name="deadline")
+            self._watch.start()
 
     # ---- client side ----
     def submit(self, req: Request) -> cf.Future:
@@ -137,10 +182,17 @@ class EnsembleServer:
         self._build.submit(self._build_job, job)
         return fut
 
-    def drain(self):
+    def drain(self, cancel_running: bool = False):
         """Stop serving requests that are not on the device yet: their futures
-        are cancelled; requests already solving finish."""
+        are cancelled.  Requests already solving finish, or with
+        cancel_running are cancelled too (their futures fail with
+        DeadlineExceeded)."""
         self._draining = True
+        if cancel_running:
+            with self._cv:
+                running = list(self._running.values())
+            for job in running:
+                job.handle.cancel()
         while True:
             try:
                 job = self._ready.get_nowait()
@@ -160,6 +212,18 @@ class EnsembleServer:
             self._ready.put(None)
         for w in self._workers:
             w.join()
+        self._watch_stop.set()
+        if self._watch is not None:
+            self._watch.join()
+
+    def _watchdog(self):
+        while not self._watch_stop.wait(0.05):
+            now = time.perf_counter()
+            with self._cv:
+                late = [j for j in self._running.values()
+                        if j.t_admit is not None and now - j.t_admit > self.deadline_s and not j.handle.cancelled]
+            for job in late:
+                job.handle.cancel()
 
     # ---- stages ----
     def _build_job(self, job: _Job):
@@ -205,8 +269,10 @@ class EnsembleServer:
                 job.future.cancel()
                 continue
             self._admit(job)
+            with self._cv:
+                self._running[wid] = job
             try:
-                U, st = self._solve(job.problem, job.req)
+                U, st = self._solve(job.problem, job.req, job.handle)
                 job.t_done = time.perf_counter()
                 job.future.set_result({
                     "rid": job.req.rid, "nc": job.problem["mesh"].nc, "U": U, "stats": st,
@@ -214,15 +280,25 @@ class EnsembleServer:
                     "queue_s": job.t_admit - job.t_built, "solve_s": job.t_done - job.t_admit,
                     "t_arrive": job.t_arrive, "t_done": job.t_done, "slot": wid})
             except BaseException as e:  # noqa: BLE001
+                job.t_done = time.perf_counter()
+                if job.handle.cancelled and not isinstance(e, DeadlineExceeded):
+                    e = DeadlineExceeded(f"request {job.req.rid} cancelled after "
+                                         f"{job.t_done - job.t_admit:.1f} s of solving ({e})")
+                e.rid, e.t_arrive, e.t_done = job.req.rid, job.t_arrive, job.t_done
+                e.latency_s = job.t_done - job.t_arrive
                 job.future.set_exception(e)
             finally:
+                with self._cv:
+                    self._running.pop(wid, None)
                 job.problem = None
                 self._release(job)
 
-    def _gpu_solve(self, P, req: Request):
+    def _gpu_solve(self, P, req: Request, handle: Optional[SolveHandle] = None):
         import torch
         from .mpfao import MPFAO
         g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        if handle is not None:
+            handle.attach(g.cancel)
         try:
             g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
             bs = self.brick[0] * self.brick[1] * self.brick[2]
@@ -240,6 +316,8 @@ class EnsembleServer:
             st["device_bytes"] = g.device_bytes()
             return out, st
         finally:
+            if handle is not None:
+                handle.attach(None)
             g.close()
 
 

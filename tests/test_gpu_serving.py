@@ -84,3 +84,70 @@ def test_many_small_requests_on_many_slots():
         assert np.array_equal(a["U"], mix[r.rid]["U"]), r
     srv.close()
     assert all(m["stats"]["status"] == 0 for m in mix.values())
+
+
+def test_cancel_stops_a_running_solve():
+    """A request cancelled from another thread stops within a GMRES cycle or a
+    Newton iteration and reports TDYHIP_ERR_CANCELLED."""
+    import threading
+    import time
+    from tdycore_amd import _lib
+    print("[case] cancel a 40x40x20, 200-step solve after 0.5 s", flush=True)
+    r = serving.Request(rid=99, nx=40, ny=40, nz=20, seed=4242, sigma=1.0, steps=200)
+    P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
+    srv = serving.EnsembleServer(slots=1, build_workers=1)
+    h, box = serving.SolveHandle(), {}
+
+    def run():
+        try:
+            box["res"] = srv._gpu_solve(P, r, h)
+        except _lib.TdyHipError as e:
+            box["err"] = e
+        box["t"] = time.perf_counter()
+
+    th = threading.Thread(target=run)
+    th.start()
+    time.sleep(0.5)
+    tc = time.perf_counter()
+    h.cancel()
+    th.join(120)
+    srv.close()
+    assert "err" in box and box["err"].code == _lib.ERR_CANCELLED
+    assert box["t"] - tc < 2.0
+
+
+def test_context_solves_again_after_a_cancel():
+    """A cancel request set before the solve stops it at once; cleared, the
+    same context solves to the bits of a fresh context."""
+    import torch
+    from tdycore_amd import _lib
+    from tdycore_amd.meshgen import brick_partition
+    from tdycore_amd.mpfao import MPFAO
+    print("[case] cancel before start, then solve on the same context", flush=True)
+    r = REQS[0]
+    P = workloads.ensemble_box_problem(r.nx, r.ny, r.nz, r.seed, r.sigma)
+
+    def ctx():
+        g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        g.set_pc_partition(brick_partition(P["mesh"], 8, 8, 4))
+        return g
+
+    def solve(g):
+        o = g.solver_options(snes_rtol=1e-10, krylov_fp32=0, pc_block=256, pc_coarse=1)
+        U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
+        st = g.run_tdyti(U, r.dt, 1.25, r.dt, r.steps * r.dt, o)
+        return U.cpu().numpy(), st
+
+    g = ctx()
+    g.cancel(True)
+    with pytest.raises(_lib.TdyHipError) as ei:
+        solve(g)
+    assert ei.value.code == _lib.ERR_CANCELLED
+    g.cancel(False)
+    U1, st1 = solve(g)
+    g.close()
+    g2 = ctx()
+    U2, st2 = solve(g2)
+    g2.close()
+    assert st1["status"] == 0 and st2["status"] == 0
+    assert np.array_equal(U1, U2)

@@ -31,7 +31,7 @@ def test_admission_respects_budget_and_slots():
     state = {"held": 0, "peak": 0, "running": 0, "peak_running": 0}
     sizes = {}
 
-    def fake_solve(P, req):
+    def fake_solve(P, req, handle):
         need = sizes[req.rid]
         with lock:
             state["held"] += need
@@ -66,7 +66,7 @@ def test_admission_respects_budget_and_slots():
 
 def test_request_above_budget_is_rejected():
     srv = serving.EnsembleServer(slots=1, hbm_budget=1 << 20, build_workers=1,
-                                 solve_fn=lambda P, r: (P["U0"], {}))
+                                 solve_fn=lambda P, r, h: (P["U0"], {}))
     f = srv.submit(serving.Request(rid=0, nx=16, ny=16, nz=8, seed=1, sigma=1.0))
     with pytest.raises(MemoryError):
         f.result(timeout=60)
@@ -74,7 +74,7 @@ def test_request_above_budget_is_rejected():
 
 
 def test_drain_cancels_requests_not_yet_on_the_device():
-    def slow(P, r):
+    def slow(P, r, h):
         time.sleep(0.2)
         return P["U0"], {"status": 0}
     srv = serving.EnsembleServer(slots=2, hbm_budget=1 << 40, build_workers=2, solve_fn=slow)
@@ -85,3 +85,49 @@ def test_drain_cancels_requests_not_yet_on_the_device():
     done = [f for f in futs if f.done() and not f.cancelled()]
     assert all(f.done() for f in futs)
     assert 0 < len(done) < 16 and all(f.result()["stats"]["status"] == 0 for f in done)
+
+
+def _cancellable(run_s):
+    """A stand-in solve that runs run_s seconds unless its handle stops it."""
+    def solve(P, r, handle):
+        stop = threading.Event()
+        handle.attach(stop.set)
+        try:
+            if stop.wait(run_s):
+                raise RuntimeError("tdyhip error 6: solve cancelled")
+            return P["U0"], {"status": 0}
+        finally:
+            handle.attach(None)
+    return solve
+
+
+def test_deadline_cancels_long_solves_only():
+    def solve(P, r, handle):
+        return _cancellable(5.0 if r.rid % 2 else 0.05)(P, r, handle)
+    srv = serving.EnsembleServer(slots=4, hbm_budget=1 << 40, build_workers=2, solve_fn=solve, deadline_s=0.3)
+    futs = [srv.submit(serving.Request(rid=i, nx=16, ny=16, nz=8, seed=i, sigma=1.0)) for i in range(6)]
+    t0 = time.perf_counter()
+    for i, f in enumerate(futs):
+        if i % 2:
+            with pytest.raises(serving.DeadlineExceeded) as ei:
+                f.result(timeout=30)
+            assert ei.value.rid == i and ei.value.latency_s >= 0.3
+        else:
+            assert f.result(timeout=30)["stats"]["status"] == 0
+    assert time.perf_counter() - t0 < 4.0  # nobody waited for a 5 s solve
+    srv.close()
+
+
+def test_drain_can_cancel_running_solves():
+    srv = serving.EnsembleServer(slots=2, hbm_budget=1 << 40, build_workers=2, solve_fn=_cancellable(30.0))
+    futs = [srv.submit(serving.Request(rid=i, nx=16, ny=16, nz=8, seed=i, sigma=1.0)) for i in range(5)]
+    time.sleep(0.5)
+    t0 = time.perf_counter()
+    srv.drain(cancel_running=True)
+    srv.close()
+    assert time.perf_counter() - t0 < 5.0
+    running = [f for f in futs if not f.cancelled()]
+    assert all(f.done() for f in futs) and len(running) == 2
+    for f in running:
+        with pytest.raises(serving.DeadlineExceeded):
+            f.result()
