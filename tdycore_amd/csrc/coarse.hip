@@ -29,7 +29,7 @@ struct CoarseLayout {
   int32_t *grp = nullptr;         // nb: group of each block
   int32_t *gmem = nullptr;        // ng*gsz: blocks of each group (-1 pad)
   double *A1 = nullptr;           // nb*kCQ: Galerkin rows (column = neighbour position)
-  double *G1 = nullptr;           // ng*gsz*gsz: inverse of each group's diagonal block
+  double *G1 = nullptr;           // ng*gsz*gsz: inverse of each group's diagonal block (column-major)
   double *A2 = nullptr;           // ng*ng: level-2 operator, inverted in place
   double *rc1 = nullptr;          // nb: restricted vector
   double *z1 = nullptr;           // nb: level-1 correction
@@ -67,22 +67,33 @@ __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, 
 
 // In-place Gauss-Jordan inverse of the n x n matrix M (row-major in LDS,
 // leading dimension ld) with row pivoting; all threads of the workgroup
-// call it.  piv and col are LDS scratch of n entries.  Returns false if a
-// pivot is zero.
+// call it (blockDim a multiple of 64, n <= 128).  piv and col are LDS
+// scratch of n entries.  Returns false if a pivot is zero.  The pivot of
+// column k is the first row i >= k of largest |M[i][k]|, found by the first
+// wavefront; the elimination runs one row per 64-lane group.
 __device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col, int *flag) {
   const int t = threadIdx.x, nt = blockDim.x;
   if (t == 0) *flag = 0;
   __syncthreads();
   for (int k = 0; k < n; k++) {
-    if (t == 0) {  // pivot search (n <= 128: one thread)
-      int p = k;
-      double mx = fabs(M[k * ld + k]);
-      for (int i = k + 1; i < n; i++) {
+    if (t < 64) {
+      double mx = -1.0;
+      int p = n;
+      for (int i = k + t; i < n; i += 64) {
         const double a = fabs(M[i * ld + k]);
         if (a > mx) { mx = a; p = i; }
       }
-      piv[k] = p;
-      if (!(mx > 0.0)) *flag = 1;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double om = __shfl_xor(mx, off, 64);
+        const int op = __shfl_xor(p, off, 64);
+        if (om > mx || (om == mx && op < p)) { mx = om; p = op; }
+      }
+      if (t == 0) {
+        piv[k] = p;
+        const double d = M[k * ld + k];
+        if (!(mx > 0.0) || d != d) *flag = 1;
+      }
     }
     __syncthreads();
     if (*flag) return false;
@@ -99,11 +110,13 @@ __device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col
     for (int j = t; j < n; j += nt) M[k * ld + j] = (j == k) ? inv : M[k * ld + j] * inv;
     for (int i = t; i < n; i += nt) col[i] = (i == k) ? 0.0 : M[i * ld + k];
     __syncthreads();
-    for (int e = t; e < n * n; e += nt) {
-      const int i = e / n, j = e % n;
+    for (int i = t >> 6; i < n; i += nt >> 6) {
       if (i == k) continue;
-      const double a = (j == k) ? 0.0 : M[i * ld + j];
-      M[i * ld + j] = a - col[i] * M[k * ld + j];
+      const double ci = col[i];
+      for (int j = t & 63; j < n; j += 64) {
+        const double a = (j == k) ? 0.0 : M[i * ld + j];
+        M[i * ld + j] = a - ci * M[k * ld + j];
+      }
     }
     __syncthreads();
   }
@@ -148,8 +161,9 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     if (t == 0) atomicOr(C.status, 1);
     return;
   }
+  // G1 column-major per group (entry (i, j) at j*G + i): level 1 reads it coalesced
   for (int e = t; e < G * G; e += blockDim.x) {
-    const int i = e / G, j = e % G;
+    const int j = e / G, i = e % G;
     C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * n + j] : 0.0;
   }
   // A2[g][h] = sum over members and their neighbour entries in group h,
@@ -189,15 +203,18 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
 // Level 1 of an application: z1 = G1 rc1 per group, r2 = group sums of rc1.
 __global__ void __launch_bounds__(kGMax) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
   __shared__ double sh[1];
+  __shared__ double rg[kGMax];
   if (stop && *stop) return;
   const int g = blockIdx.x, t = threadIdx.x, G = C.gsz;
   const int bt = t < G ? C.gmem[g * G + t] : -1;
   const double rt = bt >= 0 ? C.rc1[bt] : 0.0;
+  rg[t] = rt;
+  __syncthreads();
+  // padded members have zero rows and columns in G1 and zero rg
+  const double *g1 = C.G1 + (int64_t)g * G * G + t;
   double z = 0.0;
-  for (int j = 0; j < G; j++) {
-    const int bj = C.gmem[g * G + j];
-    if (bj >= 0 && bt >= 0) z += C.G1[(int64_t)g * G * G + t * G + j] * C.rc1[bj];
-  }
+#pragma unroll 8
+  for (int j = 0; j < G; j++) z += (t < G ? g1[j * G] : 0.0) * rg[j];
   if (bt >= 0) C.z1[bt] = z;
   const double s = block_sum_n<kGMax>(rt, sh);
   if (t == 0) C.r2[g] = s;

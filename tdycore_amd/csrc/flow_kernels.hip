@@ -127,8 +127,10 @@ struct RowArgs {
   const double *rho, *ukvr, *drho, *dukvr;
   const double *rho_bnd, *ukvr_bnd;
   double *flux;
-  double *Jrow;  // kMaxCols * nrows: d(flux row r)/d(p of column k) at Jrow[k*nrows + r]
+  double *Jrow;  // nrows * kMaxCols: d(flux row r)/d(p of column k) at Jrow[r*kMaxCols + k]
 };
+
+static_assert(kMaxCols == 8, "a Jrow row is two double4");
 
 template <bool WITH_R, bool WITH_J>
 __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
@@ -188,10 +190,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     double dukvr_up = 0.0, dukvr_dn = 0.0;
     if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
     else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
+    double jr[kMaxCols];
 #pragma unroll
     for (int k = 0; k < kMaxCols; k++) {
       const int col = ck[k];
       double Jac = 0.0;
+      jr[k] = 0.0;
       if (col < 0 || col >= a.nc) continue;  // padding or boundary-pressure column: never enters J
       if (zero) Jac = 0.0;
       else if (up >= 0 && col == up)
@@ -203,8 +207,12 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
       else
         Jac = den * ukvr * Tk[k];
       // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
-      a.Jrow[(int64_t)k * a.nrows + r] = -Jac;
+      jr[k] = -Jac;
     }
+    // one 64-byte row: the gather reads a face's four rows as 256 contiguous bytes
+    double4 *dst = reinterpret_cast<double4 *>(a.Jrow + r * kMaxCols);
+    dst[0] = make_double4(jr[0], jr[1], jr[2], jr[3]);
+    dst[1] = make_double4(jr[4], jr[5], jr[6], jr[7]);
   }
 }
 
@@ -262,11 +270,14 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
         const int li = a.row_ul[2 * r + (is_up ? 0 : 1)];
         const int32_t *cols = a.vcols + (int64_t)v * kMaxCols;
         const uint8_t *slot = a.jslot + (int64_t)v * 64 + li * 8;
+        const double4 *src = reinterpret_cast<const double4 *>(a.Jrow + r * kMaxCols);
+        const double4 j0 = src[0], j1 = src[1];
+        const double d8[kMaxCols] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
         for (int k = 0; k < kMaxCols; k++) {
           const int col = cols[k];
           if (col < 0 || col >= a.nc) continue;
-          const double d = a.Jrow[(int64_t)k * a.nrows + r];
+          const double d = d8[k];
           acc[slot[k]][t] += is_up ? d : -d;
         }
       }

@@ -320,6 +320,13 @@ __global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, in
 constexpr int kRedBlock = 256;
 constexpr int kRedPerThread = 2;
 constexpr int kChunk = kRedBlock * kRedPerThread;
+// Rows per thread of the multi-vector kernels: two 8-byte values per vector,
+// so FP32 handles four rows and keeps as many bytes in flight as FP64.
+template <class TV>
+struct Red {
+  static constexpr int E = kRedPerThread * (int)(8 / sizeof(TV));
+  static constexpr int chunk = kRedBlock * E;
+};
 
 template <class TV>
 __device__ inline TV block_sum(TV v, TV *sh) {
@@ -377,10 +384,11 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
   __shared__ TV red[kRedBlock / 64][NS];
   if (stop && *stop) return;
   const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
-  const int64_t i0 = (int64_t)blockIdx.x * kChunk + t;
-  TV wv[kRedPerThread];
+  constexpr int E = Red<TV>::E;
+  const int64_t i0 = (int64_t)blockIdx.x * Red<TV>::chunk + t;
+  TV wv[E];
 #pragma unroll
-  for (int e = 0; e < kRedPerThread; e++) {
+  for (int e = 0; e < E; e++) {
     const int64_t i = i0 + (int64_t)e * kRedBlock;
     wv[e] = i < n ? w[i] : (TV)0;
   }
@@ -392,7 +400,7 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
       if (j < NS && j0 + j < nv) {
         const TV *vj = V + (int64_t)(j0 + j) * ldv;
 #pragma unroll
-        for (int e = 0; e < kRedPerThread; e++) {
+        for (int e = 0; e < E; e++) {
           const int64_t i = i0 + (int64_t)e * kRedBlock;
           if (i < n) acc[j] += wv[e] * vj[i];
         }
@@ -432,10 +440,10 @@ __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, con
                                                           const int *__restrict__ stop = nullptr) {
   __shared__ TV sh[kRedBlock / 64];
   if (stop && *stop) return;
-  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * Red<TV>::chunk + threadIdx.x;
   TV s = 0;
 #pragma unroll
-  for (int e = 0; e < kRedPerThread; e++) {
+  for (int e = 0; e < Red<TV>::E; e++) {
     const int64_t i = i0 + (int64_t)e * kRedBlock;
     if (i < n) {
       TV x = w[i];

@@ -565,6 +565,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   // the cycle has converged), the Hessenberg solve and the update of x.
   // The unnormalised Arnoldi vector alternates between w and z: step j reads
   // it from one and leaves the next in the other.
+  const int nch = (int)((N + Red<TV>::chunk - 1) / Red<TV>::chunk);  // chunks of the multi-vector kernels
   auto cycle_body = [&]() {
     for (int j = 0; j < m; j++) {
       TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
@@ -573,12 +574,12 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       // kernel per reduction with a last-block hand-off is much slower here:
       // the device-scope fences it needs write back the per-XCD L2s.)
       const double bmd = (double)(j + 2) * N * sizeof(TV);
-      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
-      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
-      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, ws.nchunk, stop));
-      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, ws.nchunk, part, ws.h, stop);
-      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
-      k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(ws.nchunk, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
+      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
+      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
+      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
+      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, part, ws.h, stop);
+      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
+      k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
     k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);

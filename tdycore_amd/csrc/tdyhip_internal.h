@@ -117,7 +117,7 @@ struct tdyhip_ctx {
   double *d_dM = nullptr, *d_d2M = nullptr, *d_acc = nullptr;
   double *d_Pbnd = nullptr, *d_ukvr_bnd = nullptr, *d_rho_bnd = nullptr;
   double *d_flux = nullptr;         // 4nf
-  double *d_jrow = nullptr;         // kMaxCols*4nf flux-row derivatives (column-major like T)
+  double *d_jrow = nullptr;         // 4nf*kMaxCols flux-row derivatives (row-major, 64 B per row)
   double *d_accum_prev = nullptr;   // nc
 
   // host copies used by diagnostics
