@@ -124,8 +124,10 @@ static double bytes_gather(const tdyhip_ctx *ctx, bool R, bool J) {
   double b = (double)ctx->nc * 24 + (double)ctx->nf * 8;  // cell faces, face cells
   if (R) b += (double)ctx->nc * (8 + 8) + (double)ctx->nf * 32;
   if (J)
-    b += (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (4 + 1) + (double)ctx->nf +
-         (double)ctx->nv * (kMaxCols * 4 + 64) + (double)ctx->nc * (8 * 4 + 1) + (double)ctx->jnnz * 8;
+    // flux-row derivatives (one 64-byte row each), the gather table, the
+    // accumulation terms and the Jacobian row written once
+    b += (double)ctx->active_rows * kMaxCols * 8 + (double)ctx->nc * 24 * 8 + (double)ctx->nc * (8 * 4 + 1) +
+         (double)ctx->jnnz * 8;
   return b;
 }
 
@@ -193,8 +195,8 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
                               bool ell, const int32_t *perm, int64_t ell_stride) {
   GatherArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
-  a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells; a.face_verts = ctx->d_face_verts;
-  a.vcols = ctx->d_vcols; a.face_flags = ctx->d_face_flags; a.row_ul = ctx->d_row_ul; a.jslot = ctx->d_jslot;
+  a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells;
+  a.face_flags = ctx->d_face_flags; a.gtab = ctx->d_gtab;
   a.jdslot = ctx->d_jdslot; a.flux = ctx->d_flux; a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
   a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
   a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
@@ -226,9 +228,9 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
                        int64_t ell_stride = 0) {
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
-  if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
-  else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, 256), 256, 0, st>>>(a));
-  else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, 256), 256, 0, st>>>(a));
+  if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
+  else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
+  else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
   GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell, perm, ell_stride);
   const unsigned g = nblk(ctx->nc, kGatherBlock);
   if (R && J)
@@ -304,8 +306,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   }
   if ((rc = dupload(ctx, &ctx->d_bnd_cell, topo.bnd_cell.data(), nbnd))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_vcols, topo.vcols.data(), topo.vcols.size()))) return fail(rc);
-  if ((rc = dupload(ctx, &ctx->d_row_ul, topo.row_ul.data(), topo.row_ul.size()))) return fail(rc);
-  if ((rc = dupload(ctx, &ctx->d_jslot, topo.jslot.data(), topo.jslot.size()))) return fail(rc);
+  if ((rc = dalloc(ctx, &ctx->d_gtab, (size_t)24 * nc))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jrp, topo.jrp.data(), topo.jrp.size()))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jcols, topo.jcols.data(), topo.jcols.size()))) return fail(rc);
   {
@@ -339,9 +340,10 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   // setup temporaries
   int32_t *d_scf = nullptr; IRDesc *d_ir = nullptr; double *d_Gsub = nullptr, *d_K = nullptr, *d_cc = nullptr,
           *d_fc = nullptr, *d_fa = nullptr, *d_fn = nullptr; int *d_status = nullptr;
+  uint8_t *d_rul = nullptr, *d_jsl = nullptr;
   auto tmp_fail = [&](int code) {
     for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc,
-                    (void *)d_fa, (void *)d_fn, (void *)d_status})
+                    (void *)d_fa, (void *)d_fn, (void *)d_status, (void *)d_rul, (void *)d_jsl})
       if (p) (void)hipFree(p);
     return fail(code);
   };
@@ -362,6 +364,8 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   TMPALLOC(d_fa, (size_t)nf, m->face_area);
   TMPALLOC(d_fn, (size_t)nf * 3, m->face_normal);
   TMPALLOC(d_status, 1, (int *)nullptr);
+  TMPALLOC(d_rul, topo.row_ul.size(), topo.row_ul.data());
+  TMPALLOC(d_jsl, topo.jslot.size(), topo.jslot.data());
 #undef TMPALLOC
   hipStream_t st = ctx->stream;
   k_fill<<<nblk((int64_t)kMaxCols * nrows, 256), 256, 0, st>>>((int64_t)kMaxCols * nrows, ctx->d_T, 0.0);
@@ -371,6 +375,8 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   KT(ctx, st, "k_local_systems", (double)nv * sizeof(IRDesc) + (double)nc * 72 * 8 + (double)ctx->trans_nnz * 8,
      k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
                                                                                  d_status));
+  k_gather_table<<<nblk(nc, 256), 256, 0, st>>>(nc, ctx->d_cell_faces, ctx->d_face_cells, ctx->d_face_verts,
+                                                ctx->d_face_flags, d_rul, ctx->d_vcols, d_jsl, ctx->d_gtab);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
                                            d_fc, d_fa, d_fn, d_K, ctx->d_G);
   // boundary rho, mu frozen at (Pref, Tref) (AllocateMemoryForBoundaryValues, tdympfao.c:297-336)
@@ -386,7 +392,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   int status = 0;
   if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
   for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc, (void *)d_fa,
-                  (void *)d_fn, (void *)d_status})
+                  (void *)d_fn, (void *)d_status, (void *)d_rul, (void *)d_jsl})
     (void)hipFree(p);
   if (status) {
     set_error("singular interaction-region system (zero LU pivot)");

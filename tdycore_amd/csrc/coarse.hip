@@ -137,7 +137,8 @@ __device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col
 // One workgroup (kGMax threads) per group: invert the group's diagonal block
 // of A1 into G1 and form the group's row of A2.
 __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
-  __shared__ double M[kGMax * kGMax];
+  constexpr int LD = kGMax + 1;  // odd row pitch: column reads do not hit one LDS bank
+  __shared__ double M[kGMax * LD];
   __shared__ double col[kGMax];
   __shared__ int piv[kGMax];
   __shared__ int flag;
@@ -153,10 +154,10 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     double a = 0.0;
     for (int q = 0; q < kCQ; q++)
       if (C.nbr[bi * kCQ + q] == bj) a = C.A1[bi * kCQ + q];
-    M[i * n + j] = a;
+    M[i * LD + j] = a;
   }
   __syncthreads();
-  const bool ok = gj_invert(M, n, n, piv, col, &flag);
+  const bool ok = gj_invert(M, n, LD, piv, col, &flag);
   if (!ok) {
     if (t == 0) atomicOr(C.status, 1);
     return;
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
   // G1 column-major per group (entry (i, j) at j*G + i): level 1 reads it coalesced
   for (int e = t; e < G * G; e += blockDim.x) {
     const int j = e / G, i = e % G;
-    C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * n + j] : 0.0;
+    C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * LD + j] : 0.0;
   }
   // A2[g][h] = sum over members and their neighbour entries in group h,
   // from an LDS list of (group, value) in a fixed order
@@ -186,18 +187,19 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
 
 // Single workgroup: A2 <- A2^-1 (ng <= kNG, held in LDS).
 __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
-  __shared__ double M[kNG * kNG];
+  constexpr int LD = kNG + 1;  // odd row pitch: column reads do not hit one LDS bank
+  __shared__ double M[kNG * LD];
   __shared__ double col[kNG];
   __shared__ int piv[kNG];
   __shared__ int flag;
   const int n = C.ng;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[e] = C.A2[e];
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[(e / n) * LD + e % n] = C.A2[e];
   __syncthreads();
-  if (!gj_invert(M, n, n, piv, col, &flag)) {
+  if (!gj_invert(M, n, LD, piv, col, &flag)) {
     if (threadIdx.x == 0) atomicOr(C.status, 1);
     return;
   }
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) C.A2[e] = M[e];
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) C.A2[e] = M[(e / n) * LD + e % n];
 }
 
 // Level 1 of an application: z1 = G1 rc1 per group, r2 = group sums of rc1.

@@ -130,11 +130,14 @@ struct RowArgs {
   double *Jrow;  // nrows * kMaxCols: d(flux row r)/d(p of column k) at Jrow[r*kMaxCols + k]
 };
 
-static_assert(kMaxCols == 8, "a Jrow row is two double4");
+constexpr int kRowBlock = 256;  // k_rows workgroup size (the Jrow staging below assumes it)
 
+// One flux row: its flux (WITH_R) and its 8 column derivatives into jr
+// (WITH_J; zeros for padding, boundary-pressure and no-flow rows).
 template <bool WITH_R, bool WITH_J>
-__global__ void __launch_bounds__(256) k_rows(RowArgs a) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void row_eval(const RowArgs &a, int64_t r, double (&jr)[kMaxCols]) {
+#pragma unroll
+  for (int k = 0; k < kMaxCols; k++) jr[k] = 0.0;
   if (r >= a.nrows) return;
   const int f = (int)(r >> 2);
   const uint8_t fl = a.face_flags[f];
@@ -190,12 +193,10 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
     double dukvr_up = 0.0, dukvr_dn = 0.0;
     if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
     else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
-    double jr[kMaxCols];
 #pragma unroll
     for (int k = 0; k < kMaxCols; k++) {
       const int col = ck[k];
       double Jac = 0.0;
-      jr[k] = 0.0;
       if (col < 0 || col >= a.nc) continue;  // padding or boundary-pressure column: never enters J
       if (zero) Jac = 0.0;
       else if (up >= 0 && col == up)
@@ -209,10 +210,63 @@ __global__ void __launch_bounds__(256) k_rows(RowArgs a) {
       // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
       jr[k] = -Jac;
     }
-    // one 64-byte row: the gather reads a face's four rows as 256 contiguous bytes
-    double4 *dst = reinterpret_cast<double4 *>(a.Jrow + r * kMaxCols);
-    dst[0] = make_double4(jr[0], jr[1], jr[2], jr[3]);
-    dst[1] = make_double4(jr[4], jr[5], jr[6], jr[7]);
+  }
+}
+
+// Thread per flux row.  Jrow is row-major (64 B per row, so the per-cell
+// gather reads a face's four rows as 256 contiguous bytes); the workgroup's
+// rows are staged in LDS and written out as one contiguous, coalesced run.
+template <bool WITH_R, bool WITH_J>
+__global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
+  const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+  double jr[kMaxCols];
+  row_eval<WITH_R, WITH_J>(a, r, jr);
+  if (WITH_J) {
+    constexpr int P = kMaxCols + 1;  // odd pitch: conflict-free LDS rows
+    __shared__ double js[kRowBlock * P];
+#pragma unroll
+    for (int k = 0; k < kMaxCols; k++) js[threadIdx.x * P + k] = jr[k];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRowBlock * kMaxCols, lim = a.nrows * kMaxCols;
+#pragma unroll
+    for (int i = 0; i < kMaxCols; i++) {
+      const int e = threadIdx.x + kRowBlock * i;
+      if (base + e < lim) a.Jrow[base + e] = js[(e / kMaxCols) * P + e % kMaxCols];
+    }
+  }
+}
+
+// Per-cell gather table, built once at setup.  Row i = 4*lf + q of cell c
+// (its local face lf, sub-face q) has 8 bytes at gtab[i*nc + c]: byte k is
+// the slot of column k in c's Jacobian row, with bit 7 set when c is the
+// face's down cell (the row enters c's residual with a minus sign), or 0xff
+// when column k does not enter J (padding, a boundary pressure) or the face
+// carries no flow.  Lanes of consecutive cells read consecutive words.
+__global__ void k_gather_table(int nc, const int32_t *__restrict__ cell_faces, const int32_t *__restrict__ face_cells,
+                               const int32_t *__restrict__ face_verts, const uint8_t *__restrict__ face_flags,
+                               const uint8_t *__restrict__ row_ul, const int32_t *__restrict__ vcols,
+                               const uint8_t *__restrict__ jslot, uint64_t *__restrict__ gtab) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  for (int lf = 0; lf < 6; lf++) {
+    const int f = cell_faces[c * 6 + lf];
+    const bool noflow = (face_flags[f] & kFaceNoFlow) != 0;
+    const bool is_up = face_cells[2 * f] == c;
+    for (int q = 0; q < 4; q++) {
+      const int64_t r = (int64_t)f * 4 + q;
+      uint64_t w = ~0ull;
+      if (!noflow) {
+        const int v = face_verts[r];
+        const int li = row_ul[2 * r + (is_up ? 0 : 1)];
+        for (int k = 0; k < kMaxCols; k++) {
+          const int col = vcols[(int64_t)v * kMaxCols + k];
+          if (col < 0 || col >= nc) continue;
+          const uint64_t sb = (uint64_t)(jslot[(int64_t)v * 64 + li * 8 + k] | (is_up ? 0u : 0x80u));
+          w = (w & ~(0xffull << (8 * k))) | (sb << (8 * k));
+        }
+      }
+      gtab[(int64_t)(lf * 4 + q) * nc + c] = w;
+    }
   }
 }
 
@@ -221,8 +275,9 @@ struct GatherArgs {
   int64_t nrows;
   int form;  // 0 TS, 1 SNES
   double shift, dtime;
-  const int32_t *cell_faces, *face_cells, *face_verts, *vcols;
-  const uint8_t *face_flags, *row_ul, *jslot, *jdslot;
+  const int32_t *cell_faces, *face_cells;
+  const uint8_t *face_flags, *jdslot;
+  const uint64_t *gtab;  // 24*nc: see k_gather_table
   const double *flux, *acc, *dM, *d2M, *Udot, *vol, *Jrow;
   const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
   const int32_t *perm; // ELL: position of row c (nullptr: c)
@@ -262,23 +317,18 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
     for (int sl = 0; sl < kJacSlots; sl++) acc[sl][t] = 0.0;
     for (int lf = 0; lf < 6; lf++) {
       const int f = a.cell_faces[c * 6 + lf];
-      if (a.face_flags[f] & kFaceNoFlow) continue;
-      const bool is_up = a.face_cells[2 * f] == c;
       for (int q = 0; q < 4; q++) {
-        const int64_t r = (int64_t)f * 4 + q;
-        const int v = a.face_verts[r];
-        const int li = a.row_ul[2 * r + (is_up ? 0 : 1)];
-        const int32_t *cols = a.vcols + (int64_t)v * kMaxCols;
-        const uint8_t *slot = a.jslot + (int64_t)v * 64 + li * 8;
-        const double4 *src = reinterpret_cast<const double4 *>(a.Jrow + r * kMaxCols);
+        const uint64_t w = a.gtab[(int64_t)(lf * 4 + q) * a.nc + c];
+        if (w == ~0ull) continue;  // no-flow face
+        const double4 *src = reinterpret_cast<const double4 *>(a.Jrow + ((int64_t)f * 4 + q) * kMaxCols);
         const double4 j0 = src[0], j1 = src[1];
         const double d8[kMaxCols] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
 #pragma unroll
         for (int k = 0; k < kMaxCols; k++) {
-          const int col = cols[k];
-          if (col < 0 || col >= a.nc) continue;
+          const unsigned sb = (unsigned)(w >> (8 * k)) & 0xffu;
+          if (sb == 0xffu) continue;
           const double d = d8[k];
-          acc[slot[k]][t] += is_up ? d : -d;
+          acc[sb & 0x7fu][t] += (sb & 0x80u) ? -d : d;
         }
       }
     }

@@ -105,8 +105,7 @@ struct tdyhip_ctx {
   int32_t *d_vcols = nullptr;       // nv*kMaxCols
   double *d_T = nullptr;            // kMaxCols * 4nf (column-major ELL over face-major rows)
   double *d_G = nullptr;            // 4nf gravity term
-  uint8_t *d_row_ul = nullptr;      // 4nf*2
-  uint8_t *d_jslot = nullptr;       // nv*64
+  uint64_t *d_gtab = nullptr;       // 24*nc per-cell gather table (built at setup from row_ul, jslot)
   int32_t *d_jrp = nullptr, *d_jcols = nullptr;
   uint8_t *d_jdslot = nullptr;      // nc: slot of the diagonal within its row
   int jell_w = 0;                   // ELL width (longest Jacobian row)

@@ -98,8 +98,8 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
     workspace), from the mesh counts.  Hexahedral meshes: at most 27 Jacobian
     entries per row."""
     tv = 8  # Krylov buffers are sized for FP64 (the fallback of a failed FP32 solve)
-    ctx = 333 * nc + 609 * nf + 96 * nv + 36 * nbnd
-    setup_tmp = 768 * nc + 56 * nf + 200 * nv
+    ctx = 525 * nc + 601 * nf + 32 * nv + 36 * nbnd
+    setup_tmp = 768 * nc + 64 * nf + 264 * nv
     # Newton vectors, block-ordered Jacobian and its columns, Krylov basis,
     # block-Jacobi tables and factor (rows = the block layout's row count)
     ws = 64 * nc + 4 * nc + rows * 27 * (8 + 4) + (restart + 5) * nc * tv + rows * (548 + 27 * tv)
