@@ -32,8 +32,7 @@ struct CoarseLayout {
   double *G1 = nullptr;           // ng*gsz*gsz: inverse of each group's diagonal block (column-major)
   double *A2 = nullptr;           // ng*ng: level-2 operator, inverted in place
   double *rc1 = nullptr;          // nb: restricted vector
-  double *z1 = nullptr;           // nb: level-1 correction
-  double *r2 = nullptr;           // ng
+  double *z1 = nullptr;           // nb: coarse correction per block (levels 1 and 2)
   int *status = nullptr;          // nonzero: singular coarse block
 };
 
@@ -202,43 +201,58 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) C.A2[e] = M[(e / n) * LD + e % n];
 }
 
-// Level 1 of an application: z1 = G1 rc1 per group, r2 = group sums of rc1.
-__global__ void __launch_bounds__(kGMax) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
-  __shared__ double sh[1];
+// Level 1 and 2 of an application, one workgroup per group g:
+//   z1[b] = (G1 rc1)[b] + (A2^-1 r2)[g]  for the blocks b of g,
+// with r2[h] the sum of rc1 over the blocks of group h.  Every workgroup
+// forms all of r2 itself (ng*gsz reads, in a fixed order), so one kernel
+// finishes the coarse solve and the prolongation only adds z1[block].
+constexpr int kL1Threads = 4 * kNG;
+__global__ void __launch_bounds__(kL1Threads) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
   __shared__ double rg[kGMax];
+  __shared__ double r2s[kNG];
+  __shared__ double sh[kL1Threads / 64];
   if (stop && *stop) return;
-  const int g = blockIdx.x, t = threadIdx.x, G = C.gsz;
+  const int g = blockIdx.x, t = threadIdx.x, G = C.gsz, ng = C.ng;
+  // r2[h]: lanes 4h..4h+3 each sum a quarter of group h's members, then
+  // ((q0 + q1) + (q2 + q3)) on all four lanes
+  {
+    const int h = t >> 2, qt = t & 3, QG = (G + 3) / 4;
+    double s = 0.0;
+    if (h < ng) {
+      const int m1 = min(G, (qt + 1) * QG);
+      for (int m = qt * QG; m < m1; m++) {
+        const int bm = C.gmem[h * G + m];
+        if (bm >= 0) s += C.rc1[bm];
+      }
+    }
+    const double o1 = __shfl_xor(s, 1, 64);
+    const double p = (t & 1) ? o1 + s : s + o1;
+    const double o2 = __shfl_xor(p, 2, 64);
+    if (h < ng && qt == 0) r2s[h] = p + o2;
+  }
   const int bt = t < G ? C.gmem[g * G + t] : -1;
-  const double rt = bt >= 0 ? C.rc1[bt] : 0.0;
-  rg[t] = rt;
+  if (t < kGMax) rg[t] = bt >= 0 ? C.rc1[bt] : 0.0;
   __syncthreads();
-  // padded members have zero rows and columns in G1 and zero rg
-  const double *g1 = C.G1 + (int64_t)g * G * G + t;
-  double z = 0.0;
+  // z2 = row g of A2^-1 times r2
+  const double z2 = block_sum_n<kL1Threads>(t < ng ? C.A2[(int64_t)g * ng + t] * r2s[t] : 0.0, sh);
+  // G1 column-major per group; padded members have zero rows, columns and rg
+  if (t < G) {
+    const double *g1 = C.G1 + (int64_t)g * G * G + t;
+    double z = 0.0;
 #pragma unroll 8
-  for (int j = 0; j < G; j++) z += (t < G ? g1[j * G] : 0.0) * rg[j];
-  if (bt >= 0) C.z1[bt] = z;
-  const double s = block_sum_n<kGMax>(rt, sh);
-  if (t == 0) C.r2[g] = s;
+    for (int j = 0; j < G; j++) z += g1[j * G] * rg[j];
+    if (bt >= 0) C.z1[bt] = z + z2;
+  }
 }
 
-// Prolongation: out[row] += z1[b] + (A2^-1 r2)[group(b)] for the rows of block b.
+// Prolongation: out[row] += z1[b] for the rows of block b.
 template <class TV, int BS>
 __global__ void __launch_bounds__(BS) k_coarse_prolong(CoarseLayout C, const int32_t *__restrict__ rowof,
                                                        TV *__restrict__ out, const int *__restrict__ stop) {
-  __shared__ double corr;
   if (stop && *stop) return;
   const int b = blockIdx.x, t = threadIdx.x;
-  if (t < 64) {  // one wavefront: the level-2 row of this block's group
-    const int g = C.grp[b];
-    double s = 0.0;
-    for (int h = t; h < C.ng; h += 64) s += C.A2[(int64_t)g * C.ng + h] * C.r2[h];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (t == 0) corr = C.z1[b] + s;
-  }
-  __syncthreads();
   const int row = rowof[(int64_t)b * BS + t];
-  if (row >= 0) out[row] += (TV)corr;
+  if (row >= 0) out[row] += (TV)C.z1[b];
 }
 
 }  // namespace tdyhip

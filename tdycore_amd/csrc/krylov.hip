@@ -375,13 +375,22 @@ __device__ inline TV wave_reduce_scatter(TV (&v)[32]) {
   return r;
 }
 
+// Coarse prolongation applied on the fly: w[i] += z1[pos[i] >> bs_log2]
+// (pos = block-layout position of row i).  z1 == nullptr: none.
+struct Prolong {
+  const double *z1 = nullptr;
+  const int32_t *pos = nullptr;
+  int bs_log2 = 0;
+};
+
 // partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1.  Each
 // thread accumulates up to NS dots in registers; one reduce-scatter per
-// wavefront and one barrier per group of NS vectors finish the chunk.
+// wavefront and one barrier per group of NS vectors finish the chunk.  With
+// pro.z1 set, w first receives the coarse prolongation (written back).
 template <class TV, int NS>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
-                                                    const TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
-                                                    const int *__restrict__ stop = nullptr) {
+                                                    TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
+                                                    Prolong pro, const int *__restrict__ stop = nullptr) {
   __shared__ TV red[kRedBlock / 64][NS];
   if (stop && *stop) return;
   const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
@@ -392,6 +401,10 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
   for (int e = 0; e < E; e++) {
     const int64_t i = i0 + (int64_t)e * kRedBlock;
     wv[e] = i < n ? w[i] : (TV)0;
+    if (pro.z1 && i < n) {
+      wv[e] += (TV)pro.z1[pro.pos[i] >> pro.bs_log2];
+      w[i] = wv[e];
+    }
   }
   for (int j0 = 0; j0 < nv; j0 += NS) {
     TV acc[32];

@@ -414,7 +414,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
       if ((rc = ws_alloc(ws, &C.A1, (size_t)hc.nb * kCQ)) ||
           (rc = ws_alloc(ws, &C.G1, (size_t)hc.ng * hc.gsz * hc.gsz)) ||
           (rc = ws_alloc(ws, &C.A2, (size_t)hc.ng * hc.ng)) || (rc = ws_alloc(ws, &C.rc1, (size_t)hc.nb)) ||
-          (rc = ws_alloc(ws, &C.z1, (size_t)hc.nb)) || (rc = ws_alloc(ws, &C.r2, (size_t)hc.ng)))
+          (rc = ws_alloc(ws, &C.z1, (size_t)hc.nb)))
         return rc;
       C.status = ws.status;
       ws.coarse = true;
@@ -508,12 +508,17 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     KT(ctx, st, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C));
     KT(ctx, st, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, st>>>(C));
   }
+  // level-1/2 solve of the coarse correction: z1 per block
+  auto coarse_solve = [&](const int *stp) {
+    const CoarseLayout &C = ws.cl;
+    KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24 + (double)C.ng * C.ng * 8,
+       k_coarse_level1<<<C.ng, kL1Threads, 0, st>>>(C, stp));
+  };
   auto coarse_add = [&](TV *out, const int *stp) {
     if (!ws.coarse) return;
     const CoarseLayout &C = ws.cl;
-    KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24,
-       k_coarse_level1<<<C.ng, kGMax, 0, st>>>(C, stp));
-    const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * C.ng * 8;
+    coarse_solve(stp);
+    const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * 8;
     if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp));
     else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp));
     else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
@@ -540,7 +545,8 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int32_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
       else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int32_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
     }
-    coarse_add(out, stp);
+    // the prolongation of the coarse correction is added by the multi-dot that follows
+    if (ws.coarse) coarse_solve(stp);
   };
   auto norm2_to_h0 = [&](const TV *v) {
     k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
@@ -566,6 +572,12 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   // The unnormalised Arnoldi vector alternates between w and z: step j reads
   // it from one and leaves the next in the other.
   const int nch = (int)((N + Red<TV>::chunk - 1) / Red<TV>::chunk);  // chunks of the multi-vector kernels
+  Prolong pro;  // coarse prolongation folded into the multi-dot
+  if (ws.coarse) {
+    pro.z1 = ws.cl.z1;
+    pro.pos = ws.d_perm;
+    pro.bs_log2 = ws.bj.bs == 64 ? 6 : ws.bj.bs == 256 ? 8 : 9;
+  }
   auto cycle_body = [&]() {
     for (int j = 0; j < m; j++) {
       TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
@@ -573,10 +585,10 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
       // kernel per reduction with a last-block hand-off is much slower here:
       // the device-scope fences it needs write back the per-XCD L2s.)
-      const double bmd = (double)(j + 2) * N * sizeof(TV);
-      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
-      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
-      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, stop));
+      const double bmd = (double)(j + 2) * N * sizeof(TV) + (pro.z1 ? (double)N * (4 + sizeof(TV)) : 0.0);
+      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
+      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
+      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, part, ws.h, stop);
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
       k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
