@@ -289,24 +289,87 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32
   y[row] = blk_row<TV, TI>(NB, W, p, row, ecol, a, x);
 }
 
+// Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
+// previous Givens rotations, the new rotation, residual estimate and stop test.
+__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h,
+                                      double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
+                                      double *__restrict__ g) {
+  const int ld = m + 1;
+  const double hn = sqrt(h[j + 1]);
+  for (int i = 0; i <= j; i++) H[i + j * ld] = h[i];
+  H[j + 1 + j * ld] = hn;
+  for (int i = 0; i < j; i++) {
+    const double a = H[i + j * ld], c = H[i + 1 + j * ld];
+    H[i + j * ld] = cs[i] * a + sn[i] * c;
+    H[i + 1 + j * ld] = -sn[i] * a + cs[i] * c;
+  }
+  const double a = H[j + j * ld], c = H[j + 1 + j * ld];
+  const double d = sqrt(a * a + c * c);
+  cs[j] = d > 0 ? a / d : 1.0;
+  sn[j] = d > 0 ? c / d : 0.0;
+  H[j + j * ld] = d;
+  H[j + 1 + j * ld] = 0.0;
+  g[j + 1] = -sn[j] * g[j];
+  g[j] = cs[j] * g[j];
+  gd->its++;
+  gd->jcur = j + 1;
+  gd->resid = fabs(g[j + 1]);
+  gd->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
+  const int conv = gd->resid <= gd->tol || hn == 0.0;
+  gd->converged = conv;
+  if (conv || gd->its >= gd->maxit || j + 1 == m) gd->done = 1;
+}
+
 // ---------------------------------------------------------------- fused Arnoldi matvec
+// The previous Arnoldi column, finished inside the next matvec (NORM mode):
+// ||u||^2 is the sum of the nchunk norm partials, reduced by every workgroup
+// in the same fixed order; workgroup 0 also stores it and runs the Givens
+// update of column j.
+struct NormArgs {
+  const void *part = nullptr;
+  int nchunk = 0, j = 0, m = 0;
+  double *h = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr;
+};
+
 // One GMRES step's  V_j = s u  and  out = M^-1 A V_j = s M^-1 A u  (s = 1/||u||
-// read on the device): the block's rows of A u go straight into LDS and the
+// read on the device, or with NORM formed here from the norm partials of the
+// previous step): the block's rows of A u go straight into LDS and the
 // triangular sweeps follow, so neither A u nor a separate scaling pass
 // touches HBM; while some workgroups sweep their levels, others stream their
 // rows of A.  u and out must be different vectors.
-template <class TV, int BS, class TI>
-__global__ void __launch_bounds__(BS) k_bj_spmv_apply(BJLayout L, int64_t NB, int W, const TI *__restrict__ ecol,
+template <class TV, int BS, class TI, bool NORM>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_bj_spmv_apply(BJLayout L, int64_t NB, int W, const TI *__restrict__ ecol,
                                                       const TV *__restrict__ a, const TV *__restrict__ u,
-                                                      const GmresDev *__restrict__ gd, TV *__restrict__ vj,
+                                                      GmresDev *__restrict__ gd, TV *__restrict__ vj,
                                                       const TV *__restrict__ fac, TV *__restrict__ out,
-                                                      const int *__restrict__ stop, double *__restrict__ rc1) {
+                                                      const int *__restrict__ stop, double *__restrict__ rc1,
+                                                      NormArgs na) {
   __shared__ TV x[BS + 1];
-  if (stop && *stop) return;  // uniform over the workgroup
+  __shared__ TV shn[BS / 64];
+  __shared__ int stop_s;
   const int b = blockIdx.x, t = threadIdx.x;
+  TV sc;
+  if (NORM) {
+    // workgroup 0 of this same launch may set the stop flag: read it once
+    if (t == 0) stop_s = stop ? *stop : 0;
+    __syncthreads();
+    if (stop_s) return;  // uniform over the workgroup
+    const TV *part = static_cast<const TV *>(na.part);
+    TV s = 0;
+    for (int c = t; c < na.nchunk; c += BS) s += part[c];
+    s = block_sum_n<BS>(s, shn);
+    const double hn = sqrt((double)s);
+    sc = (TV)(hn > 0.0 ? 1.0 / hn : 0.0);
+    if (b == 0 && t == 0) {
+      na.h[na.j + 1] = (double)s;
+      arnoldi_update(gd, na.j, na.m, na.h, na.H, na.cs, na.sn, na.g);
+    }
+  } else {
+    if (stop && *stop) return;  // uniform over the workgroup
+    sc = (TV)gd->inv_h;
+  }
   const int64_t p = (int64_t)b * BS + t;
   const int row = L.d_row[p];
-  const TV sc = (TV)gd->inv_h;
   // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
   x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, a, u) : (TV)0;
   if (row >= 0) vj[row] = sc * u[row];
@@ -558,37 +621,6 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
   const int conv = !(beta > gd->tol);
   gd->converged = conv;
   gd->done = conv || gd->its >= gd->maxit;
-}
-
-// Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
-// previous Givens rotations, the new rotation, residual estimate and stop test.
-__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h,
-                                      double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
-                                      double *__restrict__ g) {
-  const int ld = m + 1;
-  const double hn = sqrt(h[j + 1]);
-  for (int i = 0; i <= j; i++) H[i + j * ld] = h[i];
-  H[j + 1 + j * ld] = hn;
-  for (int i = 0; i < j; i++) {
-    const double a = H[i + j * ld], c = H[i + 1 + j * ld];
-    H[i + j * ld] = cs[i] * a + sn[i] * c;
-    H[i + 1 + j * ld] = -sn[i] * a + cs[i] * c;
-  }
-  const double a = H[j + j * ld], c = H[j + 1 + j * ld];
-  const double d = sqrt(a * a + c * c);
-  cs[j] = d > 0 ? a / d : 1.0;
-  sn[j] = d > 0 ? c / d : 0.0;
-  H[j + j * ld] = d;
-  H[j + 1 + j * ld] = 0.0;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  gd->its++;
-  gd->jcur = j + 1;
-  gd->resid = fabs(g[j + 1]);
-  gd->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
-  const int conv = gd->resid <= gd->tol || hn == 0.0;
-  gd->converged = conv;
-  if (conv || gd->its >= gd->maxit || j + 1 == m) gd->done = 1;
 }
 
 __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,

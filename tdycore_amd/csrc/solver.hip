@@ -529,22 +529,31 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
     coarse_add(out, stp);
   };
-  // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused into the ILU sweeps)
-  auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp) {
+  // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused
+  // into the ILU sweeps); with na, ||u|| comes from the previous step's norm
+  // partials and the launch also finishes that step's Arnoldi column
+  auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp, const NormArgs *na) {
     const int W = ctx->jell_w;
     const double by = bytes_spmv(ctx, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
     const unsigned nb = ws.bj.nblocks;
+    const NormArgs nv = na ? *na : NormArgs{};
+#define TDY_SPMV(TI, EC, NRM)                                                                                      \
+  do {                                                                                                             \
+    if (ws.bj.bs == 64)                                                                                            \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, TI, NRM><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+    else if (ws.bj.bs == 256)                                                                                      \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, TI, NRM><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+    else                                                                                                           \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, TI, NRM><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+  } while (0)
     if (ws.d_ecol16) {
-      const int16_t *ec = ws.d_ecol16;
-      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int16_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
-      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int16_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
-      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int16_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      if (na) TDY_SPMV(int16_t, ws.d_ecol16, true);
+      else TDY_SPMV(int16_t, ws.d_ecol16, false);
     } else {
-      const int32_t *ec = ws.d_ecol;
-      if (ws.bj.bs == 64) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, int32_t><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
-      else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, int32_t><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
-      else KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, int32_t><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, ec, Jk, u, gd, vj, fac, out, stp, rc1));
+      if (na) TDY_SPMV(int32_t, ws.d_ecol, true);
+      else TDY_SPMV(int32_t, ws.d_ecol, false);
     }
+#undef TDY_SPMV
     // the prolongation of the coarse correction is added by the multi-dot that follows
     if (ws.coarse) coarse_solve(stp);
   };
@@ -572,6 +581,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   // The unnormalised Arnoldi vector alternates between w and z: step j reads
   // it from one and leaves the next in the other.
   const int nch = (int)((N + Red<TV>::chunk - 1) / Red<TV>::chunk);  // chunks of the multi-vector kernels
+  constexpr int kFuseNormChunks = 512;
   Prolong pro;  // coarse prolongation folded into the multi-dot
   if (ws.coarse) {
     pro.z1 = ws.cl.z1;
@@ -581,7 +591,13 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   auto cycle_body = [&]() {
     for (int j = 0; j < m; j++) {
       TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
-      spmv_apply(u, V + (int64_t)j * N, o, stop);
+      // On meshes of up to 512 chunks, step j > 0 finishes column j-1 (its
+      // norm and Givens update) inside the matvec: every workgroup re-reads
+      // the norm partials, which costs less than a launch while they are few.
+      // (At 1M rows, 2048 partials per workgroup cost as much as they save.)
+      const NormArgs na{part, nch, j - 1, m, ws.h, dH, dcs, dsn, dg};
+      const bool fuse_norm = nch <= kFuseNormChunks;
+      spmv_apply(u, V + (int64_t)j * N, o, stop, (fuse_norm && j > 0) ? &na : nullptr);
       // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
       // kernel per reduction with a last-block hand-off is much slower here:
       // the device-scope fences it needs write back the per-XCD L2s.)
@@ -591,7 +607,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
       k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, part, ws.h, stop);
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
-      k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
+      if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
     }
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
     k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
