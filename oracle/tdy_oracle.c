@@ -291,6 +291,14 @@ static int face_has_vertex(const ora_ctx *ctx, int f, int v) {
   for (int q = 0; q < 4; q++) if (ctx->face_verts[f * 4 + q] == v) return q;
   return -1;
 }
+/* Vertices of face f: 4 for a quadrilateral, 3 for a triangle (face_verts
+ * padded with -1).  A sub-face is the face area over this count
+ * (tdymeshplex.c:2937-2943). */
+static int face_nverts(const ora_ctx *ctx, int f) {
+  int n = 0;
+  for (int q = 0; q < 4; q++) n += ctx->face_verts[f * 4 + q] >= 0;
+  return n;
+}
 static int is_bnd_face(const ora_ctx *ctx, int f) { return ctx->face_cells[2 * f] < 0 || ctx->face_cells[2 * f + 1] < 0; }
 static int face_bc_of(const ora_ctx *ctx, int f) { return ctx->face_bc[f]; }
 static int is_dirichlet_like(int bc) { return bc == 0 || bc == 2; }
@@ -310,9 +318,11 @@ static int build_subcells(ora_ctx *ctx) {
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) Kc[i][j] = ctx->K0[c * 9 + i * 3 + j];
     for (int s = 0; s < 8; s++) {
       int v = ctx->cell_verts[c * 8 + s];
+      if (v < 0) continue; /* prism: 6 of 8 vertex slots */
       int fid[3], nfound = 0;
       for (int lf = 0; lf < 6; lf++) {
         int f = ctx->cell_faces[c * 6 + lf];
+        if (f < 0) continue; /* prism: 5 of 6 face slots */
         if (face_has_vertex(ctx, f, v) >= 0) { if (nfound < 3) fid[nfound] = f; nfound++; }
       }
       if (nfound != 3) { set_err("subcell does not have 3 faces"); return 1; }
@@ -348,7 +358,7 @@ static int build_subcells(ora_ctx *ctx) {
       double *G = ctx->G + (size_t)(c * 8 + s) * 9;
       for (int i = 0; i < 3; i++) {
         int f = fid[i];
-        int nverts = 4;
+        int nverts = face_nverts(ctx, f);
         double area = ctx->farea[f] / nverts;
         const double *n = ctx->fnorm + 3 * f;
         for (int j = 0; j < 3; j++) {
@@ -403,7 +413,9 @@ static int build_trans(ora_ctx *ctx) {
   int nc = ctx->nc, nf = ctx->nf, nv = ctx->nv;
   /* vertex -> cells */
   int *vc_cnt = calloc(nv + 1, sizeof(int));
-  for (int c = 0; c < nc; c++) for (int s = 0; s < 8; s++) vc_cnt[ctx->cell_verts[c * 8 + s] + 1]++;
+  for (int c = 0; c < nc; c++)
+    for (int s = 0; s < 8; s++)
+      if (ctx->cell_verts[c * 8 + s] >= 0) vc_cnt[ctx->cell_verts[c * 8 + s] + 1]++;
   for (int v = 0; v < nv; v++) vc_cnt[v + 1] += vc_cnt[v];
   int ncell_tot = vc_cnt[nv];
   ctx->ir_cells = malloc(ncell_tot * sizeof(int));
@@ -412,12 +424,15 @@ static int build_trans(ora_ctx *ctx) {
   for (int c = 0; c < nc; c++)
     for (int s = 0; s < 8; s++) {
       int v = ctx->cell_verts[c * 8 + s];
+      if (v < 0) continue;
       int k = vc_cnt[v] + fill[v]++;
       ctx->ir_cells[k] = c; ctx->ir_subc[k] = s;
     }
   /* vertex -> faces (internal first, then boundary, each by face id) */
   int *vf_cnt = calloc(nv + 1, sizeof(int));
-  for (int f = 0; f < nf; f++) for (int q = 0; q < 4; q++) vf_cnt[ctx->face_verts[f * 4 + q] + 1]++;
+  for (int f = 0; f < nf; f++)
+    for (int q = 0; q < 4; q++)
+      if (ctx->face_verts[f * 4 + q] >= 0) vf_cnt[ctx->face_verts[f * 4 + q] + 1]++;
   for (int v = 0; v < nv; v++) vf_cnt[v + 1] += vf_cnt[v];
   int nface_tot = vf_cnt[nv];
   ctx->ir_faces = malloc(nface_tot * sizeof(int));
@@ -428,6 +443,7 @@ static int build_trans(ora_ctx *ctx) {
       if (is_bnd_face(ctx, f) != pass) continue;
       for (int q = 0; q < 4; q++) {
         int v = ctx->face_verts[f * 4 + q];
+        if (v < 0) continue; /* triangle: 3 of 4 vertex slots */
         int k = vf_cnt[v] + fill[v]++;
         ctx->ir_faces[k] = f; ctx->ir_subface[k] = q;
       }
@@ -617,7 +633,7 @@ static void build_gravdis(ora_ctx *ctx) {
       int f = ctx->ir_faces[r->off_face + k], q = ctx->ir_subface[r->off_face + k];
       int up = ctx->face_cells[2 * f], dn = ctx->face_cells[2 * f + 1];
       if ((up < 0 || dn < 0) && ctx->face_bc[f] == 1) continue; /* no-flow face */
-      double area = ctx->farea[f] / 4;
+      double area = ctx->farea[f] / face_nverts(ctx, f);
       const double *fc = ctx->fcen + 3 * f;
       const double *n = ctx->fnorm + 3 * f;
       double cu[3], cd[3], u[3];
@@ -665,11 +681,14 @@ static void build_jac_pattern(ora_ctx *ctx) {
   int *mark = malloc(nc * sizeof(int));
   for (int i = 0; i < nc; i++) mark[i] = -1;
   int *rp = calloc(nc + 1, sizeof(int));
-  int *tmp = malloc(64 * 8 * sizeof(int));
+  int maxc = 0;
+  for (int v = 0; v < nv; v++) if (ctx->ir[v].ncell > maxc) maxc = ctx->ir[v].ncell;
+  int *tmp = malloc((size_t)(maxc > 0 ? maxc : 1) * 8 * sizeof(int));
   for (int pass = 0; pass < 2; pass++) {
     for (int c = 0; c < nc; c++) {
       int n = 0;
       for (int s = 0; s < 8; s++) {
+        if (ctx->cell_verts[c * 8 + s] < 0) continue;
         vtx_ir *r = &ctx->ir[ctx->cell_verts[c * 8 + s]];
         for (int i = 0; i < r->ncell; i++) {
           int cc = ctx->ir_cells[r->off_cell + i];

@@ -327,3 +327,155 @@ def brick_partition(mesh: Mesh, bx: int, by: int, bz: int) -> np.ndarray:
     i, j, k = c % mesh.nx, (c // mesh.nx) % mesh.ny, c // (mesh.nx * mesh.ny)
     gx, gy = -(-mesh.nx // bx), -(-mesh.ny // by)
     return ((i // bx) + gx * ((j // by) + gy * (k // bz))).astype(np.int32)
+
+
+# --------------------------------------------------------------------------
+# Right-prism meshes (a 2-D triangulation extruded through layers)
+# --------------------------------------------------------------------------
+
+def prism_mesh(points: np.ndarray, triangles: np.ndarray, z_levels: np.ndarray) -> Mesh:
+    """Extrude a triangulation through the levels ``z_levels`` (nl+1, increasing).
+
+    The arrays keep the hexahedral widths, padded with -1: a prism lists 6 of
+    8 vertex slots (bottom triangle a, b, c then top a, b, c), 5 of 6 face
+    slots (bottom, top, then the three side quads), and a triangular face 3
+    of 4 vertex slots.  Flux rows stay ``face*4 + local vertex`` as in the
+    reference (src/fv/mpfao/tdympfao.c:1451-1455), so a triangle leaves row
+    4f+3 unused.
+
+    * cell ``c = t + nt*k`` (triangle t, layer k), vertex ``v = p + np*k``;
+    * horizontal faces first (``t + nt*k`` at level k, normal +z from the
+      cell below to the cell above), then the side quads (``nh + e + ne*k``
+      for 2-D edge e at layer k);
+    * boundary faces store ``(-(b+1), cell)`` with the normal into the domain;
+      ``face_dir`` is 2 for the bottom/top (``face_side`` 0/1) and 0 with
+      ``face_side`` 0 for the lateral boundary (``face_bc(neumann=["x-"])``);
+    * geometry is exact: planar faces, cell centroid at the triangle centroid
+      and mid-layer height, volume = triangle area x thickness.
+    """
+    P = np.asarray(points, dtype=np.float64)
+    T = np.asarray(triangles, dtype=np.int64).copy()
+    Z = np.asarray(z_levels, dtype=np.float64)
+    npt, nt, nl = P.shape[0], T.shape[0], Z.shape[0] - 1
+    # counter-clockwise triangles
+    a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
+    sarea = 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    cw = sarea < 0
+    T[cw, 1], T[cw, 2] = T[cw, 2].copy(), T[cw, 1].copy()
+    tarea = np.abs(sarea)
+    # unique 2-D edges and their one or two triangles
+    E = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    Et = np.concatenate([np.arange(nt)] * 3)
+    Es = np.sort(E, axis=1)
+    key = Es[:, 0] * npt + Es[:, 1]
+    uk, inv = np.unique(key, return_inverse=True)
+    ne = uk.shape[0]
+    edges = np.stack([uk // npt, uk % npt], axis=1)
+    et = np.full((ne, 2), -1, dtype=np.int64)
+    order = np.argsort(inv, kind="stable")
+    inv_s, t_s = inv[order], Et[order]
+    first = np.ones(inv_s.shape[0], dtype=bool)
+    first[1:] = inv_s[1:] != inv_s[:-1]
+    et[inv_s[first], 0] = t_s[first]
+    et[inv_s[~first], 1] = t_s[~first]
+    tri_edge = inv.reshape(3, nt).T                      # (nt,3): edges (ab, bc, ca)
+
+    nv, nc = npt * (nl + 1), nt * nl
+    V = np.zeros((nv, 3))
+    V[:, 0] = np.tile(P[:, 0], nl + 1)
+    V[:, 1] = np.tile(P[:, 1], nl + 1)
+    V[:, 2] = np.repeat(Z, npt)
+    kk = np.repeat(np.arange(nl), nt)
+    tt = np.tile(np.arange(nt), nl)
+    cell_verts = np.full((nc, 8), -1, dtype=np.int32)
+    for s in range(3):
+        cell_verts[:, s] = T[tt, s] + npt * kk
+        cell_verts[:, 3 + s] = T[tt, s] + npt * (kk + 1)
+
+    nh = nt * (nl + 1)
+    nf = nh + ne * nl
+    face_verts = np.full((nf, 4), -1, dtype=np.int32)
+    face_cells = np.empty((nf, 2), dtype=np.int32)
+    face_dir = np.zeros(nf, dtype=np.int32)
+    face_side = np.full(nf, -1, dtype=np.int32)
+    # horizontal faces
+    hk = np.repeat(np.arange(nl + 1), nt)
+    ht = np.tile(np.arange(nt), nl + 1)
+    for s in range(3):
+        face_verts[:nh, s] = T[ht, s] + npt * hk
+    face_cells[:nh, 0] = np.where(hk > 0, ht + nt * (hk - 1), -1)
+    face_cells[:nh, 1] = np.where(hk < nl, ht + nt * hk, -1)
+    face_dir[:nh] = 2
+    face_side[:nh] = np.where(hk == 0, 0, np.where(hk == nl, 1, -1))
+    # side quads
+    vk = np.repeat(np.arange(nl), ne)
+    ve = np.tile(np.arange(ne), nl)
+    p0, p1 = edges[ve, 0], edges[ve, 1]
+    face_verts[nh:] = np.stack([p0 + npt * vk, p1 + npt * vk, p1 + npt * (vk + 1), p0 + npt * (vk + 1)], 1)
+    t0, t1 = et[ve, 0], et[ve, 1]
+    face_cells[nh:, 0] = t0 + nt * vk
+    face_cells[nh:, 1] = np.where(t1 >= 0, t1 + nt * vk, -1)
+    face_dir[nh:] = 0
+    face_side[nh:] = np.where(t1 >= 0, -1, 0)
+
+    bmask = (face_cells[:, 0] < 0) | (face_cells[:, 1] < 0)
+    bnd_face = np.nonzero(bmask)[0].astype(np.int32)
+    nb = bnd_face.shape[0]
+    cell_of_b = np.where(face_cells[bnd_face, 0] >= 0, face_cells[bnd_face, 0], face_cells[bnd_face, 1])
+    face_cells[bnd_face, 0] = -(np.arange(nb, dtype=np.int32) + 1)
+    face_cells[bnd_face, 1] = cell_of_b
+
+    cell_faces = np.full((nc, 6), -1, dtype=np.int32)
+    cell_faces[:, 0] = tt + nt * kk
+    cell_faces[:, 1] = tt + nt * (kk + 1)
+    for s in range(3):
+        cell_faces[:, 2 + s] = nh + tri_edge[tt, s] + ne * kk
+
+    # geometry
+    tc = (a + b + c) / 3.0
+    dz = Z[1:] - Z[:-1]
+    cell_cen = np.zeros((nc, 3))
+    cell_cen[:, :2] = tc[tt]
+    cell_cen[:, 2] = 0.5 * (Z[kk] + Z[kk + 1])
+    cell_vol = tarea[tt] * dz[kk]
+    face_cen = np.zeros((nf, 3))
+    face_area = np.zeros(nf)
+    fnrm = np.zeros((nf, 3))
+    face_cen[:nh, :2] = tc[ht]
+    face_cen[:nh, 2] = Z[hk]
+    face_area[:nh] = tarea[ht]
+    fnrm[:nh, 2] = 1.0
+    q0, q1 = P[p0], P[p1]
+    face_cen[nh:, :2] = 0.5 * (q0 + q1)
+    face_cen[nh:, 2] = 0.5 * (Z[vk] + Z[vk + 1])
+    el = np.sqrt(((q1 - q0) ** 2).sum(axis=1))
+    face_area[nh:] = el * dz[vk]
+    fnrm[nh:, 0] = (q1 - q0)[:, 1] / el
+    fnrm[nh:, 1] = -(q1 - q0)[:, 0] / el
+    # orient normals from up to dn (boundary: from the outside into the cell)
+    up, dn = face_cells[:, 0], face_cells[:, 1]
+    ref_from = np.where(up[:, None] >= 0, cell_cen[np.maximum(up, 0)], face_cen)
+    ref_to = cell_cen[dn]
+    sgn = np.sign(((ref_to - ref_from) * fnrm).sum(axis=1))
+    sgn[sgn == 0] = 1.0
+    fnrm = fnrm * sgn[:, None]
+    return Mesh(nx=nt, ny=1, nz=nl, vert_xyz=V, cell_verts=cell_verts, cell_faces=cell_faces,
+                face_verts=face_verts, face_cells=face_cells, cell_cen=cell_cen, cell_vol=cell_vol,
+                face_cen=face_cen, face_area=face_area, face_normal=fnrm, bnd_face=bnd_face,
+                face_dir=face_dir, face_side=face_side)
+
+
+def delaunay_square(n_tri: int, length: float, rng: np.random.Generator, jitter: float = 0.35):
+    """Random Delaunay triangulation of a square with about n_tri triangles:
+    an m x m lattice (2(m-1)^2 ~ n_tri) whose interior points move by
+    U(-jitter, +jitter) lattice spacings, triangulated by scipy's Delaunay.
+    Returns (points (np,2), triangles (nt,3))."""
+    from scipy.spatial import Delaunay
+    m = int(round(np.sqrt(n_tri / 2.0))) + 1
+    h = length / (m - 1)
+    I, J = np.meshgrid(np.arange(m), np.arange(m), indexing="xy")
+    X = np.stack([I.ravel() * h, J.ravel() * h], axis=1)
+    inner = (I.ravel() > 0) & (I.ravel() < m - 1) & (J.ravel() > 0) & (J.ravel() < m - 1)
+    X[inner] += rng.uniform(-jitter, jitter, size=(int(inner.sum()), 2)) * h
+    tri = Delaunay(X)
+    return X, tri.simplices.astype(np.int64)
