@@ -38,7 +38,9 @@ enum {
 enum { TDYHIP_BC_INTERNAL = -1, TDYHIP_BC_DIRICHLET = 0, TDYHIP_BC_NEUMANN = 1, TDYHIP_BC_SEEPAGE = 2 };
 
 /* Mesh in the form TDyMeshCreateFromPlex produces (src/fv/share/tdymeshplex.c:3685):
- * hexahedra, 8 vertices per cell, 6 faces per cell, 4 vertices per face
+ * hexahedra, 8 vertices per cell, 6 faces per cell, 4 vertices per face (right prisms:
+ * 6 vertices, 5 faces and triangles of 3 vertices, each array padded with -1; such a mesh
+ * gets the transmissibility setup only, see tdyhip_setup_info)
  * (cyclic), face_cells = (up, dn) with the unit normal from up to dn and a
  * boundary face stored as (-(b+1), cell).  Host pointers. */
 typedef struct {
@@ -192,6 +194,16 @@ int tdyhip_kernel_timer_report(tdyhip_ctx *ctx, char *buf, int64_t buflen);
 int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz);
 int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals);
 int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
+/* Setup of a general mesh (any cell other than a hexahedron: right prisms, given in the
+ * hexahedral arrays padded with -1).  Replaces ComputeTransmissibilityMatrix and
+ * ComputeGravityDiscretization for vertices of any degree (src/fv/mpfao/tdympfao.c:1801-1860,
+ * 1126-1513, 2169-2274); tdyhip_create runs it, and only the transmissibility and gravity
+ * exports above (and tdyhip_setup_info) are available on such a context: the flow calls
+ * return TDYHIP_ERR_ARG.  ms[3] = device times of the G matrices, the size-bucketed local
+ * systems and the gravity term (HIP events, general meshes only); counts[8] = interaction
+ * regions in the 16-, 32- and 64-lane buckets, the most cells and faces at one vertex, the
+ * transmissibility entries, and the algorithmic FP64 flops and bytes of the local systems. */
+int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms /* 3 */, int64_t *counts /* 8 */);
 
 #ifdef __cplusplus
 }

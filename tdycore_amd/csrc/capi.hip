@@ -7,6 +7,7 @@
 
 #include "flow_kernels.hip"
 #include "setup_kernels.hip"
+#include "setup_general.hip"
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
@@ -250,6 +251,156 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
 
 using namespace tdyhip;
 
+namespace tdyhip {
+
+static int no_flow(const tdyhip_ctx *ctx) {
+  (void)ctx;
+  set_error("this mesh has cells other than hexahedra: only its transmissibility setup runs on the device "
+            "(the residual, Jacobian and Newton kernels take at most 8 columns per flux row)");
+  return TDYHIP_ERR_ARG;
+}
+
+template <int LANES>
+static void launch_ir_general(const GenArgs &ga, hipStream_t st) {
+  constexpr int TEAMS = 64 / LANES;
+  k_ir_general<LANES><<<(unsigned)((ga.n + TEAMS - 1) / TEAMS), 64, 0, st>>>(ga);
+}
+
+// Transmissibility setup of a general (prism) mesh: G matrices, the
+// size-bucketed local systems and the gravity term, with HIP-event times of
+// the three phases in ctx->setup_ms.
+static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_materials *mat) {
+  HostGeneral *g = new HostGeneral();
+  ctx->hgen = g;
+  ctx->general = true;
+  std::string msg = build_topology_general(*m, *g);
+  if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+  const int nc = m->nc, nf = m->nf;
+  int rc;
+  if ((rc = dupload(ctx, &ctx->d_face_cells, m->face_cells, (size_t)2 * nf))) return rc;
+  if ((rc = dupload(ctx, &ctx->d_face_flags, g->face_flags.data(), nf))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_G, (size_t)4 * nf))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_Tgen, (size_t)std::max<int64_t>(g->tsize, 1)))) return rc;
+  std::vector<void *> tmp;
+  auto done = [&](int code) {
+    for (void *p : tmp) (void)hipFree(p);
+    return code;
+  };
+  auto up = [&](auto **ptr, const auto *src, size_t n) -> int {
+    hipError_t e = hipMalloc((void **)ptr, std::max<size_t>(n, 1) * sizeof(**ptr));
+    if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return e == hipErrorOutOfMemory ? TDYHIP_ERR_MEM : TDYHIP_ERR_HIP; }
+    tmp.push_back((void *)*ptr);
+    if (src && n && copy_sync(ctx, *ptr, src, n * sizeof(**ptr), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("hipMemcpy failed");
+      return TDYHIP_ERR_HIP;
+    }
+    return 0;
+  };
+  int32_t *d_scf = nullptr, *d_vdesc = nullptr, *d_csub = nullptr, *d_csck = nullptr, *d_fa = nullptr,
+          *d_fb = nullptr, *d_vl[3] = {nullptr, nullptr, nullptr};
+  uint8_t *d_fnv = nullptr;
+  double *d_cc = nullptr, *d_fc = nullptr, *d_fa2 = nullptr, *d_fn = nullptr, *d_K = nullptr, *d_Gsub = nullptr;
+  int *d_status = nullptr;
+  if ((rc = up(&d_scf, g->sc_faces.data(), g->sc_faces.size())) || (rc = up(&d_vdesc, g->vdesc.data(), g->vdesc.size())) ||
+      (rc = up(&d_csub, g->csub.data(), g->csub.size())) || (rc = up(&d_csck, g->csck.data(), g->csck.size())) ||
+      (rc = up(&d_fa, g->fa.data(), g->fa.size())) || (rc = up(&d_fb, g->fb.data(), g->fb.size())) ||
+      (rc = up(&d_fnv, g->face_nv.data(), g->face_nv.size())) || (rc = up(&d_cc, m->cell_cen, (size_t)3 * nc)) ||
+      (rc = up(&d_fc, m->face_cen, (size_t)3 * nf)) || (rc = up(&d_fa2, m->face_area, (size_t)nf)) ||
+      (rc = up(&d_fn, m->face_normal, (size_t)3 * nf)) || (rc = up(&d_K, mat->K, (size_t)9 * nc)) ||
+      (rc = up(&d_Gsub, (const double *)nullptr, (size_t)nc * 72)) || (rc = up(&d_status, (const int *)nullptr, 1)))
+    return done(rc);
+  for (int b = 0; b < 3; b++)
+    if ((rc = up(&d_vl[b], g->bucket[b].data(), g->bucket[b].size()))) return done(rc);
+  hipStream_t st = ctx->stream;
+  hipEvent_t ev[4];
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); return done(TDYHIP_ERR_HIP); }
+  (void)hipMemsetAsync(d_status, 0, sizeof(int), st);
+  (void)hipEventRecord(ev[0], st);
+  k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa2,
+                                                        d_fn, d_K, d_fnv, d_Gsub);
+  (void)hipEventRecord(ev[1], st);
+  GenArgs ga{nullptr, d_vdesc, d_csub, d_csck, d_fa, d_fb, d_Gsub, ctx->d_Tgen, d_status, 0};
+  for (int b = 0; b < 3; b++) {
+    ga.vlist = d_vl[b];
+    ga.n = (int)g->bucket[b].size();
+    if (ga.n == 0) continue;
+    if (b == 0) launch_ir_general<16>(ga, st);
+    else if (b == 1) launch_ir_general<32>(ga, st);
+    else launch_ir_general<64>(ga, st);
+  }
+  (void)hipEventRecord(ev[2], st);
+  k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
+                                           d_fc, d_fa2, d_fn, d_K, d_fnv, ctx->d_G);
+  (void)hipEventRecord(ev[3], st);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    set_error("general setup kernels failed");
+    return done(TDYHIP_ERR_HIP);
+  }
+  for (int i = 0; i < 3; i++) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+    ctx->setup_ms[i] = ms;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  int status = 0;
+  if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return done(TDYHIP_ERR_HIP);
+  if (status) { set_error("singular interaction-region system (zero LU pivot)"); return done(TDYHIP_ERR_SINGULAR); }
+  ctx->h_face_verts.assign(m->face_verts, m->face_verts + (size_t)4 * nf);
+  ctx->h_face_flags = g->face_flags;
+  return done(0);
+}
+
+// Trans as CSR (rows face*4+subface, columns sorted: cells, then nc + boundary
+// index) from the per-vertex blocks of a general mesh.
+static int trans_csr_general(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals) {
+  const HostGeneral &g = *ctx->hgen;
+  const int64_t nrows = (int64_t)4 * ctx->nf;
+  std::vector<double> T((size_t)std::max<int64_t>(g.tsize, 1));
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      copy_sync(const_cast<tdyhip_ctx *>(ctx), T.data(), ctx->d_Tgen, (size_t)g.tsize * sizeof(double),
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("copy of the transmissibility blocks failed");
+    return TDYHIP_ERR_HIP;
+  }
+  std::vector<int64_t> cnt(nrows + 1, 0);
+  for (int v = 0; v < ctx->nv; v++) {
+    const int32_t *d = g.vdesc.data() + (size_t)v * 8;
+    for (int k = 0; k < d[1]; k++)
+      if ((g.fb[d[5] + k] >> 16) > 0) cnt[g.frow[d[5] + k] + 1] += d[0] + d[3];
+  }
+  for (int64_t r = 0; r < nrows; r++) cnt[r + 1] += cnt[r];
+  if (cnt[nrows] > INT32_MAX) { set_error("transmissibility has more than 2^31 entries"); return TDYHIP_ERR_ARG; }
+  for (int64_t r = 0; r <= nrows; r++) rowptr[r] = (int32_t)cnt[r];
+  std::vector<int32_t> dcol;
+  for (int v = 0; v < ctx->nv; v++) {
+    const int32_t *d = g.vdesc.data() + (size_t)v * 8;
+    const int ncell = d[0], nface = d[1], ndir = d[3], oc = d[4], of = d[5], ot = d[6], nB = ncell + ndir;
+    dcol.assign(ndir, -1);
+    for (int k = 0; k < nface; k++)
+      if ((g.fa[of + k] & 3) == 2) dcol[(g.fa[of + k] >> 8) & 0xff] = g.fcol[of + k];
+    for (int k = 0; k < nface; k++) {
+      const int trow = (g.fb[of + k] >> 16) - 1;
+      if (trow < 0) continue;
+      const int64_t base = cnt[g.frow[of + k]];
+      int64_t n = 0;
+      for (int j = 0; j < nB; j++) {  // insertion sort by column
+        const int col = j < ncell ? g.ccell[oc + j] : dcol[j - ncell];
+        const double val = T[(size_t)ot + (size_t)trow * nB + j];
+        int64_t p = n;
+        while (p > 0 && cols[base + p - 1] > col) { cols[base + p] = cols[base + p - 1]; vals[base + p] = vals[base + p - 1]; p--; }
+        cols[base + p] = col;
+        vals[base + p] = val;
+        n++;
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace tdyhip
+
 extern "C" {
 
 const char *tdyhip_last_error(void) { return g_err.c_str(); }
@@ -259,9 +410,13 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   *out = nullptr;
   if (m->nc <= 0 || m->nf <= 0 || m->nv <= 0 || m->nbnd < 0) { set_error("empty mesh"); return TDYHIP_ERR_ARG; }
   if ((int64_t)m->nf * 4 * kMaxCols > ((int64_t)1 << 40)) { set_error("mesh too large"); return TDYHIP_ERR_ARG; }
+  // hexahedral meshes take the full path; other cells (prisms) the general setup only
+  const bool hex = mesh_is_hex(*m);
   HostTopology topo;
-  std::string msg = build_topology(*m, topo);
-  if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+  if (hex) {
+    std::string msg = build_topology(*m, topo);
+    if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+  }
 
   tdyhip_ctx *ctx = new tdyhip_ctx();
   ctx->nc = m->nc; ctx->nf = m->nf; ctx->nv = m->nv; ctx->nbnd = m->nbnd;
@@ -281,6 +436,11 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (he != hipSuccess) { set_error(std::string("hipStreamCreate: ") + hipGetErrorString(he)); delete ctx; return TDYHIP_ERR_HIP; }
   ctx->own_stream = true;
+  if (!hex) {
+    if ((rc = setup_general(ctx, m, mat))) return fail(rc);
+    *out = ctx;
+    return 0;
+  }
   const int nc = m->nc, nf = m->nf, nv = m->nv, nbnd = m->nbnd;
   const int64_t nrows = (int64_t)4 * nf;
   ctx->jnnz = topo.jrp[nc];
@@ -371,14 +531,14 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   k_fill<<<nblk((int64_t)kMaxCols * nrows, 256), 256, 0, st>>>((int64_t)kMaxCols * nrows, ctx->d_T, 0.0);
   if (hipMemsetAsync(d_status, 0, sizeof(int), st) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
   k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa,
-                                                        d_fn, d_K, d_Gsub);
+                                                        d_fn, d_K, nullptr, d_Gsub);
   KT(ctx, st, "k_local_systems", (double)nv * sizeof(IRDesc) + (double)nc * 72 * 8 + (double)ctx->trans_nnz * 8,
      k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
                                                                                  d_status));
   k_gather_table<<<nblk(nc, 256), 256, 0, st>>>(nc, ctx->d_cell_faces, ctx->d_face_cells, ctx->d_face_verts,
                                                 ctx->d_face_flags, d_rul, ctx->d_vcols, d_jsl, ctx->d_gtab);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
-                                           d_fc, d_fa, d_fn, d_K, ctx->d_G);
+                                           d_fc, d_fa, d_fn, d_K, nullptr, ctx->d_G);
   // boundary rho, mu frozen at (Pref, Tref) (AllocateMemoryForBoundaryValues, tdympfao.c:297-336)
   {
     double rb, d1, d2;
@@ -415,6 +575,7 @@ int tdyhip_destroy(tdyhip_ctx *ctx) {
   ctx->ws = nullptr;
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx->hgen;
   delete ctx;
   return 0;
 }
@@ -444,6 +605,7 @@ int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out) {
 }
 
 int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   std::memcpy(rowptr, ctx->h_jrp.data(), sizeof(int32_t) * (ctx->nc + 1));
   std::memcpy(cols, ctx->h_jcols.data(), sizeof(int32_t) * ctx->jnnz);
@@ -451,6 +613,7 @@ int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *col
 }
 
 int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr, const int32_t **cols) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   *rowptr = ctx->d_jrp;
   *cols = ctx->d_jcols;
@@ -458,6 +621,7 @@ int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr
 }
 
 int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R, tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -468,6 +632,7 @@ int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *U
 
 int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift, double *J,
                      tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -476,6 +641,7 @@ int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *U
 
 int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift,
                                double *R, double *J, tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !R || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -491,6 +657,7 @@ int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt) {
 }
 
 int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !Uprev) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   StateParams sp = state_params(ctx, 1, 0.0);
@@ -501,6 +668,7 @@ int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream s) 
 }
 
 int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   int rc = launch_state(ctx, 1, U, nullptr, st);
@@ -510,6 +678,7 @@ int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_str
 
 int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liquid_mass, double *water_mass,
                        tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   StateParams sp = state_params(ctx, 1, 0.0);
@@ -520,6 +689,7 @@ int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liqu
 }
 
 int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *J, tdyhip_stream s) {
+  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   return launch_flux(ctx, 1, 0.0, U, nullptr, nullptr, J, st);
@@ -527,6 +697,10 @@ int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *J, tdyhip_str
 
 int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz) {
   if (!ctx || !nnz) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  if (ctx->general) {
+    *nnz = ctx->hgen->tsize;
+    return 0;
+  }
   int64_t n = 0;
   for (int f = 0; f < ctx->nf; f++) {
     if (ctx->h_face_flags[f] & kFaceNoFlow) continue;
@@ -541,6 +715,7 @@ int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz) {
 
 int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals) {
   if (!ctx || !rowptr || !cols || !vals) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  if (ctx->general) return trans_csr_general(ctx, rowptr, cols, vals);
   const int64_t nrows = (int64_t)4 * ctx->nf;
   std::vector<double> T((size_t)kMaxCols * nrows);
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -563,6 +738,32 @@ int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, doub
       }
     }
     rowptr[r + 1] = (int32_t)n;
+  }
+  return 0;
+}
+
+int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms, int64_t *counts) {
+  if (!ctx || !ms || !counts) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  for (int i = 0; i < 3; i++) ms[i] = ctx->setup_ms[i];
+  for (int i = 0; i < 8; i++) counts[i] = 0;
+  if (ctx->general) {
+    const HostGeneral &g = *ctx->hgen;
+    for (int b = 0; b < 3; b++) counts[b] = (int64_t)g.bucket[b].size();
+    counts[3] = g.max_cells;
+    counts[4] = g.max_faces;
+    counts[5] = g.tsize;
+    // algorithmic work of the local systems: elimination on [A | B]
+    // (2/3 nU^3 + 2 nU^2 nB), back substitution (nU^2 nB), T = C X (6 nrow nB);
+    // bytes: descriptors, the region's G rows and its T block
+    double fl = 0.0, by = 0.0;
+    for (int v = 0; v < ctx->nv; v++) {
+      const int32_t *d = g.vdesc.data() + (size_t)v * 8;
+      const double nU = d[2], nB = d[0] + d[3], nrow = d[7];
+      fl += 2.0 / 3.0 * nU * nU * nU + 3.0 * nU * nU * nB + 6.0 * nrow * nB;
+      by += 32.0 + d[0] * (8.0 + 3 * 9 * 8.0) + d[1] * 8.0 + nrow * nB * 8.0;
+    }
+    counts[6] = (int64_t)fl;
+    counts[7] = (int64_t)by;
   }
   return 0;
 }

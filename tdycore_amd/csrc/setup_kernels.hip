@@ -21,12 +21,13 @@ __global__ void k_gmatrix(int nc, int gmatrix, const int32_t *__restrict__ sc_fa
                           const int32_t *__restrict__ face_cells, const double *__restrict__ cell_cen,
                           const double *__restrict__ face_cen, const double *__restrict__ face_area,
                           const double *__restrict__ face_normal, const double *__restrict__ K0,
-                          double *__restrict__ Gsub) {
+                          const uint8_t *__restrict__ face_nv, double *__restrict__ Gsub) {
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= nc * 8) return;
   const int c = sc >> 3;
   int fid[3];
   for (int k = 0; k < 3; k++) fid[k] = sc_faces[sc * 3 + k];
+  if (fid[0] < 0) return;  // absent corner of a prism
   double cc[3], Kc[9];
   for (int d = 0; d < 3; d++) cc[d] = cell_cen[3 * c + d];
   for (int i = 0; i < 9; i++) Kc[i] = K0[c * 9 + i];
@@ -41,7 +42,8 @@ __global__ void k_gmatrix(int nc, int gmatrix, const int32_t *__restrict__ sc_fa
   double *G = Gsub + (size_t)sc * 9;
   for (int i = 0; i < 3; i++) {
     const int f = fid[i];
-    const double area = face_area[f] / 4.0;
+    // sub-face = face area over the face's vertex count (tdymeshplex.c:2937-2943)
+    const double area = face_area[f] / (face_nv ? (double)face_nv[f] : 4.0);
     const double n0 = face_normal[3 * f], n1 = face_normal[3 * f + 1], n2 = face_normal[3 * f + 2];
     if (gmatrix == 0) {
       for (int j = 0; j < 3; j++) {
@@ -268,13 +270,13 @@ __global__ void k_gravdis(int nf, int gmatrix, double gravity, const int32_t *__
                           const uint8_t *__restrict__ face_flags, const double *__restrict__ cell_cen,
                           const double *__restrict__ face_cen, const double *__restrict__ face_area,
                           const double *__restrict__ face_normal, const double *__restrict__ K0,
-                          double *__restrict__ Gout) {
+                          const uint8_t *__restrict__ face_nv, double *__restrict__ Gout) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   double val = 0.0;
   if (!(face_flags[f] & kFaceNoFlow)) {
     const int up = face_cells[2 * f], dn = face_cells[2 * f + 1];
-    double area = face_area[f] / 4.0;
+    double area = face_area[f] / (face_nv ? (double)face_nv[f] : 4.0);
     double fc[3], cu[3], cd[3], n[3], u[3];
     for (int d = 0; d < 3; d++) {
       fc[d] = face_cen[3 * f + d];
@@ -318,7 +320,9 @@ __global__ void k_gravdis(int nf, int gmatrix, double gravity, const int32_t *__
     const double gz = fabs(gravity) * u[2];
     val = area * kval * gz;
   }
-  for (int q = 0; q < 4; q++) Gout[(int64_t)f * 4 + q] = val;
+  // a triangle's fourth row is unused (rows stay face*4 + local vertex)
+  const int nvf = face_nv ? face_nv[f] : 4;
+  for (int q = 0; q < 4; q++) Gout[(int64_t)f * 4 + q] = q < nvf ? val : 0.0;
 }
 
 }  // namespace tdyhip

@@ -53,6 +53,33 @@ struct HostTopology {
 // Builds the host topology; returns an error message or empty string.
 std::string build_topology(const tdyhip_mesh &m, HostTopology &t);
 
+// Interaction regions of a general mesh (hexahedra and right prisms, padded
+// with -1 in the hexahedral arrays), of any size up to kGenMaxLanes unknowns
+// and columns, as variable-length records for the size-bucketed setup
+// kernel.  Per vertex v, vdesc[8v..8v+7] = ncell, nface, nU, ndir, offset of
+// its cells, offset of its faces, offset of its T block, T rows.
+constexpr int kGenMaxLanes = 64;
+struct HostGeneral {
+  std::vector<int32_t> vdesc;     // nv*8
+  std::vector<int32_t> csub;      // per IR cell: subcell id (cell*8 + corner)
+  std::vector<int32_t> ccell;     // per IR cell: the cell (its T column)
+  std::vector<int32_t> csck;      // per IR cell: vertex-face index of its 3 local faces (k0 | k1<<8 | k2<<16)
+  std::vector<int32_t> fa, fb;    // per IR face: role | fidx<<8 | fup_i<<16 | fup_l<<24;
+                                  //              fdn_i | fdn_l<<8 | (T row + 1)<<16
+  std::vector<int32_t> frow;      // per IR face: flux row face*4+q
+  std::vector<int32_t> fcol;      // per IR face: T column nc+b of a Dirichlet face, else -1
+  std::vector<int32_t> bucket[3]; // vertices whose region fits 16, 32 and 64 lanes
+  std::vector<int32_t> sc_faces;  // nc*8*3 subcell face triples (oriented), -1 for absent corners
+  std::vector<uint8_t> face_nv;   // nf: vertices of each face (3 or 4)
+  std::vector<uint8_t> face_flags;// nf
+  std::vector<int32_t> bnd_cell;  // nbnd
+  int64_t tsize = 0;              // T entries (sum of rows x columns)
+  int max_cells = 0, max_faces = 0;
+};
+std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g);
+// True when every cell has 8 vertices and every face 4 (the hexahedral path).
+bool mesh_is_hex(const tdyhip_mesh &m);
+
 void set_error(const std::string &msg);
 
 struct SolverWS;
@@ -133,4 +160,9 @@ struct tdyhip_ctx {
   // algorithmic-byte bookkeeping
   int64_t trans_nnz = 0;           // transmissibility entries (cells + Dirichlet columns)
   int64_t active_rows = 0;         // flux rows of faces that carry flux
+  // general (prism) meshes: transmissibility setup only, T as per-vertex blocks
+  bool general = false;
+  double *d_Tgen = nullptr;        // sum over vertices of rows x columns, row-major blocks
+  tdyhip::HostGeneral *hgen = nullptr;
+  double setup_ms[3] = {0, 0, 0};  // general setup: G matrices, bucketed local systems, gravity
 };

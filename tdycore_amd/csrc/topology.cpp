@@ -209,4 +209,176 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
   return std::string();
 }
 
+bool mesh_is_hex(const tdyhip_mesh &m) {
+  for (int64_t i = 0; i < (int64_t)m.nc * 8; i++)
+    if (m.cell_verts[i] < 0) return false;
+  for (int64_t i = 0; i < (int64_t)m.nc * 6; i++)
+    if (m.cell_faces[i] < 0) return false;
+  for (int64_t i = 0; i < (int64_t)m.nf * 4; i++)
+    if (m.face_verts[i] < 0) return false;
+  return true;
+}
+
+// The same bookkeeping as build_topology, for cells and faces of any vertex
+// count (the reference's general path: ComputeTransmissibilityMatrix over
+// vertices of any degree, tdympfao.c:1801-1860, with the local systems of
+// 1126-1513 sized by the region).
+std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g) {
+  const int nc = m.nc, nf = m.nf, nv = m.nv, nbnd = m.nbnd;
+  const int32_t *cv = m.cell_verts, *cf = m.cell_faces, *fv = m.face_verts, *fc = m.face_cells;
+  g.face_flags.assign(nf, 0);
+  g.face_nv.assign(nf, 0);
+  g.bnd_cell.assign(nbnd, -1);
+  for (int f = 0; f < nf; f++) {
+    int n = 0;
+    for (int q = 0; q < 4; q++) {
+      const int v = fv[f * 4 + q];
+      if (v >= nv) return "face vertex out of range";
+      if (v >= 0) {
+        if (q != n) return "face vertices must come first, then -1 padding";
+        n++;
+      }
+    }
+    if (n < 3) return "face with fewer than 3 vertices";
+    g.face_nv[f] = (uint8_t)n;
+    int up = fc[2 * f], dn = fc[2 * f + 1];
+    if (up < 0 && dn < 0) return "face with two boundary sides";
+    if (up < 0 || dn < 0) {
+      if (up >= 0) return "boundary face must be stored as (-(b+1), cell)";
+      int b = -up - 1;
+      if (b >= nbnd) return "boundary index out of range";
+      g.bnd_cell[b] = dn;
+      uint8_t fl = kFaceBoundary;
+      int bc = m.face_bc[f];
+      if (bc == TDYHIP_BC_NEUMANN) fl |= kFaceNoFlow;
+      else if (bc == TDYHIP_BC_SEEPAGE) fl |= kFaceSeepage;
+      else if (bc != TDYHIP_BC_DIRICHLET) return "boundary face without a boundary type";
+      g.face_flags[f] = fl;
+    }
+  }
+  for (int b = 0; b < nbnd; b++)
+    if (g.bnd_cell[b] < 0) return "boundary index without a face";
+
+  auto fpos = [&](int f, int v) {
+    for (int q = 0; q < g.face_nv[f]; q++)
+      if (fv[f * 4 + q] == v) return q;
+    return -1;
+  };
+  g.sc_faces.assign((size_t)nc * 24, -1);
+  std::vector<int32_t> vc_n(nv + 1, 0), vf_n(nv + 1, 0);
+  for (int c = 0; c < nc; c++) {
+    for (int s = 0; s < 8; s++) {
+      const int v = cv[c * 8 + s];
+      if (v < 0) continue;
+      if (v >= nv) return "cell vertex out of range";
+      vc_n[v + 1]++;
+      int fid[3], n = 0;
+      for (int lf = 0; lf < 6; lf++) {
+        const int f = cf[c * 6 + lf];
+        if (f < 0) continue;
+        if (f >= nf) return "cell face out of range";
+        if (fpos(f, v) >= 0) {
+          if (n < 3) fid[n] = f;
+          n++;
+        }
+      }
+      if (n != 3) return "subcell without exactly 3 faces at its vertex";
+      double a[3], b[3], cvec[3];
+      for (int d = 0; d < 3; d++) {
+        a[d] = m.face_cen[3 * fid[1] + d] - m.face_cen[3 * fid[0] + d];
+        b[d] = m.face_cen[3 * fid[2] + d] - m.face_cen[3 * fid[0] + d];
+        cvec[d] = m.cell_cen[3 * c + d] - m.vert_xyz[3 * v + d];
+      }
+      double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      if (axb[0] * cvec[0] + axb[1] * cvec[1] + axb[2] * cvec[2] < 0) std::swap(fid[1], fid[2]);
+      for (int k = 0; k < 3; k++) g.sc_faces[(size_t)(c * 8 + s) * 3 + k] = fid[k];
+    }
+  }
+  for (int f = 0; f < nf; f++)
+    for (int q = 0; q < g.face_nv[f]; q++) vf_n[fv[f * 4 + q] + 1]++;
+  for (int v = 0; v < nv; v++) { vc_n[v + 1] += vc_n[v]; vf_n[v + 1] += vf_n[v]; }
+  // vertex -> (cell, corner) in cell order; vertex -> (face, q), internal faces first
+  std::vector<int32_t> vcell(vc_n[nv]), vsub(vc_n[nv]), vface(vf_n[nv]), vq(vf_n[nv]);
+  {
+    std::vector<int32_t> fill(nv, 0);
+    for (int c = 0; c < nc; c++)
+      for (int s = 0; s < 8; s++) {
+        const int v = cv[c * 8 + s];
+        if (v < 0) continue;
+        const int k = vc_n[v] + fill[v]++;
+        vcell[k] = c; vsub[k] = s;
+      }
+    std::fill(fill.begin(), fill.end(), 0);
+    for (int pass = 0; pass < 2; pass++)
+      for (int f = 0; f < nf; f++) {
+        if (((g.face_flags[f] & kFaceBoundary) != 0) != (pass == 1)) continue;
+        for (int q = 0; q < g.face_nv[f]; q++) {
+          const int v = fv[f * 4 + q];
+          const int k = vf_n[v] + fill[v]++;
+          vface[k] = f; vq[k] = q;
+        }
+      }
+  }
+  g.vdesc.assign((size_t)nv * 8, 0);
+  g.csub.assign(vc_n[nv], -1); g.ccell.assign(vc_n[nv], -1); g.csck.assign(vc_n[nv], 0);
+  g.fa.assign(vf_n[nv], 0); g.fb.assign(vf_n[nv], 0); g.frow.assign(vf_n[nv], -1); g.fcol.assign(vf_n[nv], -1);
+  for (int b = 0; b < 3; b++) g.bucket[b].clear();
+  int64_t toff = 0;
+  for (int v = 0; v < nv; v++) {
+    const int oc = vc_n[v], of = vf_n[v];
+    const int ncell = vc_n[v + 1] - oc, nface = vf_n[v + 1] - of;
+    g.max_cells = std::max(g.max_cells, ncell);
+    g.max_faces = std::max(g.max_faces, nface);
+    int nint = 0, nneu = 0, ndir = 0;
+    for (int k = 0; k < nface; k++)
+      if (!(g.face_flags[vface[of + k]] & kFaceBoundary)) nint++;
+    std::vector<int> role(nface), fidx(nface), fup_i(nface, -1), fup_l(nface, 0), fdn_i(nface, -1), fdn_l(nface, 0);
+    for (int k = 0; k < nface; k++) {
+      const int f = vface[of + k];
+      const uint8_t fl = g.face_flags[f];
+      if (!(fl & kFaceBoundary)) { role[k] = 0; fidx[k] = k; }
+      else if (fl & kFaceNoFlow) { role[k] = 1; fidx[k] = nint + nneu; nneu++; }
+      else { role[k] = 2; fidx[k] = ndir; ndir++; g.fcol[of + k] = nc + (-fc[2 * f] - 1); }
+    }
+    const int nU = nint + nneu, nB = ncell + ndir;
+    const int need = std::max(nU, nB);
+    if (need > kGenMaxLanes || nface > 255) return "interaction region larger than 64 unknowns or columns";
+    for (int i = 0; i < ncell; i++) {
+      const int c = vcell[oc + i], s = vsub[oc + i];
+      g.csub[oc + i] = c * 8 + s;
+      g.ccell[oc + i] = c;
+      int packed = 0;
+      for (int l = 0; l < 3; l++) {
+        const int f = g.sc_faces[(size_t)(c * 8 + s) * 3 + l];
+        int k = 0;
+        while (k < nface && vface[of + k] != f) k++;
+        if (k == nface) return "subcell face not at its vertex";
+        packed |= k << (8 * l);
+        if (fc[2 * f] == c || (g.face_flags[f] & kFaceBoundary)) { fup_i[k] = i; fup_l[k] = l; }
+        else { fdn_i[k] = i; fdn_l[k] = l; }
+      }
+      g.csck[oc + i] = packed;
+    }
+    // T rows: internal faces, then Dirichlet faces, in vertex face order
+    std::vector<int> trow(nface, -1);
+    int nrow = 0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int k = 0; k < nface; k++)
+        if ((pass == 0 && role[k] == 0) || (pass == 1 && role[k] == 2)) trow[k] = nrow++;
+    for (int k = 0; k < nface; k++) {
+      if (fup_i[k] < 0 || (role[k] == 0 && fdn_i[k] < 0)) return "face without its subcells";
+      g.fa[of + k] = role[k] | (fidx[k] << 8) | (fup_i[k] << 16) | (fup_l[k] << 24);
+      g.fb[of + k] = std::max(fdn_i[k], 0) | (fdn_l[k] << 8) | ((trow[k] + 1) << 16);
+      g.frow[of + k] = vface[of + k] * 4 + vq[of + k];
+    }
+    int32_t *d = g.vdesc.data() + (size_t)v * 8;
+    d[0] = ncell; d[1] = nface; d[2] = nU; d[3] = ndir; d[4] = oc; d[5] = of; d[6] = (int32_t)toff; d[7] = nrow;
+    if (toff + (int64_t)nrow * nB > INT32_MAX) return "transmissibility blocks exceed 2^31 entries";
+    toff += (int64_t)nrow * nB;
+    g.bucket[need <= 16 ? 0 : need <= 32 ? 1 : 2].push_back(v);
+  }
+  g.tsize = toff;
+  return std::string();
+}
+
 }  // namespace tdyhip

@@ -79,3 +79,53 @@ def ensemble_request_specs(n=512, seed=20261206):
     sig = rng.uniform(0.5, 1.5, size=n)
     return [dict(rid=i, nx=int(nx[i]), ny=int(ny[i]), nz=int(nz[i]), seed=int(seed + 1000 * (i + 1)),
                  sigma=float(sig[i])) for i in range(n)]
+
+
+def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lattice=128, gmatrix="mpfao",
+                  points=None, triangles=None):
+    """Right prisms: a random Delaunay triangulation of about n_tri triangles
+    on a length x length square (jittered lattice points, scipy's Delaunay),
+    extruded through nlayers layers of thickness U(0.5, 5) m.  Permeability as
+    the layered box: per layer log10 k ~ N(-12, 1) on a lattice x lattice grid
+    smoothed by a 5x5 moving average (rescaled back to sigma 1) and sampled at
+    the cell centroids, K = diag(k, k, k/10); phi ~ U(0.2, 0.4), Sr 0.1,
+    van Genuchten n = 1.5 (m = 1/3), alpha 1e-4; hydrostatic initial state
+    with the water table at mid-height; Dirichlet top and bottom at the
+    hydrostatic values, no-flow sides.  ``points``/``triangles`` replace the
+    triangulation (then n_tri is unused)."""
+    rng = np.random.default_rng(seed)
+    if points is None:
+        points, triangles = mg.delaunay_square(n_tri, length, rng)
+    Z = np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 5.0, size=nlayers))])
+    m = mg.prism_mesh(points, triangles, Z)
+    field = mg.layered_K(lattice, lattice, nlayers, rng)[:, 0].reshape(nlayers, lattice, lattice)
+    lo, hi = points.min(axis=0), points.max(axis=0)
+    ix = np.clip(((m.cell_cen[:, 0] - lo[0]) / (hi[0] - lo[0]) * lattice).astype(np.int64), 0, lattice - 1)
+    iy = np.clip(((m.cell_cen[:, 1] - lo[1]) / (hi[1] - lo[1]) * lattice).astype(np.int64), 0, lattice - 1)
+    layer = np.arange(m.nc) // triangles.shape[0]
+    k = field[layer, iy, ix]
+    K = np.zeros((m.nc, 9))
+    K[:, 0] = K[:, 4] = k
+    K[:, 8] = k / 10.0
+    phi = rng.uniform(0.2, 0.4, size=m.nc)
+    Sr = np.full(m.nc, 0.1)
+    fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-"])
+    zt = 0.5 * Z[-1]
+    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], zt))
+    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
+    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix=gmatrix, density="constant", p_bnd_fn=pb)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+
+
+def wheel_triangulation(n_spokes=14, rings=2, radius=100.0):
+    """A centre point with n_spokes triangles around it, inside rings of
+    points: its prism vertices have 2*n_spokes cells and 3*n_spokes faces,
+    the largest interaction regions the setup buckets handle."""
+    from scipy.spatial import Delaunay
+    pts = [[0.0, 0.0]]
+    for r in range(1, rings + 1):
+        n = n_spokes * r
+        a = 2 * np.pi * (np.arange(n) + 0.5 * (r % 2)) / n
+        pts += list(np.stack([r * radius * np.cos(a), r * radius * np.sin(a)], 1))
+    P = np.array(pts)
+    return P, Delaunay(P).simplices.astype(np.int64)

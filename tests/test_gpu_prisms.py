@@ -1,0 +1,70 @@
+"""Right-prism meshes on the device: the size-bucketed interaction-region
+setup against the oracle (transmissibilities and gravity term within 1e-12
+of the largest entry of their row), and the flow calls refusing such a mesh."""
+import numpy as np
+import pytest
+
+from tdycore_amd import meshgen as mg
+from tdycore_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _problem(name):
+    if name == "delaunay":
+        return workloads.prism_problem(n_tri=300, nlayers=4, seed=11, length=500.0, lattice=16)
+    if name == "delaunay_tpf":
+        return workloads.prism_problem(n_tri=300, nlayers=3, seed=12, length=500.0, lattice=16, gmatrix="tpf")
+    if name == "wheel":
+        P, T = workloads.wheel_triangulation(n_spokes=14)
+        return workloads.prism_problem(nlayers=3, seed=13, points=P, triangles=T, lattice=8)
+    if name == "wheel_all_dirichlet":
+        P, T = workloads.wheel_triangulation(n_spokes=11)
+        pr = workloads.prism_problem(nlayers=2, seed=14, points=P, triangles=T, lattice=8)
+        m = pr["mesh"]
+        pr["face_bc"] = m.face_bc(default=mg.BC_DIRICHLET)
+        pr["kw"]["p_bnd_fn"] = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 2.0))
+        return pr
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel", "wheel_all_dirichlet"])
+def test_prism_setup_matches_oracle(name):
+    import torch
+    from pyoracle import Oracle
+    from tdycore_amd.mpfao import MPFAO
+    assert torch.cuda.is_available()
+    print(f"[case] prism setup {name}", flush=True)
+    P = _problem(name)
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    rp0, c0, v0 = o.trans_csr()
+    rp1, c1, v1 = g.trans_csr()
+    assert np.array_equal(rp0, rp1) and np.array_equal(c0, c1)
+    live = np.nonzero(rp0[1:] > rp0[:-1])[0]
+    rowmax = np.maximum.reduceat(np.abs(v0), rp0[:-1][live])
+    scale = np.repeat(rowmax, rp0[1:][live] - rp0[:-1][live])
+    assert np.max(np.abs(v1 - v0) / scale) <= TOL
+    g0, g1 = o.gravdis(), g.gravdis()
+    assert np.max(np.abs(g1 - g0)) <= TOL * np.abs(g0).max()
+    info = g.setup_info()
+    assert sum(info["buckets"]) == P["mesh"].nv and info["trans_nnz"] == len(v0)
+    if name.startswith("wheel"):
+        assert info["buckets"][2] > 0  # a region of more than 32 unknowns
+    g.close()
+
+
+def test_flow_calls_refuse_a_prism_mesh():
+    import torch
+    from tdycore_amd import _lib
+    from tdycore_amd.mpfao import MPFAO
+    P = _problem("delaunay")
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    U = torch.tensor(P["U0"], device="cuda")
+    R = torch.zeros_like(U)
+    with pytest.raises(_lib.TdyHipError) as ei:
+        g.IFunction(0.0, U, torch.zeros_like(U), R)
+    assert ei.value.code == 1 and "hexahedra" in str(ei.value)
+    g.close()
