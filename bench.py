@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-Krylov variant")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--no-timers", action="store_true", help="time without per-kernel HIP events")
+    ap.add_argument("--prisms", action="store_true",
+                    help="measure the prism-mesh transmissibility setup instead (random Delaunay, extruded)")
+    ap.add_argument("--prism-tri", type=int, default=200_000, help="prisms: triangles of the 2-D mesh")
+    ap.add_argument("--prism-layers", type=int, default=20, help="prisms: layers")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving ensemble instead (Poisson arrivals of box requests)")
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
@@ -180,6 +184,70 @@ def cpu_baseline(P, dt):
                       f"line search) of step 1 on the same {nc}-cell mesh; {el:.2f} s"}
 
 
+def prisms(a):
+    """Setup of the interaction regions of a right-prism mesh (random Delaunay
+    triangles on a 10 x 10 km square extruded to layers of U(0.5, 5) m): G
+    matrices, size-bucketed local systems, gravity term.  A step is one
+    setup; the value is interaction regions per second of the local-system
+    kernels, with the FP64 roofline of that phase."""
+    import numpy as np
+    import torch
+    from tdycore_amd import workloads
+    from tdycore_amd.mpfao import MPFAO
+    torch.cuda.init()
+    t = time.perf_counter()
+    P = workloads.prism_problem(n_tri=a.prism_tri, nlayers=a.prism_layers)
+    m = P["mesh"]
+    log(f"prism mesh: {m.nc} cells, {m.nf} faces, {m.nv} vertices, built in {time.perf_counter() - t:.1f} s")
+    runs = []
+    for it in range(a.warmup + a.steps):
+        t = time.perf_counter()
+        g = MPFAO(m, P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        wall = time.perf_counter() - t
+        info = g.setup_info()
+        g.close()
+        del g
+        if it >= a.warmup:
+            runs.append((wall, info))
+        log(f"setup {it}: wall {wall:.2f} s, device {info['gmatrix_ms']:.2f} + {info['local_systems_ms']:.2f} + "
+            f"{info['gravity_ms']:.2f} ms, buckets {info['buckets']}, lanes {info['lanes']}")
+    info = runs[-1][1]
+    ls_s = np.mean([r[1]["local_systems_ms"] for r in runs]) / 1e3
+    dev_s = np.mean([r[1]["gmatrix_ms"] + r[1]["local_systems_ms"] + r[1]["gravity_ms"] for r in runs]) / 1e3
+    wall = np.mean([r[0] for r in runs])
+    tflops = info["local_systems_flops"] / ls_s / 1e12
+    line = {
+        "metric": "prism interaction-region setups/s (FP64)", "value": m.nv / ls_s, "unit": "regions/s",
+        "n_gpus": 1, "steps": len(runs), "warmup": a.warmup, "ms_per_step": 1e3 * dev_s,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (jittered-lattice random Delaunay, layered permeability)",
+        "config": {"workload": f"{a.prism_tri} random Delaunay triangles on 10x10 km extruded to "
+                               f"{a.prism_layers} layers ({m.nc} prisms, {m.nv} vertices)",
+                   "cells": m.nc, "faces": m.nf, "vertices": m.nv, "buckets_16_32_64": info["buckets"],
+                   "max_cells_at_vertex": info["max_cells"], "max_faces_at_vertex": info["max_faces"],
+                   "lanes_16_32_64_buckets": info["lanes"],
+                   "trans_nnz": info["trans_nnz"]},
+        "roofline": {"bound": "fp64", "kernel": "k_ir_general (all buckets)", "achieved": round(tflops, 3),
+                     "peak": 78.6, "unit": "TFLOP/s", "frac": round(tflops / 78.6, 4),
+                     "hbm_GB_s": round(info["local_systems_bytes"] / ls_s / 1e9, 1), "traffic": None},
+        "phases_ms": {"gmatrix": info["gmatrix_ms"], "local_systems": info["local_systems_ms"],
+                      "gravity": info["gravity_ms"]},
+        "wall_s_per_setup": wall,
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        from pyoracle import Oracle
+        S = workloads.prism_problem(n_tri=20_000, nlayers=a.prism_layers)
+        t = time.perf_counter()
+        Oracle(S["mesh"], S["face_bc"], S["K"], S["porosity"], S["Sr"], **S["kw"])
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": S["mesh"].nv / el, "unit": "regions/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle setup (G, local systems, gravity, Jacobian pattern) of a "
+                                          f"20,000-triangle x {a.prism_layers}-layer prism mesh "
+                                          f"({S['mesh'].nv} vertices): {el:.2f} s"}
+    print(json.dumps(line), flush=True)
+
+
 def serve(a):
     """Serving ensemble: requests arrive as a Poisson stream and are admitted by
     HBM footprint into a fixed number of device slots."""
@@ -283,6 +351,8 @@ def main():
         sys.exit(2)
     if a.serve:
         return serve(a)
+    if a.prisms:
+        return prisms(a)
     import numpy as np
     import torch
     from tdycore_amd import workloads

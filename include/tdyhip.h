@@ -200,10 +200,11 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
  * 1126-1513, 2169-2274); tdyhip_create runs it, and only the transmissibility and gravity
  * exports above (and tdyhip_setup_info) are available on such a context: the flow calls
  * return TDYHIP_ERR_ARG.  ms[3] = device times of the G matrices, the size-bucketed local
- * systems and the gravity term (HIP events, general meshes only); counts[8] = interaction
+ * systems and the gravity term (HIP events, general meshes only); counts[11] = interaction
  * regions in the 16-, 32- and 64-lane buckets, the most cells and faces at one vertex, the
- * transmissibility entries, and the algorithmic FP64 flops and bytes of the local systems. */
-int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms /* 3 */, int64_t *counts /* 8 */);
+ * transmissibility entries, the algorithmic FP64 flops and bytes of the local systems, and
+ * the team width the autotuner chose for each bucket (0 for an empty bucket). */
+int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms /* 3 */, int64_t *counts /* 11 */);
 
 #ifdef __cplusplus
 }

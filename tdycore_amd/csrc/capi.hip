@@ -3,6 +3,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <array>
+#include <map>
 #include <mutex>
 
 #include "flow_kernels.hip"
@@ -261,10 +263,34 @@ static int no_flow(const tdyhip_ctx *ctx) {
 }
 
 template <int LANES>
-static void launch_ir_general(const GenArgs &ga, hipStream_t st) {
+static int launch_ir_general(const GenArgs &ga, hipStream_t st) {
   constexpr int TEAMS = 64 / LANES;
-  k_ir_general<LANES><<<(unsigned)((ga.n + TEAMS - 1) / TEAMS), 64, 0, st>>>(ga);
+  const size_t lds = (size_t)TEAMS * ga.maxU * ga.pitch * sizeof(double);
+  if (lds > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ir_general<LANES>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    set_error("interaction regions too large for the LDS of one workgroup");
+    return TDYHIP_ERR_ARG;
+  }
+  k_ir_general<LANES><<<(unsigned)((ga.n + TEAMS - 1) / TEAMS), 64, lds, st>>>(ga);
+  return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }
+
+static int launch_ir_lanes(int lanes, const GenArgs &ga, hipStream_t st) {
+  return lanes == 16 ? launch_ir_general<16>(ga, st) : lanes == 32 ? launch_ir_general<32>(ga, st)
+                                                                   : launch_ir_general<64>(ga, st);
+}
+
+// Autotuned team width per bucket: a bucket of at least kTuneMin regions is
+// run once at every width that holds its largest region (16, 32, 64 lanes;
+// a wider team runs fewer regions per wavefront) and the fastest width is
+// kept for this device, bucket and largest-region shape for the rest of the
+// process.  Every width computes the same values (one lane per equation,
+// the same operations in the same order), so the tuning runs also leave T
+// complete.
+constexpr int kTuneMin = 4096;
+static std::mutex g_tune_mu;
+static std::map<std::array<int, 4>, int> g_tune;
 
 // Transmissibility setup of a general (prism) mesh: G matrices, the
 // size-bucketed local systems and the gravity term, with HIP-event times of
@@ -320,14 +346,67 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa2,
                                                         d_fn, d_K, d_fnv, d_Gsub);
   (void)hipEventRecord(ev[1], st);
-  GenArgs ga{nullptr, d_vdesc, d_csub, d_csck, d_fa, d_fb, d_Gsub, ctx->d_Tgen, d_status, 0};
+  (void)hipEventSynchronize(ev[1]);
+  float gm_ms = 0.f;
+  (void)hipEventElapsedTime(&gm_ms, ev[0], ev[1]);
+  GenArgs ga{nullptr, d_vdesc, d_csub, d_csck, d_fa, d_fb, d_Gsub, ctx->d_Tgen, d_status, 0, 1, 1};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
   for (int b = 0; b < 3; b++) {
     ga.vlist = d_vl[b];
     ga.n = (int)g->bucket[b].size();
+    g->lanes[b] = 0;
     if (ga.n == 0) continue;
-    if (b == 0) launch_ir_general<16>(ga, st);
-    else if (b == 1) launch_ir_general<32>(ga, st);
-    else launch_ir_general<64>(ga, st);
+    int mu = 1, mb = 1;
+    for (int v : g->bucket[b]) {
+      mu = std::max(mu, g->vdesc[(size_t)v * 8 + 2]);
+      mb = std::max(mb, g->vdesc[(size_t)v * 8 + 0] + g->vdesc[(size_t)v * 8 + 3]);
+    }
+    ga.maxU = mu;
+    ga.pitch = (mu + mb) | 1;
+    const std::array<int, 4> key{dev, b, mu, mb};
+    int lanes = 16 << b;
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(key);
+      if (it != g_tune.end()) { lanes = it->second; cached = true; }
+    }
+    if (!cached && ga.n >= kTuneMin) {
+      float best = 1e30f;
+      for (int w = 16 << b; w <= 64; w *= 2) {
+        hipEvent_t t0, t1;
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        (void)hipEventRecord(t0, st);
+        if ((rc = launch_ir_lanes(w, ga, st))) return done(rc);
+        (void)hipEventRecord(t1, st);
+        (void)hipEventSynchronize(t1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        g->tune_ms[b][w == 16 ? 0 : w == 32 ? 1 : 2] = ms;
+        if (ms < best) { best = ms; lanes = w; }
+      }
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = lanes;
+    }
+    g->lanes[b] = lanes;
+  }
+  (void)hipEventRecord(ev[1], st);
+  for (int b = 0; b < 3; b++) {
+    if (!g->lanes[b]) continue;
+    ga.vlist = d_vl[b];
+    ga.n = (int)g->bucket[b].size();
+    int mu = 1, mb = 1;
+    for (int v : g->bucket[b]) {
+      mu = std::max(mu, g->vdesc[(size_t)v * 8 + 2]);
+      mb = std::max(mb, g->vdesc[(size_t)v * 8 + 0] + g->vdesc[(size_t)v * 8 + 3]);
+    }
+    ga.maxU = mu;
+    ga.pitch = (mu + mb) | 1;
+    if ((rc = launch_ir_lanes(g->lanes[b], ga, st))) return done(rc);
   }
   (void)hipEventRecord(ev[2], st);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
@@ -343,6 +422,7 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
     (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
     ctx->setup_ms[i] = ms;
   }
+  ctx->setup_ms[0] = gm_ms;  // the second event pair brackets the final local-system launches only
   for (auto &e : ev) (void)hipEventDestroy(e);
   int status = 0;
   if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return done(TDYHIP_ERR_HIP);
@@ -745,7 +825,7 @@ int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, doub
 int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms, int64_t *counts) {
   if (!ctx || !ms || !counts) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   for (int i = 0; i < 3; i++) ms[i] = ctx->setup_ms[i];
-  for (int i = 0; i < 8; i++) counts[i] = 0;
+  for (int i = 0; i < 11; i++) counts[i] = 0;
   if (ctx->general) {
     const HostGeneral &g = *ctx->hgen;
     for (int b = 0; b < 3; b++) counts[b] = (int64_t)g.bucket[b].size();
@@ -764,6 +844,7 @@ int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms, int64_t *counts) {
     }
     counts[6] = (int64_t)fl;
     counts[7] = (int64_t)by;
+    for (int b = 0; b < 3; b++) counts[8 + b] = g.lanes[b];
   }
   return 0;
 }

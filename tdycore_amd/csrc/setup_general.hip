@@ -27,6 +27,7 @@ struct GenArgs {
   double *T;
   int *status;
   int n;
+  int maxU, pitch;  // the bucket's largest unknown count; LDS row pitch of [A | B] (odd)
 };
 
 template <int LANES>
@@ -67,19 +68,22 @@ __device__ inline double crow_at(const int (&col)[3], const double (&val)[3], in
 
 template <int LANES>
 __global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
-  constexpr int TEAMS = 64 / LANES, P = LANES + 1;  // odd pitch: column reads spread over the LDS banks
-  __shared__ double lds[TEAMS * 2 * LANES * P];
+  // [A | B] of a region in LDS, rows of a.pitch doubles (odd: column reads
+  // spread over the banks), sized by the bucket's largest region
+  extern __shared__ double lds[];
+  constexpr int TEAMS = 64 / LANES;
+  const int P = a.pitch;
   const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
   const int vi = blockIdx.x * TEAMS + team;
   const bool act = vi < a.n;  // idle teams still reach every barrier
-  double *A = lds + (size_t)team * 2 * LANES * P, *B = A + LANES * P;
+  double *A = lds + (size_t)team * a.maxU * P, *B = A + a.maxU;
   int ncell = 0, nface = 0, nU = 0, ndir = 0, oc = 0, of = 0, ot = 0;
   if (act) {
     const int32_t *d = a.vdesc + (size_t)a.vlist[vi] * 8;
     ncell = d[0]; nface = d[1]; nU = d[2]; ndir = d[3]; oc = d[4]; of = d[5]; ot = d[6];
   }
   const int nB = ncell + ndir;
-  for (int r = 0; r < LANES; r++) { A[r * P + lane] = 0.0; B[r * P + lane] = 0.0; }
+  for (int e = lane; e < a.maxU * P; e += LANES) A[e] = 0.0;
   __syncthreads();
   // assemble: one vertex face (equation) per lane
   for (int k = lane; k < nface; k += LANES) {
@@ -110,10 +114,10 @@ __global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
     }
   }
   __syncthreads();
-  // elimination with partial pivoting on [A | B]; every team runs LANES
+  // elimination with partial pivoting on [A | B]; every team runs maxU
   // steps (the workgroup's barriers), a region's steps past its nU are empty
   int bad = 0;
-  for (int k = 0; k < LANES; k++) {
+  for (int k = 0; k < a.maxU; k++) {
     const bool step = k < nU && !bad;
     double mx = (lane >= k && lane < nU) ? fabs(A[lane * P + k]) : -1.0;
     int p = (lane >= k && lane < nU) ? lane : LANES;

@@ -75,6 +75,8 @@ struct HostGeneral {
   std::vector<int32_t> bnd_cell;  // nbnd
   int64_t tsize = 0;              // T entries (sum of rows x columns)
   int max_cells = 0, max_faces = 0;
+  int lanes[3] = {0, 0, 0};       // team width chosen for each bucket (0: empty bucket)
+  float tune_ms[3][3] = {};       // tuning runs: bucket x width 16/32/64 (0: not run)
 };
 std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g);
 // True when every cell has 8 vertices and every face 4 (the hexahedral path).

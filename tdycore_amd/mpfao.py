@@ -138,13 +138,15 @@ class MPFAO:
         """Device times (ms) of the general-mesh setup phases and its bucket
         counts: {'gmatrix_ms', 'local_systems_ms', 'gravity_ms', 'buckets'
         (regions per 16/32/64-lane bucket), 'max_cells', 'max_faces',
-        'trans_nnz', 'local_systems_flops', 'local_systems_bytes'}.  Times are zero for hexahedral meshes."""
+        'trans_nnz', 'local_systems_flops', 'local_systems_bytes', 'lanes'
+        (team width the autotuner chose per bucket)}.  Times are zero for hexahedral meshes."""
         ms = np.zeros(3)
-        cnt = np.zeros(8, np.int64)
+        cnt = np.zeros(11, np.int64)
         _lib.check(_lib.load().tdyhip_setup_info(self._h, ms.ctypes.data, cnt.ctypes.data))
         return {"gmatrix_ms": float(ms[0]), "local_systems_ms": float(ms[1]), "gravity_ms": float(ms[2]),
                 "buckets": [int(x) for x in cnt[:3]], "max_cells": int(cnt[3]), "max_faces": int(cnt[4]),
-                "trans_nnz": int(cnt[5]), "local_systems_flops": int(cnt[6]), "local_systems_bytes": int(cnt[7])}
+                "trans_nnz": int(cnt[5]), "local_systems_flops": int(cnt[6]), "local_systems_bytes": int(cnt[7]),
+                "lanes": [int(x) for x in cnt[8:11]]}
 
     # ---- TS callbacks ----
     def IFunction(self, t, U, U_t, R):

@@ -15,6 +15,8 @@ TOL = 1e-12
 def _problem(name):
     if name == "delaunay":
         return workloads.prism_problem(n_tri=300, nlayers=4, seed=11, length=500.0, lattice=16)
+    if name == "delaunay_tuned":  # buckets above the autotuner's threshold (4096 regions)
+        return workloads.prism_problem(n_tri=8000, nlayers=4, seed=15, length=2000.0, lattice=32)
     if name == "delaunay_tpf":
         return workloads.prism_problem(n_tri=300, nlayers=3, seed=12, length=500.0, lattice=16, gmatrix="tpf")
     if name == "wheel":
@@ -30,7 +32,7 @@ def _problem(name):
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel", "wheel_all_dirichlet"])
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tuned", "delaunay_tpf", "wheel", "wheel_all_dirichlet"])
 def test_prism_setup_matches_oracle(name):
     import torch
     from pyoracle import Oracle
@@ -53,6 +55,10 @@ def test_prism_setup_matches_oracle(name):
     assert sum(info["buckets"]) == P["mesh"].nv and info["trans_nnz"] == len(v0)
     if name.startswith("wheel"):
         assert info["buckets"][2] > 0  # a region of more than 32 unknowns
+    for b, w in zip(info["buckets"], info["lanes"]):
+        assert (w == 0) == (b == 0) and w in (0, 16, 32, 64)
+    if name == "delaunay_tuned":
+        assert max(info["buckets"]) >= 4096
     g.close()
 
 
