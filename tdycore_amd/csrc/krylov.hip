@@ -44,6 +44,11 @@ struct BJLayout {
   // 16 bits, pairs q and q+1 of slot p sharing the 32-bit word p*kPairW + q/2.
   uint8_t *d_npair = nullptr;  // nblocks*kLU*bs
   uint32_t *d_pair = nullptr;  // nblocks*kPair*bs (kPair = kLU*kPairW words per row)
+  // Shared index patterns: block b's L/U columns are pattern d_pat[b] of
+  // d_cL/d_cU (and its int16 column offsets pattern d_pat[b] of the solver's
+  // offsets); nullptr: every block has its own.
+  int32_t *d_pat = nullptr;    // nblocks
+  int npat = 0;
 };
 
 constexpr int kPairW = (kLU + 1) / 2;  // 32-bit words per lower slot (two 16-bit pairs each)
@@ -191,7 +196,7 @@ template <class TV, int BS>
 __device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, TV *x, int b, int t) {
   const int64_t base = (int64_t)b * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
-  const int64_t cbase = (int64_t)b * kLU * BS;
+  const int64_t cbase = (int64_t)(L.d_pat ? L.d_pat[b] : b) * kLU * BS;
   TV vl[kLU], vu[kLU];
   uint16_t cl[kLU], cu[kLU];
 #pragma unroll
@@ -257,36 +262,54 @@ __device__ inline int blk_col(const TI *__restrict__ ecol, int64_t q, int row) {
 
 // UNROLL: issue a hexahedral row's 27 loads at once (more registers, more
 // memory-level parallelism); worth it where occupancy is set by other phases.
+// Slot k's column index sits at ecol[eb + k*es]: eb = p, es = NB in the
+// slot-major layout; eb = (pattern*W)*bs + t, es = bs with shared patterns.
 template <class TV, class TI, bool UNROLL = (sizeof(TV) == 4)>
-__device__ inline TV blk_row(int64_t NB, int W, int64_t p, int row, const TI *__restrict__ ecol,
-                             const TV *__restrict__ a, const TV *__restrict__ x) {
+__device__ inline TV blk_row(int64_t NB, int W, int64_t p, int row, const TI *__restrict__ ecol, int64_t eb,
+                             int64_t es, const TV *__restrict__ a, const TV *__restrict__ x) {
   TV s = 0;
   if (UNROLL && W == kW) {
     int c[kW];
     TV v[kW];
 #pragma unroll
     for (int k = 0; k < kW; k++) {
-      c[k] = blk_col(ecol, k * NB + p, row);
+      c[k] = blk_col(ecol, eb + k * es, row);
       v[k] = a[k * NB + p];
     }
 #pragma unroll
     for (int k = 0; k < kW; k++) s += v[k] * x[c[k]];
   } else {
-    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[blk_col(ecol, k * NB + p, row)];
+    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[blk_col(ecol, eb + k * es, row)];
   }
   return s;
+}
+
+// Base and stride of block position p's column indices (see blk_row).
+__device__ inline void ecol_addr(const int32_t *__restrict__ pat, int bs, int W, int64_t NB, int64_t p, int64_t &eb,
+                                 int64_t &es) {
+  if (pat) {
+    const int64_t b = p / bs;
+    eb = ((int64_t)pat[b] * W) * bs + (p - b * bs);
+    es = bs;
+  } else {
+    eb = p;
+    es = NB;
+  }
 }
 
 // y[row(p)] = (A x)[row(p)]
 template <class TV, class TI>
 __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32_t *__restrict__ rowof,
-                                                  const TI *__restrict__ ecol, const TV *__restrict__ a,
-                                                  const TV *__restrict__ x, TV *__restrict__ y) {
+                                                  const TI *__restrict__ ecol, const int32_t *__restrict__ pat,
+                                                  int bs, const TV *__restrict__ a, const TV *__restrict__ x,
+                                                  TV *__restrict__ y) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NB) return;
   const int row = rowof[p];
   if (row < 0) return;
-  y[row] = blk_row<TV, TI>(NB, W, p, row, ecol, a, x);
+  int64_t eb, es;
+  ecol_addr(pat, bs, W, NB, p, eb, es);
+  y[row] = blk_row<TV, TI>(NB, W, p, row, ecol, eb, es, a, x);
 }
 
 // Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
@@ -371,7 +394,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
   const int64_t p = (int64_t)b * BS + t;
   const int row = L.d_row[p];
   // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
-  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, a, u) : (TV)0;
+  int64_t eb, es;
+  ecol_addr(L.d_pat, BS, W, NB, p, eb, es);
+  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, eb, es, a, u) : (TV)0;
   if (row >= 0) vj[row] = sc * u[row];
   if (t == 0) x[BS] = (TV)0;
   if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <deque>
 
+#include <unordered_map>
+
 #include "krylov.hip"
 #include "coarse.hip"
 #include "tdyhip_internal.h"
@@ -24,6 +26,12 @@ struct HostBJ {
   std::vector<int32_t> perm;  // nc: block-layout position of each cell
   std::vector<int32_t> ecol;  // W*nblocks*bs: global column of slot k at block position p (k*NB + p)
   std::vector<int16_t> ecol16;  // the same as offsets from the row, when every offset fits in int16
+  // Deduplicated index patterns (int16 offsets only): blocks whose L/U local
+  // columns and column offsets are identical share one copy.  When set, cL/cU
+  // hold npat patterns and ecol16 is pattern-major: slot k of thread t of
+  // pattern q at (q*W + k)*bs + t.
+  std::vector<int32_t> pat;     // nblocks: pattern of each block (empty: one copy per block)
+  int npat = 0;
 };
 
 // Breadth-first partition of the cells into blocks of at most `bs` cells over
@@ -202,6 +210,44 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
         out.ecol16[(size_t)k * NB + p] = r < 0 ? 0 : (int16_t)(out.ecol[(size_t)k * NB + p] - r);
     }
     out.ecol.clear();
+    // Blocks of a structured mesh repeat a few index patterns (an interior
+    // brick's in-block columns and column offsets are those of every other
+    // interior brick): keep one copy per pattern when that at least halves
+    // them, so the matvec streams values, not indices.
+    const size_t lu = (size_t)kLU * S;
+    std::unordered_map<std::string, int> seen;
+    std::vector<int> rep;
+    std::vector<int32_t> pat(nb);
+    std::string key;
+    for (int b = 0; b < nb; b++) {
+      key.assign((const char *)(out.cL.data() + (size_t)b * lu), lu * 2);
+      key.append((const char *)(out.cU.data() + (size_t)b * lu), lu * 2);
+      for (int k = 0; k < W; k++) key.append((const char *)(out.ecol16.data() + (size_t)k * NB + (size_t)b * S), (size_t)S * 2);
+      auto it = seen.find(key);
+      if (it == seen.end()) {
+        it = seen.emplace(key, (int)rep.size()).first;
+        rep.push_back(b);
+      }
+      pat[b] = it->second;
+    }
+    const int npat = (int)rep.size();
+    if (2 * npat <= nb) {
+      std::vector<uint16_t> cL(npat * lu), cU(npat * lu);
+      std::vector<int16_t> e16((size_t)npat * W * S);
+      for (int q = 0; q < npat; q++) {
+        const int b = rep[q];
+        std::copy(out.cL.begin() + (size_t)b * lu, out.cL.begin() + (size_t)(b + 1) * lu, cL.begin() + (size_t)q * lu);
+        std::copy(out.cU.begin() + (size_t)b * lu, out.cU.begin() + (size_t)(b + 1) * lu, cU.begin() + (size_t)q * lu);
+        for (int k = 0; k < W; k++)
+          std::copy(out.ecol16.begin() + (size_t)k * NB + (size_t)b * S,
+                    out.ecol16.begin() + (size_t)k * NB + (size_t)(b + 1) * S, e16.begin() + ((size_t)q * W + k) * S);
+      }
+      out.cL.swap(cL);
+      out.cU.swap(cU);
+      out.ecol16.swap(e16);
+      out.pat.swap(pat);
+      out.npat = npat;
+    }
   }
   return std::string();
 }
@@ -376,6 +422,9 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   ws.bj.bs = hb.bs;
   ws.NB = (int64_t)hb.nblocks * hb.bs;
   if ((rc = up(&ws.d_perm, hb.perm))) return rc;
+  ws.bj.d_pat = nullptr;
+  ws.bj.npat = hb.npat;
+  if (!hb.pat.empty() && (rc = up(&ws.bj.d_pat, hb.pat))) return rc;
   if (!hb.ecol16.empty()) {
     if ((rc = up(&ws.d_ecol16, hb.ecol16))) return rc;
   } else if ((rc = up(&ws.d_ecol, hb.ecol))) {
@@ -448,8 +497,12 @@ static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t s
   return rc;
 }
 
-static double bytes_spmv(const tdyhip_ctx *ctx, size_t tv, bool idx16) {
-  return (double)ctx->jell_w * ctx->nc * (tv + (idx16 ? 2 : 4)) + 2.0 * ctx->nc * tv;
+// Index arrays shared by pattern count once per launch (npat patterns of
+// W*bs offsets), not once per row.
+static double bytes_spmv(const tdyhip_ctx *ctx, const SolverWS &ws, size_t tv, bool idx16) {
+  const double idx = ws.bj.npat ? (double)ws.bj.npat * ctx->jell_w * ws.bj.bs * 2 + (double)ws.bj.nblocks * 4
+                                : (double)ctx->jell_w * ctx->nc * (idx16 ? 2 : 4);
+  return (double)ctx->jell_w * ctx->nc * tv + idx + 2.0 * ctx->nc * tv;
 }
 
 // y = J x with the FP64 Jacobian in the block layout
@@ -457,15 +510,16 @@ static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, dou
   const unsigned g = (unsigned)((ws.NB + 255) / 256);
   const bool i16 = ws.d_ecol16 != nullptr;
   if (i16)
-    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double), true),
-       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.J, x, y));
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), true),
+       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.bj.d_pat, ws.bj.bs, ws.J, x, y));
   else
-    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, sizeof(double), false),
-       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, ws.J, x, y));
+    KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), false),
+       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, nullptr, ws.bj.bs, ws.J, x, y));
 }
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
-  return rows * ((2 * kLU + 1) * tv + 2 * kLU * 2 + 4 + 2 + 2) + 2.0 * ws.n * tv;
+  const double lu_idx = ws.bj.npat ? (double)ws.bj.npat * ws.bj.bs * 2 * kLU * 2 : rows * 2 * kLU * 2;
+  return rows * ((2 * kLU + 1) * tv + 4 + 2 + 2) + lu_idx + 2.0 * ws.n * tv;
 }
 static double bytes_factor(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
@@ -534,7 +588,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   // partials and the launch also finishes that step's Arnoldi column
   auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp, const NormArgs *na) {
     const int W = ctx->jell_w;
-    const double by = bytes_spmv(ctx, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
+    const double by = bytes_spmv(ctx, ws, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
     const unsigned nb = ws.bj.nblocks;
     const NormArgs nv = na ? *na : NormArgs{};
 #define TDY_SPMV(TI, EC, NRM)                                                                                      \
