@@ -30,7 +30,7 @@ struct BJLayout {
   int nblocks = 0;
   int bs = 0;                  // stride / launch width (64, 256 or 512)
   int32_t *d_row = nullptr;    // nblocks*bs global row (-1 pad)
-  int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian (-1 pad), slot-major
+  int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian less the block's first position (-1 pad)
   uint16_t *d_lcol = nullptr;  // nblocks*kW*bs local column (0xffff pad)
   uint8_t *d_diag = nullptr;   // nblocks*bs slot of the diagonal
   uint8_t *d_nslot = nullptr;  // nblocks*bs number of slots
@@ -44,9 +44,9 @@ struct BJLayout {
   // 16 bits, pairs q and q+1 of slot p sharing the 32-bit word p*kPairW + q/2.
   uint8_t *d_npair = nullptr;  // nblocks*kLU*bs
   uint32_t *d_pair = nullptr;  // nblocks*kPair*bs (kPair = kLU*kPairW words per row)
-  // Shared index patterns: block b's L/U columns are pattern d_pat[b] of
-  // d_cL/d_cU (and its int16 column offsets pattern d_pat[b] of the solver's
-  // offsets); nullptr: every block has its own.
+  // Shared index patterns: every index table above except d_row (and the
+  // solver's int16 column offsets) holds one entry per pattern, block b using
+  // pattern d_pat[b]; nullptr: every block has its own.
   int32_t *d_pat = nullptr;    // nblocks
   int npat = 0;
 };
@@ -93,12 +93,15 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   __shared__ uint8_t dg[BS];
   __shared__ uint8_t ns[BS];
   const int b = blockIdx.x, tid = threadIdx.x;
-  const int64_t base = (int64_t)b * BS;
+  // index tables of the block's pattern (shared by identical blocks); values per block
+  const int pb = L.d_pat ? L.d_pat[b] : b;
+  const int64_t base = (int64_t)pb * BS;
+  const int64_t ibase = (int64_t)pb * kW * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
   for (int e = tid; e < kW * BS; e += NT) {
-    const int32_t pos = L.d_jpos[sbase + e];
-    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos] : (TV)0;
-    lc[e / BS][e % BS] = L.d_lcol[sbase + e];
+    const int32_t pos = L.d_jpos[ibase + e];  // relative to the block's first position
+    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos + (int64_t)b * BS] : (TV)0;
+    lc[e / BS][e % BS] = L.d_lcol[ibase + e];
   }
   for (int r = tid; r < BS; r += NT) {
     ns[r] = L.d_nslot[base + r];
@@ -106,14 +109,14 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   }
   const int t = tid / TPR, sub = tid % TPR;  // row of the block, lane within the row
   const int lev = L.d_levL[base + t];
-  const int nlev = L.d_nlev[2 * b];
+  const int nlev = L.d_nlev[2 * pb];
   const int nsl = L.d_nslot[base + t];
   const int di0 = L.d_diag[base + t];
   // The update lists of lower slot p+1 (count and this lane's packed pair
   // words) are fetched while slot p is eliminated, and slot 0's before the
   // level loop, so global-memory latency is off the elimination chain.
-  const int64_t pbase = (int64_t)b * kPair * BS;
-  const int64_t nbase = (int64_t)b * kLU * BS;
+  const int64_t pbase = (int64_t)pb * kPair * BS;
+  const int64_t nbase = (int64_t)pb * kLU * BS;
   auto fetch = [&](int p, int &cnt, uint32_t (&w)[WPL]) {
     cnt = (p < di0) ? L.d_npair[nbase + (int64_t)p * BS + t] : 0;
 #pragma unroll
@@ -194,9 +197,10 @@ __device__ inline void restrict_block(const TV *x, double *rc1, int b, int t) {
 // return.
 template <class TV, int BS>
 __device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, TV *x, int b, int t) {
-  const int64_t base = (int64_t)b * BS;
+  const int pb = L.d_pat ? L.d_pat[b] : b;  // index tables of the block's pattern
+  const int64_t base = (int64_t)pb * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
-  const int64_t cbase = (int64_t)(L.d_pat ? L.d_pat[b] : b) * kLU * BS;
+  const int64_t cbase = (int64_t)pb * kLU * BS;
   TV vl[kLU], vu[kLU];
   uint16_t cl[kLU], cu[kLU];
 #pragma unroll
@@ -208,7 +212,7 @@ __device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, 
   }
   const TV dinv = fac[sbase + (int64_t)(2 * kLU) * BS + t];
   const int levL = L.d_levL[base + t], levU = L.d_levU[base + t];
-  const int nL = L.d_nlev[2 * b], nU = L.d_nlev[2 * b + 1];
+  const int nL = L.d_nlev[2 * pb], nU = L.d_nlev[2 * pb + 1];
   __syncthreads();
   // forward: branch-free inner loop, all LDS reads of a level in flight together
   for (int l = 0; l < nL; l++) {

@@ -120,7 +120,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
         if (blk[c] != b) continue;
         if (ns >= kW) return "more than 27 in-block entries in a row";
         // position of (r, slot k-jrp[r]) in the block-ordered, slot-major Jacobian
-        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (int32_t)((k - jrp[r]) * NB + (int64_t)b * S + i);
+        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (int32_t)((k - jrp[r]) * NB + i);
         out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint16_t)local[c];
         if (c == r) out.diag[(size_t)b * S + i] = (uint8_t)ns;
         ns++;
@@ -214,14 +214,22 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     // brick's in-block columns and column offsets are those of every other
     // interior brick): keep one copy per pattern when that at least halves
     // them, so the matvec streams values, not indices.
-    const size_t lu = (size_t)kLU * S;
+    const size_t lu = (size_t)kLU * S, sw = (size_t)kW * S, pw = (size_t)kPair * S;
     std::unordered_map<std::string, int> seen;
     std::vector<int> rep;
     std::vector<int32_t> pat(nb);
     std::string key;
+    auto add = [&](const auto &v, size_t off, size_t n) {
+      key.append((const char *)(v.data() + off), n * sizeof(v[0]));
+    };
     for (int b = 0; b < nb; b++) {
-      key.assign((const char *)(out.cL.data() + (size_t)b * lu), lu * 2);
-      key.append((const char *)(out.cU.data() + (size_t)b * lu), lu * 2);
+      key.clear();
+      add(out.cL, b * lu, lu); add(out.cU, b * lu, lu);
+      add(out.jpos, b * sw, sw); add(out.lcol, b * sw, sw);
+      add(out.npair, b * lu, lu); add(out.pair, b * pw, pw);
+      add(out.diag, (size_t)b * S, S); add(out.nslot, (size_t)b * S, S);
+      add(out.levL, (size_t)b * S, S); add(out.levU, (size_t)b * S, S);
+      add(out.nlev, (size_t)2 * b, 2);
       for (int k = 0; k < W; k++) key.append((const char *)(out.ecol16.data() + (size_t)k * NB + (size_t)b * S), (size_t)S * 2);
       auto it = seen.find(key);
       if (it == seen.end()) {
@@ -232,18 +240,21 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     }
     const int npat = (int)rep.size();
     if (2 * npat <= nb) {
-      std::vector<uint16_t> cL(npat * lu), cU(npat * lu);
+      // keep the representative block of every pattern, in pattern order
+      auto shrink = [&](auto &v, size_t per) {
+        std::remove_reference_t<decltype(v)> w(npat * per);
+        for (int q = 0; q < npat; q++)
+          std::copy(v.begin() + (size_t)rep[q] * per, v.begin() + (size_t)(rep[q] + 1) * per, w.begin() + (size_t)q * per);
+        v.swap(w);
+      };
+      shrink(out.cL, lu); shrink(out.cU, lu); shrink(out.jpos, sw); shrink(out.lcol, sw);
+      shrink(out.npair, lu); shrink(out.pair, pw); shrink(out.diag, S); shrink(out.nslot, S);
+      shrink(out.levL, S); shrink(out.levU, S); shrink(out.nlev, 2);
       std::vector<int16_t> e16((size_t)npat * W * S);
-      for (int q = 0; q < npat; q++) {
-        const int b = rep[q];
-        std::copy(out.cL.begin() + (size_t)b * lu, out.cL.begin() + (size_t)(b + 1) * lu, cL.begin() + (size_t)q * lu);
-        std::copy(out.cU.begin() + (size_t)b * lu, out.cU.begin() + (size_t)(b + 1) * lu, cU.begin() + (size_t)q * lu);
+      for (int q = 0; q < npat; q++)
         for (int k = 0; k < W; k++)
-          std::copy(out.ecol16.begin() + (size_t)k * NB + (size_t)b * S,
-                    out.ecol16.begin() + (size_t)k * NB + (size_t)(b + 1) * S, e16.begin() + ((size_t)q * W + k) * S);
-      }
-      out.cL.swap(cL);
-      out.cU.swap(cU);
+          std::copy(out.ecol16.begin() + (size_t)k * NB + (size_t)rep[q] * S,
+                    out.ecol16.begin() + (size_t)k * NB + (size_t)(rep[q] + 1) * S, e16.begin() + ((size_t)q * W + k) * S);
       out.ecol16.swap(e16);
       out.pat.swap(pat);
       out.npat = npat;
@@ -518,12 +529,13 @@ static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, dou
 }
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
-  const double lu_idx = ws.bj.npat ? (double)ws.bj.npat * ws.bj.bs * 2 * kLU * 2 : rows * 2 * kLU * 2;
-  return rows * ((2 * kLU + 1) * tv + 4 + 2 + 2) + lu_idx + 2.0 * ws.n * tv;
+  const double idx_rows = ws.bj.npat ? (double)ws.bj.npat * ws.bj.bs : rows;
+  return rows * ((2 * kLU + 1) * tv + 4) + idx_rows * (2 * kLU * 2 + 2 + 2) + 2.0 * ws.n * tv;
 }
 static double bytes_factor(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
-  return rows * (kW * (4 + 2 + 8) + kLU + kPair * 4 + (2 * kLU + 1) * tv + 1 + 1 + 2);
+  const double idx_rows = ws.bj.npat ? (double)ws.bj.npat * ws.bj.bs : rows;
+  return rows * (kW * 8 + (2 * kLU + 1) * tv) + idx_rows * (kW * (4 + 2) + kLU + kPair * 4 + 1 + 1 + 2);
 }
 
 // Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
