@@ -226,9 +226,12 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
 // ell = true: J is the solver's block-ordered slot-major ELL array (row c at
 // perm[c], slot stride ell_stride), else CSR values in the pattern order.
 // Every entry of J is written (no prior clear is needed).
+static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
+                               double *R, double *J, hipStream_t st);
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
                        double *J, hipStream_t st, bool ell = false, const int32_t *perm = nullptr,
                        int64_t ell_stride = 0) {
+  if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st);
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
   if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
@@ -257,8 +260,8 @@ namespace tdyhip {
 
 static int no_flow(const tdyhip_ctx *ctx) {
   (void)ctx;
-  set_error("this mesh has cells other than hexahedra: only its transmissibility setup runs on the device "
-            "(the residual, Jacobian and Newton kernels take at most 8 columns per flux row)");
+  set_error("this mesh has cells other than hexahedra: its setup, residual and Jacobian run on the device, "
+            "the Newton/Krylov solver does not (its layouts take at most 27 Jacobian entries per row)");
   return TDYHIP_ERR_ARG;
 }
 
@@ -429,7 +432,75 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   if (status) { set_error("singular interaction-region system (zero LU pivot)"); return done(TDYHIP_ERR_SINGULAR); }
   ctx->h_face_verts.assign(m->face_verts, m->face_verts + (size_t)4 * nf);
   ctx->h_face_flags = g->face_flags;
+  // residual and Jacobian: per-cell data, state, flux rows, pattern
+  const int nbnd = m->nbnd;
+  if ((rc = dupload(ctx, &ctx->d_cell_faces, m->cell_faces, (size_t)6 * nc)) ||
+      (rc = dupload(ctx, &ctx->d_cell_vol, m->cell_vol, nc)) || (rc = dupload(ctx, &ctx->d_phi, mat->porosity, nc)) ||
+      (rc = dupload(ctx, &ctx->d_Sr, mat->Sr, nc)) || (rc = dupload(ctx, &ctx->d_bnd_cell, g->bnd_cell.data(), nbnd)) ||
+      (rc = dupload(ctx, &ctx->d_rtoff, g->rtoff.data(), g->rtoff.size())) ||
+      (rc = dupload(ctx, &ctx->d_rinfo, g->rinfo.data(), g->rinfo.size())) ||
+      (rc = dupload(ctx, &ctx->d_roc, g->roc.data(), g->roc.size())) ||
+      (rc = dupload(ctx, &ctx->d_rdc, g->rdc.data(), g->rdc.size())) ||
+      (rc = dupload(ctx, &ctx->d_ccell, g->ccell.data(), g->ccell.size())) ||
+      (rc = dupload(ctx, &ctx->d_vdcol, g->vdcol.data(), std::max<size_t>(g->vdcol.size(), 1))) ||
+      (rc = dupload(ctx, &ctx->d_jrp, g->jrp.data(), g->jrp.size())) ||
+      (rc = dupload(ctx, &ctx->d_jcols, g->jcols.data(), g->jcols.size())) ||
+      (rc = dupload(ctx, &ctx->d_jdiag, g->jdiag.data(), g->jdiag.size())) ||
+      (rc = dalloc(ctx, &ctx->d_Dgen, (size_t)std::max<int64_t>(g->tsize, 1))) ||
+      (rc = dalloc(ctx, &ctx->d_flux, (size_t)4 * nf)))
+    return done(rc);
+  if (mat->p_bnd_fn) {
+    ctx->has_pbnd_fn = true;
+    if ((rc = dupload(ctx, &ctx->d_pbnd_fn, mat->p_bnd_fn, nbnd))) return done(rc);
+  }
+  for (double **q : {&ctx->d_rho, &ctx->d_ukvr, &ctx->d_drho, &ctx->d_dukvr, &ctx->d_dM, &ctx->d_d2M, &ctx->d_acc,
+                     &ctx->d_accum_prev})
+    if ((rc = dalloc(ctx, q, nc))) return done(rc);
+  for (double **q : {&ctx->d_Pbnd, &ctx->d_ukvr_bnd, &ctx->d_rho_bnd})
+    if ((rc = dalloc(ctx, q, std::max(nbnd, 1)))) return done(rc);
+  {
+    double rb, d1, d2;  // boundary rho frozen at (Pref, Tref) (tdympfao.c:297-336)
+    water_density(ctx->density_type, ctx->pref, rb, d1, d2);
+    if (nbnd > 0) k_fill<<<nblk(nbnd, 256), 256, 0, st>>>(nbnd, ctx->d_rho_bnd, rb);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) { set_error("general setup failed"); return done(TDYHIP_ERR_HIP); }
+  ctx->jnnz = g->jrp[nc];
+  ctx->h_jrp = g->jrp;
+  ctx->h_jcols = g->jcols;
   return done(0);
+}
+
+// Residual / Jacobian of a general mesh (TS or SNES form): flux rows from the
+// vertex blocks, the residual by k_cell_gather, the Jacobian as CSR values in
+// the pattern order.
+static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
+                               double *R, double *J, hipStream_t st) {
+  GenRowArgs a;
+  a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
+  a.face_cells = ctx->d_face_cells; a.face_flags = ctx->d_face_flags;
+  a.rtoff = ctx->d_rtoff; a.rinfo = ctx->d_rinfo; a.roc = ctx->d_roc; a.rdc = ctx->d_rdc;
+  a.ccell = ctx->d_ccell; a.vdcol = ctx->d_vdcol;
+  a.T = ctx->d_Tgen; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd; a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr;
+  a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr; a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
+  a.flux = ctx->d_flux; a.D = ctx->d_Dgen;
+  const unsigned gr = nblk(a.nrows, 256);
+  if (R && J) k_rows_general<true, true><<<gr, 256, 0, st>>>(a);
+  else if (R) k_rows_general<true, false><<<gr, 256, 0, st>>>(a);
+  else k_rows_general<false, true><<<gr, 256, 0, st>>>(a);
+  if (R) {
+    GatherArgs ga = gather_args(ctx, form, shift, Udot, R, nullptr, false, nullptr, 0);
+    k_cell_gather<true, false><<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga);
+  }
+  if (J) {
+    GenGatherArgs gg;
+    gg.nc = ctx->nc; gg.form = form; gg.shift = shift; gg.dtime = ctx->dtime;
+    gg.cell_faces = ctx->d_cell_faces; gg.face_cells = ctx->d_face_cells; gg.rtoff = ctx->d_rtoff;
+    gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
+    gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags;
+    gg.D = ctx->d_Dgen; gg.dM = ctx->d_dM; gg.d2M = ctx->d_d2M; gg.Udot = Udot; gg.vol = ctx->d_cell_vol; gg.J = J;
+    k_gather_general_J<<<nblk(ctx->nc, 256), 256, 0, st>>>(gg);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }
 
 // Trans as CSR (rows face*4+subface, columns sorted: cells, then nc + boundary
@@ -685,7 +756,6 @@ int tdyhip_sizes(const tdyhip_ctx *ctx, int64_t *out) {
 }
 
 int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   std::memcpy(rowptr, ctx->h_jrp.data(), sizeof(int32_t) * (ctx->nc + 1));
   std::memcpy(cols, ctx->h_jcols.data(), sizeof(int32_t) * ctx->jnnz);
@@ -693,7 +763,6 @@ int tdyhip_jacobian_pattern(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *col
 }
 
 int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr, const int32_t **cols) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !rowptr || !cols) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   *rowptr = ctx->d_jrp;
   *cols = ctx->d_jcols;
@@ -701,7 +770,6 @@ int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr
 }
 
 int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R, tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -712,7 +780,6 @@ int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *U
 
 int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift, double *J,
                      tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -721,7 +788,6 @@ int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *U
 
 int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift,
                                double *R, double *J, tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   (void)t;
   if (!ctx || !U || !Udot || !R || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
@@ -737,7 +803,6 @@ int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt) {
 }
 
 int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !Uprev) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   StateParams sp = state_params(ctx, 1, 0.0);
@@ -748,7 +813,6 @@ int tdyhip_snes_presolve(tdyhip_ctx *ctx, const double *Uprev, tdyhip_stream s) 
 }
 
 int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   int rc = launch_state(ctx, 1, U, nullptr, st);
@@ -758,7 +822,6 @@ int tdyhip_snes_function(tdyhip_ctx *ctx, const double *U, double *R, tdyhip_str
 
 int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liquid_mass, double *water_mass,
                        tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   StateParams sp = state_params(ctx, 1, 0.0);
@@ -769,7 +832,6 @@ int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liqu
 }
 
 int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *J, tdyhip_stream s) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   if (!ctx || !U || !J) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   hipStream_t st; check_stream(ctx, s, &st);
   return launch_flux(ctx, 1, 0.0, U, nullptr, nullptr, J, st);

@@ -236,6 +236,133 @@ __global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- general (prism) meshes
+// Flux rows whose vertex has any number of columns: the row's T entries sit in
+// its vertex's block (row-major, columns = the region's cells, then its
+// Dirichlet faces), and its derivatives D in the same place for the cell
+// columns.  The physics is k_rows' (tdympfao_ts.c:12-150, 235-456).
+struct GenRowArgs {
+  int nc;
+  int64_t nrows;
+  double pref;
+  const int32_t *face_cells;
+  const uint8_t *face_flags;
+  const int32_t *rtoff, *rinfo, *roc, *rdc, *ccell, *vdcol;
+  const double *T, *G, *U, *Pbnd, *rho, *ukvr, *drho, *dukvr, *rho_bnd, *ukvr_bnd;
+  double *flux, *D;
+};
+
+template <bool WITH_R, bool WITH_J>
+__global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.nrows) return;
+  const int f = (int)(r >> 2);
+  const uint8_t fl = a.face_flags[f];
+  const int to = a.rtoff[r];
+  if (to < 0 || (fl & kFaceNoFlow)) {
+    if (WITH_R) a.flux[r] = 0.0;
+    return;
+  }
+  const int ncell = a.rinfo[r] & 0xff, ndir = (a.rinfo[r] >> 8) & 0xff;
+  const int32_t *cols = a.ccell + a.roc[r];
+  const int32_t *dcols = a.vdcol + a.rdc[r];
+  const double *Tr = a.T + to;
+  double TP = 0.0;
+  for (int j = 0; j < ncell; j++) TP += Tr[j] * a.U[cols[j]];
+  for (int j = 0; j < ndir; j++) TP += Tr[ncell + j] * a.Pbnd[dcols[j]];
+  const int up = a.face_cells[2 * f], dn = a.face_cells[2 * f + 1];
+  double den = 0.0;
+  den += up >= 0 ? a.rho[up] : a.rho_bnd[-up - 1];
+  den += dn >= 0 ? a.rho[dn] : a.rho_bnd[-dn - 1];
+  den *= 0.5;
+  const double G = a.G[r];
+  const bool upwind_up = (TP + den * G < 0.0);
+  const int uw = upwind_up ? up : dn;
+  double ukvr;
+  bool zero = false;
+  if (uw >= 0) {
+    ukvr = a.ukvr[uw];
+  } else {
+    ukvr = a.ukvr_bnd[-uw - 1];
+    if ((fl & kFaceSeepage) && a.Pbnd[-uw - 1] <= a.pref) zero = true;
+  }
+  if (WITH_R) {
+    double fluxm = 0.0;
+    if (!zero) {
+      fluxm = den * ukvr * (-TP);
+      fluxm += -(den * den) * ukvr * G;
+    }
+    a.flux[r] = fluxm;
+  }
+  if (WITH_J) {
+    const double dden_up = up >= 0 ? 0.5 * a.drho[up] : 0.0;
+    const double dden_dn = dn >= 0 ? 0.5 * a.drho[dn] : 0.0;
+    double dukvr_up = 0.0, dukvr_dn = 0.0;
+    if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
+    else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
+    for (int j = 0; j < ncell; j++) {
+      const int col = cols[j];
+      double Jac;
+      if (zero) Jac = 0.0;
+      else if (up >= 0 && col == up)
+        Jac = dden_up * ukvr * TP + den * dukvr_up * TP + den * ukvr * Tr[j] + 2 * den * dden_up * ukvr * G +
+              (den * den) * dukvr_up * G;
+      else if (dn >= 0 && col == dn)
+        Jac = dden_dn * ukvr * TP + den * dukvr_dn * TP + den * ukvr * Tr[j] + 2 * den * dden_dn * ukvr * G +
+              (den * den) * dukvr_dn * G;
+      else
+        Jac = den * ukvr * Tr[j];
+      a.D[to + j] = -Jac;  // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
+    }
+  }
+}
+
+// Jacobian row of each cell (CSR in the vertex-adjacency pattern): the
+// derivatives of its faces' flux rows in a fixed (face, sub-face, column)
+// order, then the accumulation diagonal.  One thread owns one row.
+struct GenGatherArgs {
+  int nc, form;
+  double shift, dtime;
+  const int32_t *cell_faces, *face_cells, *rtoff, *rinfo, *roc, *ccell, *jrp, *jcols, *jdiag;
+  const uint8_t *face_flags;
+  const double *D, *dM, *d2M, *Udot, *vol;
+  double *J;
+};
+
+__global__ void __launch_bounds__(256) k_gather_general_J(GenGatherArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nc) return;
+  const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
+  const int32_t *jc = a.jcols + r0;
+  double *row = a.J + r0;
+  for (int k = 0; k < len; k++) row[k] = 0.0;
+  for (int lf = 0; lf < 6; lf++) {
+    const int f = a.cell_faces[c * 6 + lf];
+    if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
+    const bool is_up = a.face_cells[2 * f] == c;
+    for (int q = 0; q < 4; q++) {
+      const int64_t r = (int64_t)f * 4 + q;
+      const int to = a.rtoff[r];
+      if (to < 0) continue;
+      const int ncell = a.rinfo[r] & 0xff;
+      const int32_t *cols = a.ccell + a.roc[r];
+      for (int j = 0; j < ncell; j++) {
+        const int col = cols[j];
+        int lo = 0, hi = len - 1;  // the column's slot in the sorted row
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (jc[mid] < col) lo = mid + 1; else hi = mid;
+        }
+        const double d = a.D[to + j];
+        row[lo] += is_up ? d : -d;
+      }
+    }
+  }
+  const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
+                                  : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
+  row[a.jdiag[c]] += dv;
+}
+
 // Per-cell gather table, built once at setup.  Row i = 4*lf + q of cell c
 // (its local face lf, sub-face q) has 8 bytes at gtab[i*nc + c]: byte k is
 // the slot of column k in c's Jacobian row, with bit 7 set when c is the
@@ -306,6 +433,7 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
 #pragma unroll
     for (int lf = 0; lf < 6; lf++) {
       const int f = a.cell_faces[c * 6 + lf];
+      if (f < 0) continue;  // a prism has 5 faces (a triangle's fourth flux row is zero)
       const double4 q = *reinterpret_cast<const double4 *>(a.flux + (int64_t)f * 4);
       const double fs = ((q.x + q.y) + q.z) + q.w;
       s += (a.face_cells[2 * f] == c) ? fs : -fs;

@@ -75,6 +75,12 @@ struct HostGeneral {
   std::vector<int32_t> bnd_cell;  // nbnd
   int64_t tsize = 0;              // T entries (sum of rows x columns)
   int max_cells = 0, max_faces = 0;
+  // flux rows (face*4 + q) of the residual and Jacobian kernels
+  std::vector<int32_t> rtoff;     // 4nf: offset of the row's T block row (-1: no row)
+  std::vector<int32_t> rinfo;     // 4nf: ncell | ndir << 8 of the row's vertex
+  std::vector<int32_t> roc, rdc;  // 4nf: offsets of the vertex's IR cells and Dirichlet columns
+  std::vector<int32_t> vdcol;     // per vertex, its Dirichlet columns' boundary indices (fidx order)
+  std::vector<int32_t> jrp, jcols, jdiag;  // vertex-adjacency Jacobian pattern
   int lanes[3] = {0, 0, 0};       // team width chosen for each bucket (0: empty bucket)
   float tune_ms[3][3] = {};       // tuning runs: bucket x width 16/32/64 (0: not run)
 };
@@ -165,6 +171,9 @@ struct tdyhip_ctx {
   // general (prism) meshes: transmissibility setup only, T as per-vertex blocks
   bool general = false;
   double *d_Tgen = nullptr;        // sum over vertices of rows x columns, row-major blocks
+  double *d_Dgen = nullptr;        // flux-row derivatives, same blocks (cell columns)
+  int32_t *d_rtoff = nullptr, *d_rinfo = nullptr, *d_roc = nullptr, *d_rdc = nullptr;  // 4nf each
+  int32_t *d_ccell = nullptr, *d_vdcol = nullptr, *d_jdiag = nullptr;
   tdyhip::HostGeneral *hgen = nullptr;
   double setup_ms[3] = {0, 0, 0};  // general setup: G matrices, bucketed local systems, gravity
 };

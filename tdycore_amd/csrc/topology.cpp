@@ -320,6 +320,11 @@ std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g) {
       }
   }
   g.vdesc.assign((size_t)nv * 8, 0);
+  g.rtoff.assign((size_t)4 * nf, -1);
+  g.rinfo.assign((size_t)4 * nf, 0);
+  g.roc.assign((size_t)4 * nf, 0);
+  g.rdc.assign((size_t)4 * nf, 0);
+  g.vdcol.clear();
   g.csub.assign(vc_n[nv], -1); g.ccell.assign(vc_n[nv], -1); g.csck.assign(vc_n[nv], 0);
   g.fa.assign(vf_n[nv], 0); g.fb.assign(vf_n[nv], 0); g.frow.assign(vf_n[nv], -1); g.fcol.assign(vf_n[nv], -1);
   for (int b = 0; b < 3; b++) g.bucket[b].clear();
@@ -373,11 +378,52 @@ std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g) {
     }
     int32_t *d = g.vdesc.data() + (size_t)v * 8;
     d[0] = ncell; d[1] = nface; d[2] = nU; d[3] = ndir; d[4] = oc; d[5] = of; d[6] = (int32_t)toff; d[7] = nrow;
+    // flux rows of this vertex: where their T rows and columns are
+    const int od = (int)g.vdcol.size();
+    g.vdcol.resize(od + ndir);
+    for (int k = 0; k < nface; k++)
+      if (role[k] == 2) g.vdcol[od + fidx[k]] = -fc[2 * vface[of + k]] - 1;
+    for (int k = 0; k < nface; k++) {
+      if (trow[k] < 0) continue;
+      const int r = g.frow[of + k];
+      g.rtoff[r] = (int32_t)(toff + (int64_t)trow[k] * (ncell + ndir));
+      g.rinfo[r] = ncell | (ndir << 8);
+      g.roc[r] = oc;
+      g.rdc[r] = od;
+    }
     if (toff + (int64_t)nrow * nB > INT32_MAX) return "transmissibility blocks exceed 2^31 entries";
     toff += (int64_t)nrow * nB;
     g.bucket[need <= 16 ? 0 : need <= 32 ? 1 : 2].push_back(v);
   }
   g.tsize = toff;
+  // Jacobian pattern: cells sharing a vertex, sorted (src/tdydiscretization.c:156-181)
+  g.jrp.assign(nc + 1, 0);
+  std::vector<int32_t> mark(nc, -1), tmp;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int c = 0; c < nc; c++) {
+      tmp.clear();
+      for (int s = 0; s < 8; s++) {
+        const int v = cv[c * 8 + s];
+        if (v < 0) continue;
+        for (int i = vc_n[v]; i < vc_n[v + 1]; i++)
+          if (mark[vcell[i]] != c) { mark[vcell[i]] = c; tmp.push_back(vcell[i]); }
+      }
+      if (pass == 0) g.jrp[c + 1] = (int32_t)tmp.size();
+      else {
+        std::sort(tmp.begin(), tmp.end());
+        std::copy(tmp.begin(), tmp.end(), g.jcols.begin() + g.jrp[c]);
+      }
+    }
+    if (pass == 0) {
+      for (int c = 0; c < nc; c++) g.jrp[c + 1] += g.jrp[c];
+      g.jcols.assign(g.jrp[nc], 0);
+      std::fill(mark.begin(), mark.end(), -1);
+    }
+  }
+  g.jdiag.assign(nc, -1);
+  for (int c = 0; c < nc; c++)
+    for (int k = g.jrp[c]; k < g.jrp[c + 1]; k++)
+      if (g.jcols[k] == c) g.jdiag[c] = k - g.jrp[c];
   return std::string();
 }
 

@@ -1,6 +1,6 @@
 """Right-prism meshes on the device: the size-bucketed interaction-region
-setup against the oracle (transmissibilities and gravity term within 1e-12
-of the largest entry of their row), and the flow calls refusing such a mesh."""
+setup, the residual and the Jacobian against the oracle (1e-12 of the
+largest entry of their row), and the Newton solver refusing such a mesh."""
 import numpy as np
 import pytest
 
@@ -62,15 +62,70 @@ def test_prism_setup_matches_oracle(name):
     g.close()
 
 
-def test_flow_calls_refuse_a_prism_mesh():
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet"])
+def test_prism_residual_and_jacobian_match_oracle(name):
+    """TS and SNES residuals and Jacobians on prisms, through the C ABI,
+    against the oracle at 1e-12 (residual: of its largest entry; Jacobian:
+    of each row's largest entry)."""
+    import torch
+    from pyoracle import Oracle
+    from tdycore_amd.mpfao import MPFAO
+    print(f"[case] prism residual/Jacobian {name}", flush=True)
+    P = _problem(name)
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    rp, cols = o.jac_pattern()
+    grp, gcols = g.jacobian_pattern()
+    assert np.array_equal(rp, grp) and np.array_equal(cols, gcols)
+    rng = np.random.default_rng(21)
+    nc = P["mesh"].nc
+
+    def close_rows(Jo, Jg):
+        live = rp[1:] > rp[:-1]
+        rowmax = np.maximum.reduceat(np.abs(Jo), rp[:-1][live])
+        scale = np.repeat(rowmax, (rp[1:] - rp[:-1])[live])
+        assert np.max(np.abs(Jg - Jo) / scale) <= TOL
+
+    for trial in range(2):
+        U = P["U0"] + 2e4 * (rng.random(nc) - 0.5)
+        Udot = 1e-2 * (rng.random(nc) - 0.5)
+        shift = 1.0 / 300.0
+        Ro, Jo = o.ifunction(U, Udot), o.ijacobian(U, Udot, shift)
+        dU, dUd = torch.tensor(U, device="cuda"), torch.tensor(Udot, device="cuda")
+        dR = torch.zeros(nc, dtype=torch.float64, device="cuda")
+        dJ = torch.zeros(g.jac_nnz, dtype=torch.float64, device="cuda")
+        g.IFunction(0.0, dU, dUd, dR)
+        g.IJacobian(0.0, dU, dUd, shift, dJ)
+        torch.cuda.synchronize()
+        assert np.abs(dR.cpu().numpy() - Ro).max() <= TOL * np.abs(Ro).max()
+        close_rows(Jo, dJ.cpu().numpy())
+    # SNES form
+    Uprev = P["U0"] + 2e4 * (rng.random(nc) - 0.5)
+    U = Uprev + 500.0 * (rng.random(nc) - 0.5)
+    o.set_dtime(1800.0)
+    o.snes_presolve(Uprev)
+    Ro, Jo = o.snes_function(U), o.snes_jacobian(U)
+    g.SetDtime(1800.0)
+    dR = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    dJ = torch.zeros(g.jac_nnz, dtype=torch.float64, device="cuda")
+    g.SNESPreSolve(torch.tensor(Uprev, device="cuda"))
+    dU = torch.tensor(U, device="cuda")
+    g.SNESFunction(dU, dR)
+    g.SNESJacobian(dU, dJ)
+    torch.cuda.synchronize()
+    assert np.abs(dR.cpu().numpy() - Ro).max() <= TOL * np.abs(Ro).max()
+    close_rows(Jo, dJ.cpu().numpy())
+    g.close()
+
+
+def test_newton_refuses_a_prism_mesh():
     import torch
     from tdycore_amd import _lib
     from tdycore_amd.mpfao import MPFAO
     P = _problem("delaunay")
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     U = torch.tensor(P["U0"], device="cuda")
-    R = torch.zeros_like(U)
     with pytest.raises(_lib.TdyHipError) as ei:
-        g.IFunction(0.0, U, torch.zeros_like(U), R)
+        g.run_snes_steps(U, 3600.0, 1)
     assert ei.value.code == 1 and "hexahedra" in str(ei.value)
     g.close()

@@ -69,3 +69,38 @@ def test_prism_linear_pressure_is_reproduced_exactly():
     # a wrong sub-face area (a quarter of a triangle) would fail
     wrong = m.face_area[f] / 4 * (m.face_normal[f] @ (Kc[0].reshape(3, 3) @ g))
     assert np.abs(TP - wrong).max() > 1e-3 * np.abs(exact).max()
+
+
+def test_prism_ts_jacobian_matches_finite_differences():
+    """On prisms, as on hexahedra: where the reference Jacobian is exact
+    (boundary pressures from a BC function, no upwind switch under the
+    perturbation) it is the derivative of the residual; the residual of a
+    closed no-flow box sums to the accumulation (the fluxes cancel)."""
+    rng = np.random.default_rng(17)
+    P, T = mg.delaunay_square(24, 50.0, rng)
+    m = mg.prism_mesh(P, T, np.array([0.0, 2.0, 3.5]))
+    K = mg.random_rotated_K(m.nc, rng)
+    pb = 9e4 + 2e3 * rng.random(m.nbnd)
+    o = Oracle(m, m.face_bc(default=mg.BC_DIRICHLET), K, rng.uniform(0.2, 0.4, m.nc), np.full(m.nc, 0.1),
+               vg_m=1 / 3, p_bnd_fn=pb)
+    U = 9.5e4 + 1.5e4 * rng.random(m.nc)
+    Uprev = U - 50.0 * rng.random(m.nc)
+    dt = 300.0
+
+    def R(X):
+        return o.ifunction(X, (X - Uprev) / dt)
+    R(U)
+    J = o.jac_to_csr(o.ijacobian(U, (U - Uprev) / dt, 1.0 / dt)).toarray()
+    h = 1e-7 * np.abs(U)
+    Jfd = np.empty_like(J)
+    for j in range(m.nc):
+        e = np.zeros(m.nc)
+        e[j] = h[j]
+        Jfd[:, j] = (R(U + e) - R(U - e)) / (2 * h[j])
+    scale = np.abs(J).max(axis=1, keepdims=True)
+    assert np.all(np.abs(J - Jfd) <= 1e-5 * scale)
+    # closed box: no-flow everywhere, the fluxes cancel in the sum of R
+    o2 = Oracle(m, m.face_bc(default=mg.BC_NEUMANN), K, np.full(m.nc, 0.3), np.full(m.nc, 0.1), vg_m=1 / 3)
+    Rs = o2.ifunction(U, np.zeros(m.nc))
+    Rf = o2.ifunction(U, np.zeros(m.nc))
+    assert np.allclose(Rs, Rf) and abs(Rs.sum()) <= 1e-10 * np.abs(Rs).max()
