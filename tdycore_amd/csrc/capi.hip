@@ -227,11 +227,11 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
 // perm[c], slot stride ell_stride), else CSR values in the pattern order.
 // Every entry of J is written (no prior clear is needed).
 static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
-                               double *R, double *J, hipStream_t st);
+                               double *R, double *J, hipStream_t st, const int32_t *perm, int64_t ell_stride);
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
                        double *J, hipStream_t st, bool ell = false, const int32_t *perm = nullptr,
                        int64_t ell_stride = 0) {
-  if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st);
+  if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st, ell ? perm : nullptr, ell_stride);
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
   if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
@@ -257,13 +257,6 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
 using namespace tdyhip;
 
 namespace tdyhip {
-
-static int no_flow(const tdyhip_ctx *ctx) {
-  (void)ctx;
-  set_error("this mesh has cells other than hexahedra: its setup, residual and Jacobian run on the device, "
-            "the Newton/Krylov solver does not (its layouts take at most 27 Jacobian entries per row)");
-  return TDYHIP_ERR_ARG;
-}
 
 template <int LANES>
 static int launch_ir_general(const GenArgs &ga, hipStream_t st) {
@@ -465,6 +458,8 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   }
   if (hipStreamSynchronize(st) != hipSuccess) { set_error("general setup failed"); return done(TDYHIP_ERR_HIP); }
   ctx->jnnz = g->jrp[nc];
+  ctx->jell_w = 0;
+  for (int c = 0; c < nc; c++) ctx->jell_w = std::max(ctx->jell_w, g->jrp[c + 1] - g->jrp[c]);
   ctx->h_jrp = g->jrp;
   ctx->h_jcols = g->jcols;
   return done(0);
@@ -474,7 +469,7 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
 // vertex blocks, the residual by k_cell_gather, the Jacobian as CSR values in
 // the pattern order.
 static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
-                               double *R, double *J, hipStream_t st) {
+                               double *R, double *J, hipStream_t st, const int32_t *perm, int64_t ell_stride) {
   GenRowArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
   a.face_cells = ctx->d_face_cells; a.face_flags = ctx->d_face_flags;
@@ -494,6 +489,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   if (J) {
     GenGatherArgs gg;
     gg.nc = ctx->nc; gg.form = form; gg.shift = shift; gg.dtime = ctx->dtime;
+    gg.ell_w = ctx->jell_w; gg.perm = perm; gg.stride = perm ? ell_stride : 1;
     gg.cell_faces = ctx->d_cell_faces; gg.face_cells = ctx->d_face_cells; gg.rtoff = ctx->d_rtoff;
     gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
     gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags;

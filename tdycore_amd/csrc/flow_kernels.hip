@@ -321,7 +321,9 @@ __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
 // derivatives of its faces' flux rows in a fixed (face, sub-face, column)
 // order, then the accumulation diagonal.  One thread owns one row.
 struct GenGatherArgs {
-  int nc, form;
+  int nc, form, ell_w;
+  int64_t stride;        // distance between a row's slots: 1 (CSR) or the ELL row count
+  const int32_t *perm;   // ELL: position of row c (nullptr: CSR at jrp[c])
   double shift, dtime;
   const int32_t *cell_faces, *face_cells, *rtoff, *rinfo, *roc, *ccell, *jrp, *jcols, *jdiag;
   const uint8_t *face_flags;
@@ -334,8 +336,10 @@ __global__ void __launch_bounds__(256) k_gather_general_J(GenGatherArgs a) {
   if (c >= a.nc) return;
   const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
   const int32_t *jc = a.jcols + r0;
-  double *row = a.J + r0;
-  for (int k = 0; k < len; k++) row[k] = 0.0;
+  const int64_t st = a.stride;
+  double *row = a.J + (a.perm ? (int64_t)a.perm[c] : (int64_t)r0);
+  const int nslot = a.perm ? a.ell_w : len;  // ELL rows are padded to ell_w slots with zeros
+  for (int k = 0; k < nslot; k++) row[k * st] = 0.0;
   for (int lf = 0; lf < 6; lf++) {
     const int f = a.cell_faces[c * 6 + lf];
     if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
@@ -354,13 +358,13 @@ __global__ void __launch_bounds__(256) k_gather_general_J(GenGatherArgs a) {
           if (jc[mid] < col) lo = mid + 1; else hi = mid;
         }
         const double d = a.D[to + j];
-        row[lo] += is_up ? d : -d;
+        row[lo * st] += is_up ? d : -d;
       }
     }
   }
   const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
                                   : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
-  row[a.jdiag[c]] += dv;
+  row[a.jdiag[c] * st] += dv;
 }
 
 // Per-cell gather table, built once at setup.  Row i = 4*lf + q of cell c

@@ -465,7 +465,8 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   ws.coarse_req = o.pc_coarse ? 1 : 0;
   if (o.pc_coarse) {
     HostCoarse hc;
-    if (build_coarse(hb, ctx->h_jrp, ctx->h_jcols, ctx->nc, ctx->jell_w, hc).empty()) {
+    // (the Galerkin row kernel handles up to 32 slots per row)
+    if (ctx->jell_w <= 32 && build_coarse(hb, ctx->h_jrp, ctx->h_jcols, ctx->nc, ctx->jell_w, hc).empty()) {
       CoarseLayout &C = ws.cl;
       C.nb = hc.nb; C.ng = hc.ng; C.gsz = hc.gsz;
       if ((rc = up(&C.nbr, hc.nbr)) || (rc = up(&C.cq, hc.cq)) || (rc = up(&C.grp, hc.grp)) ||
@@ -923,7 +924,6 @@ int tdyhip_kernel_timer_report(tdyhip_ctx *ctx, char *buf, int64_t buflen) {
 }
 
 int tdyhip_set_pc_partition(tdyhip_ctx *ctx, const int32_t *block_of_cell) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   using namespace tdyhip;
   if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
   if (block_of_cell) {
@@ -938,7 +938,6 @@ int tdyhip_set_pc_partition(tdyhip_ctx *ctx, const int32_t *block_of_cell) {
 }
 
 int tdyhip_newton_snes(tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, tdyhip_solver_stats *st) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   using namespace tdyhip;
   if (!ctx || !U || !st) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   int rc = check_opts(o);
@@ -952,7 +951,6 @@ int tdyhip_newton_snes(tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, 
 
 int tdyhip_run_snes_steps(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
                           tdyhip_solver_stats *st) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   using namespace tdyhip;
   if (!ctx || !U || !st || nsteps < 0) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
   int rc = check_opts(o);
@@ -972,7 +970,6 @@ int tdyhip_run_snes_steps(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps,
 
 int tdyhip_run_ts_be(tdyhip_ctx *ctx, double *U, double dt, int32_t nsteps, const tdyhip_solver_opts *o,
                      tdyhip_solver_stats *st) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   using namespace tdyhip;
   if (!ctx || !U || !st || nsteps < 0 || !(dt > 0)) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
   int rc = check_opts(o);
@@ -998,7 +995,6 @@ int tdyhip_cancel(tdyhip_ctx *ctx, int32_t on) {
 
 int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, double dt_max, double final_time,
                      const tdyhip_solver_opts *o, tdyhip_solver_stats *st) {
-  if (ctx && ctx->general) return tdyhip::no_flow(ctx);
   using namespace tdyhip;
   if (!ctx || !U || !st || !(dt_init > 0)) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
   int rc = check_opts(o);

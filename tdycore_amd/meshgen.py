@@ -479,3 +479,21 @@ def delaunay_square(n_tri: int, length: float, rng: np.random.Generator, jitter:
     X[inner] += rng.uniform(-jitter, jitter, size=(int(inner.sum()), 2)) * h
     tri = Delaunay(X)
     return X, tri.simplices.astype(np.int64)
+
+
+def column_partition(mesh: Mesh, per_block: int, grid: int = 256) -> np.ndarray:
+    """Block id per cell of a prism mesh (cell = t + nt*k): the vertical
+    columns of per_block triangles that are neighbours in a grid x grid
+    ordering of their centroids, so a block holds per_block * nlayers cells
+    and its rows couple through at most a few in-block neighbours."""
+    nt = mesh.nx
+    cen = mesh.cell_cen[:nt, :2]
+    lo, hi = cen.min(axis=0), cen.max(axis=0)
+    ij = np.clip(((cen - lo) / np.maximum(hi - lo, 1e-300) * grid).astype(np.int64), 0, grid - 1)
+    # serpentine row order keeps consecutive triangles close
+    row = ij[:, 1]
+    col = np.where(row % 2 == 0, ij[:, 0], grid - 1 - ij[:, 0])
+    order = np.lexsort((col, row))
+    blk_of_tri = np.empty(nt, dtype=np.int32)
+    blk_of_tri[order] = np.arange(nt) // per_block
+    return blk_of_tri[np.arange(mesh.nc) % nt].astype(np.int32)

@@ -82,7 +82,7 @@ def ensemble_request_specs(n=512, seed=20261206):
 
 
 def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lattice=128, gmatrix="mpfao",
-                  points=None, triangles=None):
+                  points=None, triangles=None, infiltration=False):
     """Right prisms: a random Delaunay triangulation of about n_tri triangles
     on a length x length square (jittered lattice points, scipy's Delaunay),
     extruded through nlayers layers of thickness U(0.5, 5) m.  Permeability as
@@ -92,7 +92,11 @@ def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lat
     van Genuchten n = 1.5 (m = 1/3), alpha 1e-4; hydrostatic initial state
     with the water table at mid-height; Dirichlet top and bottom at the
     hydrostatic values, no-flow sides.  ``points``/``triangles`` replace the
-    triangulation (then n_tri is unused)."""
+    triangulation (then n_tri is unused).  On right prisms the hydrostatic
+    state is an exact discrete equilibrium (flat layers, vertical walls), so
+    with ``infiltration`` the top boundary is held at atmospheric pressure
+    instead (ponded infiltration into the unsaturated upper half), which makes
+    every time step a genuine Newton solve."""
     rng = np.random.default_rng(seed)
     if points is None:
         points, triangles = mg.delaunay_square(n_tri, length, rng)
@@ -112,6 +116,9 @@ def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lat
     fbc = m.face_bc(dirichlet=["z-", "z+"], neumann=["x-"])
     zt = 0.5 * Z[-1]
     pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], zt))
+    if infiltration:
+        top = m.face_cen[m.bnd_face, 2] >= Z[-1] - 1e-9
+        pb[top] = 101325.0
     U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
     kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix=gmatrix, density="constant", p_bnd_fn=pb)
     return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)

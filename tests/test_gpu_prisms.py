@@ -1,6 +1,6 @@
 """Right-prism meshes on the device: the size-bucketed interaction-region
 setup, the residual and the Jacobian against the oracle (1e-12 of the
-largest entry of their row), and the Newton solver refusing such a mesh."""
+largest entry of their row), and Newton steps against the oracle's."""
 import numpy as np
 import pytest
 
@@ -118,14 +118,26 @@ def test_prism_residual_and_jacobian_match_oracle(name):
     g.close()
 
 
-def test_newton_refuses_a_prism_mesh():
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_prism_newton_steps_match_oracle(fp32):
+    """Backward-Euler SNES steps on a prism mesh: device Newton with GMRES and
+    block-Jacobi ILU(0) on vertical-column blocks, against the oracle's Newton
+    with direct solves (1e-8 relative pressure)."""
     import torch
-    from tdycore_amd import _lib
+    from pyoracle import Oracle
     from tdycore_amd.mpfao import MPFAO
-    P = _problem("delaunay")
-    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
-    U = torch.tensor(P["U0"], device="cuda")
-    with pytest.raises(_lib.TdyHipError) as ei:
-        g.run_snes_steps(U, 3600.0, 1)
-    assert ei.value.code == 1 and "hexahedra" in str(ei.value)
+    print(f"[case] prism Newton fp32={fp32}", flush=True)
+    P = workloads.prism_problem(n_tri=150, nlayers=3, seed=31, length=300.0, lattice=8, infiltration=True)
+    m = P["mesh"]
+    o = Oracle(m, P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    Uo, st_o = o.run_snes_steps(P["U0"], 3600.0, 2, lin="lu", rtol=1e-10)
+    assert st_o[3] == 0
+    g = MPFAO(m, P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    g.set_pc_partition(mg.column_partition(m, 4))
+    U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
+    st = g.run_snes_steps(U, 3600.0, 2, g.solver_options(snes_rtol=1e-10, ksp_rtol=1e-8, pc_block=64,
+                                                         krylov_fp32=fp32))
+    assert st["status"] == 0
+    Ug = U.cpu().numpy()
+    assert np.max(np.abs(Ug - Uo) / np.abs(Uo)) <= 1e-8
     g.close()
