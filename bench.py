@@ -262,14 +262,23 @@ def prisms(a):
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         from pyoracle import Oracle
-        S = workloads.prism_problem(n_tri=20_000, nlayers=a.prism_layers)
+        S = workloads.prism_problem(n_tri=20_000, nlayers=a.prism_layers, infiltration=True)
         t = time.perf_counter()
-        Oracle(S["mesh"], S["face_bc"], S["K"], S["porosity"], S["Sr"], **S["kw"])
+        o = Oracle(S["mesh"], S["face_bc"], S["K"], S["porosity"], S["Sr"], **S["kw"])
+        el_setup = time.perf_counter() - t
+        Uc = S["U0"].copy()
+        o.set_dtime(3600.0)
+        o.snes_presolve(Uc)
+        o.set_newton_maxit(1)
+        t = time.perf_counter()
+        _, sto, _ = o.newton_snes(Uc, lin="gmres", rtol=1e-10)
         el = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": S["mesh"].nv / el, "unit": "regions/s", "cores": 1, "kind": "port",
-                                "sample": f"oracle setup (G, local systems, gravity, Jacobian pattern) of a "
-                                          f"20,000-triangle x {a.prism_layers}-layer prism mesh "
-                                          f"({S['mesh'].nv} vertices): {el:.2f} s"}
+        ncs = S["mesh"].nc
+        line["cpu_baseline"] = {"value": ncs * int(sto[0]) / el, "unit": "Newton cells/s", "cores": 1, "kind": "port",
+                                "sample": f"1 Newton iteration (residual, Jacobian, ILU(0), GMRES(30) {int(sto[1])} its, "
+                                          f"line search) of step 1 on a 20,000-triangle x {a.prism_layers}-layer prism "
+                                          f"mesh ({ncs} cells): {el:.2f} s",
+                                "setup_regions_per_s": S["mesh"].nv / el_setup}
     print(json.dumps(line), flush=True)
 
 

@@ -460,6 +460,7 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   ctx->jnnz = g->jrp[nc];
   ctx->jell_w = 0;
   for (int c = 0; c < nc; c++) ctx->jell_w = std::max(ctx->jell_w, g->jrp[c + 1] - g->jrp[c]);
+  if (ctx->jell_w > kGenJSlots) { set_error("more than 64 Jacobian entries in a row"); return done(TDYHIP_ERR_ARG); }
   ctx->h_jrp = g->jrp;
   ctx->h_jcols = g->jcols;
   return done(0);
@@ -494,7 +495,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
     gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags;
     gg.D = ctx->d_Dgen; gg.dM = ctx->d_dM; gg.d2M = ctx->d_d2M; gg.Udot = Udot; gg.vol = ctx->d_cell_vol; gg.J = J;
-    k_gather_general_J<<<nblk(ctx->nc, 256), 256, 0, st>>>(gg);
+    k_gather_general_J<<<nblk(ctx->nc, kGenJBlock), kGenJBlock, 0, st>>>(gg);
   }
   return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }

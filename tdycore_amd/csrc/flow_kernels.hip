@@ -317,9 +317,13 @@ __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
   }
 }
 
-// Jacobian row of each cell (CSR in the vertex-adjacency pattern): the
-// derivatives of its faces' flux rows in a fixed (face, sub-face, column)
-// order, then the accumulation diagonal.  One thread owns one row.
+// Jacobian row of each cell (CSR in the vertex-adjacency pattern, or the
+// solver's ELL): the derivatives of its faces' flux rows in a fixed (face,
+// sub-face, column) order, then the accumulation diagonal.  One thread owns
+// one row; the row is summed in LDS (up to kGenJSlots entries) and every
+// slot written once.
+constexpr int kGenJBlock = 64;
+constexpr int kGenJSlots = 64;
 struct GenGatherArgs {
   int nc, form, ell_w;
   int64_t stride;        // distance between a row's slots: 1 (CSR) or the ELL row count
@@ -331,15 +335,14 @@ struct GenGatherArgs {
   double *J;
 };
 
-__global__ void __launch_bounds__(256) k_gather_general_J(GenGatherArgs a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a) {
+  __shared__ double acc[kGenJSlots][kGenJBlock];
+  const int t = threadIdx.x;
+  const int c = blockIdx.x * kGenJBlock + t;
   if (c >= a.nc) return;
   const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
   const int32_t *jc = a.jcols + r0;
-  const int64_t st = a.stride;
-  double *row = a.J + (a.perm ? (int64_t)a.perm[c] : (int64_t)r0);
-  const int nslot = a.perm ? a.ell_w : len;  // ELL rows are padded to ell_w slots with zeros
-  for (int k = 0; k < nslot; k++) row[k * st] = 0.0;
+  for (int k = 0; k < len; k++) acc[k][t] = 0.0;
   for (int lf = 0; lf < 6; lf++) {
     const int f = a.cell_faces[c * 6 + lf];
     if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
@@ -358,13 +361,17 @@ __global__ void __launch_bounds__(256) k_gather_general_J(GenGatherArgs a) {
           if (jc[mid] < col) lo = mid + 1; else hi = mid;
         }
         const double d = a.D[to + j];
-        row[lo * st] += is_up ? d : -d;
+        acc[lo][t] += is_up ? d : -d;
       }
     }
   }
   const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
                                   : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
-  row[a.jdiag[c] * st] += dv;
+  acc[a.jdiag[c]][t] += dv;
+  const int64_t st = a.stride;
+  double *row = a.J + (a.perm ? (int64_t)a.perm[c] : (int64_t)r0);
+  const int nslot = a.perm ? a.ell_w : len;  // ELL rows are padded to ell_w slots with zeros
+  for (int k = 0; k < nslot; k++) row[k * st] = k < len ? acc[k][t] : 0.0;
 }
 
 // Per-cell gather table, built once at setup.  Row i = 4*lf + q of cell c
