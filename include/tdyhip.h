@@ -39,8 +39,8 @@ enum { TDYHIP_BC_INTERNAL = -1, TDYHIP_BC_DIRICHLET = 0, TDYHIP_BC_NEUMANN = 1, 
 
 /* Mesh in the form TDyMeshCreateFromPlex produces (src/fv/share/tdymeshplex.c:3685):
  * hexahedra, 8 vertices per cell, 6 faces per cell, 4 vertices per face (right prisms:
- * 6 vertices, 5 faces and triangles of 3 vertices, each array padded with -1; such a mesh
- * gets the transmissibility setup only, see tdyhip_setup_info)
+ * 6 vertices, 5 faces and triangles of 3 vertices, each array padded with -1; see
+ * tdyhip_setup_info)
  * (cyclic), face_cells = (up, dn) with the unit normal from up to dn and a
  * boundary face stored as (-(b+1), cell).  Host pointers. */
 typedef struct {
@@ -197,9 +197,9 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
 /* Setup of a general mesh (any cell other than a hexahedron: right prisms, given in the
  * hexahedral arrays padded with -1).  Replaces ComputeTransmissibilityMatrix and
  * ComputeGravityDiscretization for vertices of any degree (src/fv/mpfao/tdympfao.c:1801-1860,
- * 1126-1513, 2169-2274); tdyhip_create runs it, and only the transmissibility and gravity
- * exports above (and tdyhip_setup_info) are available on such a context: the flow calls
- * return TDYHIP_ERR_ARG.  ms[3] = device times of the G matrices, the size-bucketed local
+ * 1126-1513, 2169-2274); tdyhip_create runs it.  Every other call works on such a context
+ * (rows of up to 64 Jacobian entries); its Newton solver needs a block-Jacobi partition with
+ * at most 27 in-block entries per row (tdyhip_set_pc_partition, e.g. vertical columns).  ms[3] = device times of the G matrices, the size-bucketed local
  * systems and the gravity term (HIP events, general meshes only); counts[11] = interaction
  * regions in the 16-, 32- and 64-lane buckets, the most cells and faces at one vertex, the
  * transmissibility entries, the algorithmic FP64 flops and bytes of the local systems, and
