@@ -268,6 +268,7 @@ __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
   const int32_t *dcols = a.vdcol + a.rdc[r];
   const double *Tr = a.T + to;
   double TP = 0.0;
+#pragma unroll 4
   for (int j = 0; j < ncell; j++) TP += Tr[j] * a.U[cols[j]];
   for (int j = 0; j < ndir; j++) TP += Tr[ncell + j] * a.Pbnd[dcols[j]];
   const int up = a.face_cells[2 * f], dn = a.face_cells[2 * f + 1];
@@ -300,6 +301,7 @@ __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
     double dukvr_up = 0.0, dukvr_dn = 0.0;
     if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
     else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
+#pragma unroll 4
     for (int j = 0; j < ncell; j++) {
       const int col = cols[j];
       double Jac;
@@ -337,12 +339,15 @@ struct GenGatherArgs {
 
 __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a) {
   __shared__ double acc[kGenJSlots][kGenJBlock];
+  __shared__ int32_t jcs[kGenJSlots][kGenJBlock];  // the row's sorted columns, searched in LDS
   const int t = threadIdx.x;
   const int c = blockIdx.x * kGenJBlock + t;
   if (c >= a.nc) return;
   const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
-  const int32_t *jc = a.jcols + r0;
-  for (int k = 0; k < len; k++) acc[k][t] = 0.0;
+  for (int k = 0; k < len; k++) {
+    acc[k][t] = 0.0;
+    jcs[k][t] = a.jcols[r0 + k];
+  }
   for (int lf = 0; lf < 6; lf++) {
     const int f = a.cell_faces[c * 6 + lf];
     if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
@@ -358,7 +363,7 @@ __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a
         int lo = 0, hi = len - 1;  // the column's slot in the sorted row
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
-          if (jc[mid] < col) lo = mid + 1; else hi = mid;
+          if (jcs[mid][t] < col) lo = mid + 1; else hi = mid;
         }
         const double d = a.D[to + j];
         acc[lo][t] += is_up ? d : -d;
