@@ -37,6 +37,8 @@ struct BJLayout {
   uint16_t *d_levL = nullptr;  // nblocks*bs
   uint16_t *d_levU = nullptr;  // nblocks*bs
   int32_t *d_nlev = nullptr;   // nblocks*2 (L, U level counts)
+  int32_t *d_nlu = nullptr;    // nblocks: L/U slots in use (the factor writes and the sweeps read only
+                               // these); nullptr when every block uses all kLU
   uint16_t *d_cL = nullptr;    // nblocks*kLU*bs local column of L slot (bs = zero entry)
   uint16_t *d_cU = nullptr;    // nblocks*kLU*bs local column of U slot
   // ILU(0) update lists: for row i and lower slot p (row k), pairs (d, s) with
@@ -159,7 +161,8 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   }
   // split layout: L multipliers, U entries, inverse diagonal
   const int di = dg[t];
-  for (int p = sub; p < kLU; p += TPR) {
+  const int nlu = L.d_nlu ? L.d_nlu[pb] : kLU;
+  for (int p = sub; p < nlu; p += TPR) {
     fac[sbase + (int64_t)p * BS + t] = (p < di) ? a[p][t] : (TV)0;
     const int q = di + 1 + p;
     fac[sbase + (int64_t)(kLU + p) * BS + t] = (q < nsl) ? a[q][t] : (TV)0;
@@ -203,12 +206,14 @@ __device__ inline void bj_sweeps(const BJLayout &L, const TV *__restrict__ fac, 
   const int64_t cbase = (int64_t)pb * kLU * BS;
   TV vl[kLU], vu[kLU];
   uint16_t cl[kLU], cu[kLU];
+  const int nlu = L.d_nlu ? L.d_nlu[pb] : kLU;  // slots past it hold 0 against the zero entry x[BS]
 #pragma unroll
   for (int p = 0; p < kLU; p++) {
-    vl[p] = fac[sbase + (int64_t)p * BS + t];
-    vu[p] = fac[sbase + (int64_t)(kLU + p) * BS + t];
-    cl[p] = L.d_cL[cbase + (int64_t)p * BS + t];
-    cu[p] = L.d_cU[cbase + (int64_t)p * BS + t];
+    const bool in = p < nlu;
+    vl[p] = in ? fac[sbase + (int64_t)p * BS + t] : (TV)0;
+    vu[p] = in ? fac[sbase + (int64_t)(kLU + p) * BS + t] : (TV)0;
+    cl[p] = in ? L.d_cL[cbase + (int64_t)p * BS + t] : (uint16_t)BS;
+    cu[p] = in ? L.d_cU[cbase + (int64_t)p * BS + t] : (uint16_t)BS;
   }
   const TV dinv = fac[sbase + (int64_t)(2 * kLU) * BS + t];
   const int levL = L.d_levL[base + t], levU = L.d_levU[base + t];

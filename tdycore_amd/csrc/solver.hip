@@ -23,6 +23,7 @@ struct HostBJ {
   std::vector<uint32_t> pair;
   std::vector<uint8_t> npair;
   std::vector<int32_t> nlev;
+  std::vector<int32_t> nlu;   // nblocks: most strictly lower (or upper) in-block entries of a row
   std::vector<int32_t> perm;  // nc: block-layout position of each cell
   std::vector<int32_t> ecol;  // W*nblocks*bs: global column of slot k at block position p (k*NB + p)
   std::vector<int16_t> ecol16;  // the same as offsets from the row, when every offset fits in int16
@@ -106,6 +107,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   out.npair.assign((size_t)nb * kLU * S, 0);
   out.pair.assign((size_t)nb * kPair * S, 0);
   out.nlev.assign((size_t)nb * 2, 0);
+  out.nlu.assign((size_t)nb, 0);
   for (int b = 0; b < nb; b++) {
     const auto &mem = members[b];
     const int m = (int)mem.size();
@@ -154,6 +156,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
     for (int i = 0; i < m; i++) {
       const int di = out.diag[(size_t)b * S + i], ns = out.nslot[(size_t)b * S + i];
       if (di > kLU || ns - di - 1 > kLU) return "more than 13 lower or upper in-block entries in a row";
+      out.nlu[b] = std::max(out.nlu[b], std::max(di, ns - di - 1));
       // ILU(0) update pairs: row i slot d -= l_ik * row k slot s, for the entries of row k
       // after its diagonal whose column is in row i's pattern
       for (int p = 0; p < di; p++) {
@@ -230,6 +233,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
       add(out.diag, (size_t)b * S, S); add(out.nslot, (size_t)b * S, S);
       add(out.levL, (size_t)b * S, S); add(out.levU, (size_t)b * S, S);
       add(out.nlev, (size_t)2 * b, 2);
+      add(out.nlu, (size_t)b, 1);
       for (int k = 0; k < W; k++) key.append((const char *)(out.ecol16.data() + (size_t)k * NB + (size_t)b * S), (size_t)S * 2);
       auto it = seen.find(key);
       if (it == seen.end()) {
@@ -249,7 +253,7 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
       };
       shrink(out.cL, lu); shrink(out.cU, lu); shrink(out.jpos, sw); shrink(out.lcol, sw);
       shrink(out.npair, lu); shrink(out.pair, pw); shrink(out.diag, S); shrink(out.nslot, S);
-      shrink(out.levL, S); shrink(out.levU, S); shrink(out.nlev, 2);
+      shrink(out.levL, S); shrink(out.levU, S); shrink(out.nlev, 2); shrink(out.nlu, 1);
       std::vector<int16_t> e16((size_t)npat * W * S);
       for (int q = 0; q < npat; q++)
         for (int k = 0; k < W; k++)
@@ -369,6 +373,7 @@ struct SolverWS {
   // coarse correction (when requested and the mesh is within its limits)
   bool coarse = false;
   int coarse_req = -1;
+  double lu_mean = 0.0;          // mean per-block L/U slot count the sweeps read
   CoarseLayout cl;
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
@@ -435,6 +440,14 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if ((rc = up(&ws.d_perm, hb.perm))) return rc;
   ws.bj.d_pat = nullptr;
   ws.bj.npat = hb.npat;
+  ws.bj.d_nlu = nullptr;
+  if (std::any_of(hb.nlu.begin(), hb.nlu.end(), [](int32_t v) { return v < kLU; }) && (rc = up(&ws.bj.d_nlu, hb.nlu)))
+    return rc;
+  {
+    double sum = 0.0;  // mean stored L/U width over the blocks (byte model)
+    for (size_t b = 0; b < (size_t)hb.nblocks; b++) sum += hb.nlu[hb.pat.empty() ? b : (size_t)hb.pat[b]];
+    ws.lu_mean = hb.nblocks ? sum / hb.nblocks : 0.0;
+  }
   if (!hb.pat.empty() && (rc = up(&ws.bj.d_pat, hb.pat))) return rc;
   if (!hb.ecol16.empty()) {
     if ((rc = up(&ws.d_ecol16, hb.ecol16))) return rc;
@@ -531,7 +544,7 @@ static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, dou
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
   const double idx_rows = ws.bj.npat ? (double)ws.bj.npat * ws.bj.bs : rows;
-  return rows * ((2 * kLU + 1) * tv + 4) + idx_rows * (2 * kLU * 2 + 2 + 2) + 2.0 * ws.n * tv;
+  return rows * ((2 * ws.lu_mean + 1) * tv + 4) + idx_rows * (2 * ws.lu_mean * 2 + 2 + 2) + 2.0 * ws.n * tv;
 }
 static double bytes_factor(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
