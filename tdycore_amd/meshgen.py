@@ -497,3 +497,58 @@ def column_partition(mesh: Mesh, per_block: int, grid: int = 256) -> np.ndarray:
     blk_of_tri = np.empty(nt, dtype=np.int32)
     blk_of_tri[order] = np.arange(nt) // per_block
     return blk_of_tri[np.arange(mesh.nc) % nt].astype(np.int32)
+
+
+def polyhedral_mesh(V: np.ndarray, cell_verts: np.ndarray, cell_faces: np.ndarray, face_verts: np.ndarray,
+                    face_cells: np.ndarray) -> Mesh:
+    """Geometry and boundary bookkeeping of a mesh of hexahedra and prisms
+    given by connectivity (arrays padded with -1; face_cells = (c0, c1) with
+    c1 = -1 on the boundary).  Faces: triangles exact, quadrilaterals by the
+    four-triangle fan of ``box_mesh``; cells: volume and centroid by fan
+    tetrahedra about the vertex mean (exact for planar faces); boundary faces
+    stored as (-(b+1), cell), normals from up to dn."""
+    V = np.asarray(V, dtype=np.float64)
+    nc, nf = cell_verts.shape[0], face_verts.shape[0]
+    nvf = (face_verts >= 0).sum(axis=1)
+    fnrm = np.zeros((nf, 3)); farea = np.zeros(nf); fcen = np.zeros((nf, 3))
+    fsvec = np.zeros((nf, 3)); fmean = np.zeros((nf, 3))
+    q = nvf == 4
+    if q.any():
+        n4, a4, c4, s4, m4 = _face_geometry(V[face_verts[q]])
+        fnrm[q], farea[q], fcen[q], fsvec[q], fmean[q] = n4, a4, c4, s4, m4
+    t = nvf == 3
+    if t.any():
+        X = V[face_verts[t, :3]]
+        cr = _cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+        ln = np.sqrt((cr * cr).sum(axis=1))
+        fnrm[t] = cr / ln[:, None]; farea[t] = 0.5 * ln; fcen[t] = X.mean(axis=1)
+        fsvec[t] = 0.5 * cr; fmean[t] = fcen[t]
+    vol = np.zeros(nc); cen = np.zeros((nc, 3))
+    for c in range(nc):
+        vs = cell_verts[c][cell_verts[c] >= 0]
+        p0 = V[vs].mean(axis=0)
+        for f in cell_faces[c][cell_faces[c] >= 0]:
+            fv = face_verts[f][face_verts[f] >= 0]
+            mf = fmean[f]
+            sgn = np.sign(fsvec[f] @ (mf - p0))
+            for i in range(len(fv)):
+                a, b = V[fv[i]], V[fv[(i + 1) % len(fv)]]
+                tv = sgn * ((mf - p0) @ _cross(a - p0, b - p0)) / 6.0
+                vol[c] += tv
+                cen[c] += tv * (p0 + mf + a + b) / 4.0
+        cen[c] /= vol[c]
+    fc = np.array(face_cells, dtype=np.int32).copy()
+    bmask = fc[:, 1] < 0
+    bnd_face = np.nonzero(bmask)[0].astype(np.int32)
+    fc[bnd_face, 1] = fc[bnd_face, 0]
+    fc[bnd_face, 0] = -(np.arange(bnd_face.shape[0], dtype=np.int32) + 1)
+    up, dn = fc[:, 0], fc[:, 1]
+    ref_from = np.where(up[:, None] >= 0, cen[np.maximum(up, 0)], fcen)
+    s = np.sign(((cen[dn] - ref_from) * fnrm).sum(axis=1))
+    s[s == 0] = 1.0
+    fnrm = fnrm * s[:, None]
+    face_side = np.where(bmask, 0, -1).astype(np.int32)
+    return Mesh(nx=nc, ny=1, nz=1, vert_xyz=V, cell_verts=cell_verts.astype(np.int32),
+                cell_faces=cell_faces.astype(np.int32), face_verts=face_verts.astype(np.int32), face_cells=fc,
+                cell_cen=cen, cell_vol=vol, face_cen=fcen, face_area=farea, face_normal=fnrm, bnd_face=bnd_face,
+                face_dir=np.zeros(nf, dtype=np.int32), face_side=face_side)

@@ -4,6 +4,7 @@ import os
 
 import numpy as np
 
+from tdycore_amd import exodus
 from tdycore_amd import meshgen as mg
 from tdycore_amd import workloads
 
@@ -20,7 +21,10 @@ def top110k(x):
 
 def gold_problem(case):
     """-> dict(mesh, face_bc, K, porosity, Sr, kwargs, U0) for a reference gold case."""
-    m = mg.box_mesh(*case["mesh"])
+    if "mesh_file" in case:
+        m = exodus.exodus_mesh(os.path.join(HERE, "golden", case["mesh_file"]))
+    else:
+        m = mg.box_mesh(*case["mesh"])
     fbc = m.face_bc(default=BC[case["bc"]])
     pb = None
     if case["bc_fn"] == "const1e5":

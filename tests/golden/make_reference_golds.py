@@ -5,6 +5,7 @@ container where the reference tree is mounted; the JSON it writes is data
 (expected outputs plus the case definitions taken from the .cfg files)."""
 import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -61,6 +62,11 @@ CASES = {
         gold="demo/transient/transient-snes-mpfaof90-seepage.regression.gold",
         form="snes", mesh=[5, 5, 5], bc="seepage", gmatrix="tpf", density="exponential",
         K=1e-10, porosity=0.115, Sr=0.115, ic=90325.0, dt=1800.0, nsteps=48, bc_fn="top110k", nsample=2),
+    # prisms read from the reference's Exodus mesh (copied beside this script as data)
+    "transient-snes-mpfaof90-wedge": dict(
+        gold="demo/transient/transient-snes-mpfaof90-wedge.regression.gold",
+        form="snes", mesh_file="2x2x2_wedge_mesh.exo", bc="dirichlet", gmatrix="tpf", density="exponential",
+        K=1e-10, porosity=0.115, Sr=0.115, ic=102325.0, dt=1800.0, nsteps=3, bc_fn="top110k", nsample=10),
     "richards-driver-snes-prob1": dict(
         gold="demo/richards/richards-driver-snes-prob1.regression.gold",
         form="tdyti", mesh=[2, 2, 2], bc="dirichlet", gmatrix="mpfao", density="exponential",
@@ -79,6 +85,8 @@ def main():
     for name, case in CASES.items():
         c = dict(case)
         c["expected"] = parse_gold(os.path.join(REF, c.pop("gold")))
+        if "mesh_file" in c:
+            shutil.copyfile(os.path.join(REF, "share/meshes", c["mesh_file"]), os.path.join(HERE, c["mesh_file"]))
         out["cases"][name] = c
     raw = open(os.path.join(REF, "demo/richards/richards_driver_snes_prob1_3x3x3_random.vec"), "rb").read()
     hdr = np.frombuffer(raw[:8], dtype=">i4")

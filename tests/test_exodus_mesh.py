@@ -1,0 +1,74 @@
+"""Exodus II mesh input (the reference's DMPlexCreateFromFile path,
+demo/transient/transient_snes_mpfaof90.F90:149) and the geometry of meshes
+given by connectivity alone."""
+import os
+
+import numpy as np
+
+from tdycore_amd import exodus
+from tdycore_amd import meshgen as mg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEDGE = os.path.join(HERE, "golden", "2x2x2_wedge_mesh.exo")
+
+
+def _closed(m):
+    """Sum over each cell of its outward area vectors (zero for a closed cell)."""
+    acc = np.zeros((m.nc, 3))
+    for f in range(m.nf):
+        up, dn = m.face_cells[f]
+        v = m.face_normal[f] * m.face_area[f]
+        acc[dn] -= v
+        if up >= 0:
+            acc[up] += v
+    return acc
+
+
+def test_reads_wedge_exodus_file():
+    v = exodus.read_netcdf(WEDGE)
+    assert v["connect1"].shape == (16, 6) and v["coordx"].shape == (27,)
+    m = exodus.exodus_mesh(WEDGE)
+    assert (m.nc, m.nf, m.nv, m.nbnd) == (16, 56, 27, 32)
+    assert np.all(m.cell_verts[:, 6:] == -1) and np.all(m.cell_faces[:, 5] == -1)
+    nvf = (m.face_verts >= 0).sum(axis=1)
+    assert (nvf == 3).sum() == 24 and (nvf == 4).sum() == 32
+    np.testing.assert_allclose(m.cell_vol, 0.125, rtol=1e-14)
+    assert abs(_closed(m)).max() < 1e-14
+    # normals run from up to dn
+    up, dn = m.face_cells[:, 0], m.face_cells[:, 1]
+    frm = np.where(up[:, None] >= 0, m.cell_cen[np.maximum(up, 0)], m.face_cen)
+    assert np.all(((m.cell_cen[dn] - frm) * m.face_normal).sum(axis=1) > 0)
+    # cells keep the file's element order
+    X = np.stack([v["coordx"], v["coordy"], v["coordz"]], 1)
+    np.testing.assert_allclose(m.cell_cen, X[v["connect1"] - 1].mean(axis=1), atol=1e-15)
+
+
+def _reconnect(m):
+    """Connectivity of m with boundary faces given as (cell, -1)."""
+    fc = m.face_cells.copy()
+    b = fc[:, 0] < 0
+    fc[b, 0], fc[b, 1] = fc[b, 1], -1
+    return mg.polyhedral_mesh(m.vert_xyz, m.cell_verts, m.cell_faces, m.face_verts, fc)
+
+
+def test_geometry_from_connectivity_matches_box_mesh():
+    m = mg.box_mesh(3, 4, 2, jitter=0.2, seed=5)
+    g = _reconnect(m)
+    np.testing.assert_allclose(g.cell_vol, m.cell_vol, rtol=1e-12)
+    np.testing.assert_allclose(g.cell_cen, m.cell_cen, atol=1e-12)
+    np.testing.assert_allclose(g.face_area, m.face_area, rtol=1e-12)
+    np.testing.assert_allclose(g.face_cen, m.face_cen, atol=1e-12)
+    np.testing.assert_allclose(g.face_normal, m.face_normal, atol=1e-12)
+    assert np.array_equal(g.face_cells, m.face_cells) and np.array_equal(g.bnd_face, m.bnd_face)
+
+
+def test_geometry_from_connectivity_matches_prism_mesh():
+    from tdycore_amd import workloads
+    pts, tri = workloads.wheel_triangulation(n_spokes=7, rings=2, radius=10.0)
+    m = mg.prism_mesh(pts, tri, np.array([0.0, 0.7, 1.5, 3.0]))
+    g = _reconnect(m)
+    np.testing.assert_allclose(g.cell_vol, m.cell_vol, rtol=1e-12)
+    np.testing.assert_allclose(g.cell_cen, m.cell_cen, atol=1e-12)
+    np.testing.assert_allclose(g.face_area, m.face_area, rtol=1e-12)
+    np.testing.assert_allclose(g.face_normal, m.face_normal, atol=1e-12)
+    assert np.array_equal(g.face_cells, m.face_cells)

@@ -20,6 +20,9 @@ def test_device_newton_reproduces_reference_gold(name):
     # the golds were made with direct solves (-pc_type lu, or ILU(0) of a dense
     # 8x8 Jacobian): solve the Newton systems to 1e-14 here
     opts = g.solver_options(ksp_rtol=1e-14, ksp_restart=60)
+    if (P["mesh"].cell_verts < 0).any():
+        # prisms: blocks of two cells keep the in-block ILU rows within their limit
+        g.set_pc_partition(np.arange(P["mesh"].nc) // 2)
     U = torch.tensor(P["U0"], device="cuda")
     if case["form"] == "ts":
         st = g.run_ts_be(U, case["dt"], case["nsteps"], opts)
