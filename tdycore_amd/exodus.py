@@ -93,19 +93,24 @@ _HEX_FACES = [(0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (0, 3, 2, 
 _WEDGE_FACES = [(0, 1, 4, 3), (1, 2, 5, 4), (0, 3, 5, 2), (0, 2, 1), (3, 4, 5)]
 
 
-def exodus_mesh(path: str) -> mg.Mesh:
-    """Hexahedra and wedges of an Exodus file as a ``meshgen.Mesh`` (arrays
-    padded with -1 for wedges and triangles)."""
-    v = read_netcdf(path)
+def _coords(v: dict) -> np.ndarray:
     if "coord" in v:
-        X = v["coord"].T.copy()
-    else:
-        X = np.stack([v["coordx"], v["coordy"], v.get("coordz", np.zeros_like(v["coordx"]))], 1)
-    blocks = [v[k] for k in sorted(k for k in v if k.startswith("connect"))]
-    cells = []
-    for b in blocks:
-        for e in b - 1:
-            cells.append(list(e))
+        return v["coord"].T.copy()
+    cols = [v["coordx"], v["coordy"]] + ([v["coordz"]] if "coordz" in v else [])
+    return np.stack(cols, 1)
+
+
+def _elements(v: dict) -> list:
+    out = []
+    for k in sorted((k for k in v if k.startswith("connect")), key=lambda k: int(k[7:])):
+        out.extend(list(e) for e in v[k] - 1)
+    return out
+
+
+def connectivity_mesh(X: np.ndarray, cells: list) -> mg.Mesh:
+    """Mesh of hexahedra (8 nodes) and wedges (6 nodes) in Exodus node order:
+    faces are found by their vertex sets, numbered in order of first use, and
+    the geometry follows from ``meshgen.polyhedral_mesh``."""
     nc = len(cells)
     cell_verts = np.full((nc, 8), -1, dtype=np.int32)
     faces, face_cells, cell_faces = {}, [], np.full((nc, 6), -1, dtype=np.int32)
@@ -123,6 +128,8 @@ def exodus_mesh(path: str) -> mg.Mesh:
             key = tuple(sorted(fv))
             if key in faces:
                 f = faces[key]
+                if face_cells[f][1] >= 0:
+                    raise ValueError("a face shared by more than two cells")
                 face_cells[f][1] = c
             else:
                 f = len(face_verts)
@@ -135,4 +142,36 @@ def exodus_mesh(path: str) -> mg.Mesh:
     for f, fv in enumerate(face_verts):
         FV[f, :len(fv)] = fv
     FC = np.array(face_cells, dtype=np.int32)
-    return mg.polyhedral_mesh(X, cell_verts, cell_faces, FV, FC)
+    return mg.polyhedral_mesh(np.asarray(X, dtype=np.float64), cell_verts, cell_faces, FV, FC)
+
+
+def exodus_mesh(path: str) -> mg.Mesh:
+    """Hexahedra and wedges of an Exodus file as a ``meshgen.Mesh`` (arrays
+    padded with -1 for wedges and triangles).  Only the default boundary type
+    applies to such a mesh (``face_bc(default=...)``)."""
+    v = read_netcdf(path)
+    X = _coords(v)
+    if X.shape[1] == 2:
+        raise ValueError(f"{path}: a surface mesh; use extruded_exodus_mesh")
+    return connectivity_mesh(X, _elements(v))
+
+
+def extruded_exodus_mesh(path: str, layers: int, height: float = 1.0) -> mg.Mesh:
+    """A surface mesh of quadrilaterals and triangles extruded along +z into
+    ``layers`` equal layers of total ``height`` (PETSc's DMPlexExtrude, as the
+    reference drives it with -dm_plex_extrude_layers,
+    demo/transient/transient_snes_mpfaof90.cfg:35).  Surface element c gives
+    cells c*layers + l, l counted from the bottom, as DMPlexExtrude numbers
+    them."""
+    v = read_netcdf(path)
+    X2 = _coords(v)[:, :2]
+    ns = X2.shape[0]
+    z = np.arange(layers + 1, dtype=np.float64) * (height / layers)
+    X = np.concatenate([np.column_stack([X2, np.full(ns, zz)]) for zz in z])
+    cells = []
+    for e in _elements(v):
+        if len(e) not in (3, 4):
+            raise ValueError("surface elements must be triangles or quadrilaterals")
+        for l in range(layers):
+            cells.append([int(i) + l * ns for i in e] + [int(i) + (l + 1) * ns for i in e])
+    return connectivity_mesh(X, cells)

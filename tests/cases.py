@@ -21,7 +21,9 @@ def top110k(x):
 
 def gold_problem(case):
     """-> dict(mesh, face_bc, K, porosity, Sr, kwargs, U0) for a reference gold case."""
-    if "mesh_file" in case:
+    if "extrude_layers" in case:
+        m = exodus.extruded_exodus_mesh(os.path.join(HERE, "golden", case["mesh_file"]), case["extrude_layers"])
+    elif "mesh_file" in case:
         m = exodus.exodus_mesh(os.path.join(HERE, "golden", case["mesh_file"]))
     else:
         m = mg.box_mesh(*case["mesh"])

@@ -67,6 +67,13 @@ CASES = {
         gold="demo/transient/transient-snes-mpfaof90-wedge.regression.gold",
         form="snes", mesh_file="2x2x2_wedge_mesh.exo", bc="dirichlet", gmatrix="tpf", density="exponential",
         K=1e-10, porosity=0.115, Sr=0.115, ic=102325.0, dt=1800.0, nsteps=3, bc_fn="top110k", nsample=10),
+    # a quadrilateral surface mesh extruded into 3 layers of hexahedra (max_steps keeps the
+    # demo's default of 2, transient_snes_mpfaof90.F90:223)
+    "transient-snes-mpfaof90-dmplex-extrude": dict(
+        gold="demo/transient/transient-snes-mpfaof90-dmplex-extrude.regression.gold",
+        form="snes", mesh_file="3x3_quad_surface_mesh.exo", extrude_layers=3, bc="dirichlet", gmatrix="tpf",
+        density="exponential", K=1e-10, porosity=0.115, Sr=0.115, ic=102325.0, dt=1800.0, nsteps=2,
+        bc_fn="top110k", nsample=10),
     "richards-driver-snes-prob1": dict(
         gold="demo/richards/richards-driver-snes-prob1.regression.gold",
         form="tdyti", mesh=[2, 2, 2], bc="dirichlet", gmatrix="mpfao", density="exponential",
