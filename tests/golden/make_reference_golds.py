@@ -67,6 +67,12 @@ CASES = {
         gold="demo/transient/transient-snes-mpfaof90-wedge.regression.gold",
         form="snes", mesh_file="2x2x2_wedge_mesh.exo", bc="dirichlet", gmatrix="tpf", density="exponential",
         K=1e-10, porosity=0.115, Sr=0.115, ic=102325.0, dt=1800.0, nsteps=3, bc_fn="top110k", nsample=10),
+    # a terrain-following transect of hexahedra read from Exodus (-tdy_read_mesh), no-flow boundaries
+    "transient-snes-mpfaof90-transect-neumann": dict(
+        gold="demo/transient/transient-snes-mpfaof90-transect-neumann.regression.gold",
+        form="snes", mesh_file="transect_44x1x3_hex_uniform_dz_mesh.exo", bc="neumann", gmatrix="mpfao",
+        density="exponential", K=1e-10, porosity=0.115, Sr=0.115, ic=57180.0, dt=1800.0, nsteps=10,
+        snes_rtol=1e-30, bc_fn=None, nsample=2),
     # a quadrilateral surface mesh extruded into 3 layers of hexahedra (max_steps keeps the
     # demo's default of 2, transient_snes_mpfaof90.F90:223)
     "transient-snes-mpfaof90-dmplex-extrude": dict(

@@ -19,7 +19,7 @@ def test_device_newton_reproduces_reference_gold(name):
     g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
     # the golds were made with direct solves (-pc_type lu, or ILU(0) of a dense
     # 8x8 Jacobian): solve the Newton systems to 1e-14 here
-    opts = g.solver_options(ksp_rtol=1e-14, ksp_restart=60)
+    opts = g.solver_options(ksp_rtol=1e-14, ksp_restart=60, snes_rtol=case.get("snes_rtol", 1e-8))
     if (P["mesh"].cell_verts < 0).any():
         # prisms: blocks of two cells keep the in-block ILU rows within their limit
         g.set_pc_partition(np.arange(P["mesh"].nc) // 2)
