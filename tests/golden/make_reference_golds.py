@@ -42,6 +42,10 @@ CASES = {
         gold="demo/transient/transient-mpfaof90-tpf-neumann.regression.gold",
         form="ts", mesh=[3, 3, 3], bc="neumann", gmatrix="tpf", density="constant",
         K=1e-10, porosity=0.115, Sr=0.115, ic=90325.0, dt=100.0, nsteps=10, bc_fn="const1e5", nsample=2),
+    "transient-mpfaof90-tpf": dict(
+        gold="demo/transient/transient-mpfaof90-tpf.regression.gold",
+        form="ts", mesh=[3, 3, 3], bc="dirichlet", gmatrix="tpf", density="constant",
+        K=1e-10, porosity=0.115, Sr=0.115, ic=90325.0, dt=100.0, nsteps=10, bc_fn="const1e5", nsample=2),
     "transient-mpfaof90": dict(
         gold="demo/transient/transient-mpfaof90.regression.gold",
         form="ts", mesh=[3, 3, 3], bc="dirichlet", gmatrix="mpfao", density="constant",
