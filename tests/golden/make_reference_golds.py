@@ -94,6 +94,13 @@ CASES = {
         form="tdyti", mesh=[2, 2, 2], bc="dirichlet", gmatrix="mpfao", density="exponential",
         K=1e-12, porosity=0.25, Sr=0.15, ic="vec", dt_init=0.1, growth=1.5, dt_max=600.0,
         final_time=3.1536e3, bc_fn=None, nsample=2),
+    # restarts from the checkpoint that -checkpoint-write leaves at t = 3153.6 s (dataset fields/IC,
+    # src/tdyio.c:340-359): the IC is that run's final state, and the integrator starts again from dt_init
+    "richards-driver-snes-checkpoint-read": dict(
+        gold="demo/richards/richards-driver-snes-checkpoint-read.regression.gold",
+        form="tdyti", mesh=[2, 2, 2], bc="dirichlet", gmatrix="mpfao", density="exponential",
+        K=1e-12, porosity=0.25, Sr=0.15, ic="vec", dt_init=0.1, growth=1.5, dt_max=600.0,
+        final_time=3.1536e3, bc_fn=None, nsample=2, restart=1),
 }
 
 
