@@ -122,6 +122,16 @@ int tdyhip_snes_jacobian(tdyhip_ctx *ctx, const double *U, double *Jvals, tdyhip
 int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liquid_mass, double *water_mass,
                        tdyhip_stream stream);
 
+/* Velocity recovery: TDyMPFAORecoverVelocity (src/fv/mpfao/tdympfao_utils.c:661-688).  With the
+ * cell potential Phi = p + rho(p)*|g|*z_c, every flux row of face f adds -(T_row . Phi)/A_f to
+ * vel[f] and 1 to count[f] (the reference's vel and vel_count, tdympfao_utils.c:103-217).  A
+ * boundary column takes the boundary-pressure function value (no gravity term) when one is set,
+ * else Phi of the face's cell (tdympfao_utils.c:314-324).  Boundary faces sum all their sub-face
+ * rows, as interior faces do; the reference fills them only from the single-cell corner vertices
+ * (534-659).  No-flow faces get 0.  The reference's error norm against a prescribed velocity
+ * field (MMS runs) is not computed.  Device arrays of nf entries; count may be NULL. */
+int tdyhip_recover_velocity(tdyhip_ctx *ctx, const double *U, double *vel, int32_t *count, tdyhip_stream stream);
+
 /* ---- Newton / Krylov (PETSc SNESNEWTONLS + KSPGMRES + PCBJACOBI/ILU(0) defaults) ---- */
 typedef struct {
   double snes_rtol, snes_atol, snes_stol; /* 1e-8, 1e-50, 1e-8 */

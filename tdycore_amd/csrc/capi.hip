@@ -66,6 +66,14 @@ static int dupload(tdyhip_ctx *ctx, T **p, const T *h, size_t n) {
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// cell-centroid heights and face areas, kept for velocity recovery
+static int upload_geometry_for_velocity(tdyhip_ctx *ctx, const tdyhip_mesh *m) {
+  std::vector<double> z((size_t)m->nc);
+  for (int c = 0; c < m->nc; c++) z[c] = m->cell_cen[(size_t)3 * c + 2];
+  int rc = dupload(ctx, &ctx->d_cell_z, z.data(), z.size());
+  return rc ? rc : dupload(ctx, &ctx->d_face_area, m->face_area, (size_t)m->nf);
+}
+
 // ---- kernel timers (events on the launching stream) ----
 static hipEvent_t kt_event(tdyhip_ctx *ctx) {
   hipEvent_t e = nullptr;
@@ -430,6 +438,7 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   if ((rc = dupload(ctx, &ctx->d_cell_faces, m->cell_faces, (size_t)6 * nc)) ||
       (rc = dupload(ctx, &ctx->d_cell_vol, m->cell_vol, nc)) || (rc = dupload(ctx, &ctx->d_phi, mat->porosity, nc)) ||
       (rc = dupload(ctx, &ctx->d_Sr, mat->Sr, nc)) || (rc = dupload(ctx, &ctx->d_bnd_cell, g->bnd_cell.data(), nbnd)) ||
+      (rc = upload_geometry_for_velocity(ctx, m)) ||
       (rc = dupload(ctx, &ctx->d_rtoff, g->rtoff.data(), g->rtoff.size())) ||
       (rc = dupload(ctx, &ctx->d_rinfo, g->rinfo.data(), g->rinfo.size())) ||
       (rc = dupload(ctx, &ctx->d_roc, g->roc.data(), g->roc.size())) ||
@@ -613,6 +622,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
     if ((rc = dupload(ctx, &ctx->d_pbnd_fn, mat->p_bnd_fn, nbnd))) return fail(rc);
   }
   if ((rc = dupload(ctx, &ctx->d_bnd_cell, topo.bnd_cell.data(), nbnd))) return fail(rc);
+  if ((rc = upload_geometry_for_velocity(ctx, m))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_vcols, topo.vcols.data(), topo.vcols.size()))) return fail(rc);
   if ((rc = dalloc(ctx, &ctx->d_gtab, (size_t)24 * nc))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jrp, topo.jrp.data(), topo.jrp.size()))) return fail(rc);
@@ -824,6 +834,23 @@ int tdyhip_diagnostics(tdyhip_ctx *ctx, const double *U, double *S, double *liqu
   StateParams sp = state_params(ctx, 1, 0.0);
   k_diagnostics<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol, S,
                                                     liquid_mass, water_mass);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int tdyhip_recover_velocity(tdyhip_ctx *ctx, const double *U, double *vel, int32_t *count, tdyhip_stream s) {
+  if (!ctx || !U || !vel) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  hipStream_t st; check_stream(ctx, s, &st);
+  VelArgs a{};
+  a.nc = ctx->nc; a.nf = ctx->nf; a.general = ctx->general ? 1 : 0; a.has_fn = ctx->has_pbnd_fn ? 1 : 0;
+  a.density_type = ctx->density_type; a.nrows = (int64_t)4 * ctx->nf; a.gabs = fabs(ctx->gravity);
+  a.face_flags = ctx->d_face_flags;
+  a.face_verts = ctx->d_face_verts; a.vcols = ctx->d_vcols; a.T = ctx->d_T;
+  a.rtoff = ctx->d_rtoff; a.rinfo = ctx->d_rinfo; a.roc = ctx->d_roc; a.rdc = ctx->d_rdc;
+  a.ccell = ctx->d_ccell; a.vdcol = ctx->d_vdcol; a.Tgen = ctx->d_Tgen;
+  a.bnd_cell = ctx->d_bnd_cell; a.U = U; a.cell_z = ctx->d_cell_z; a.pbnd_fn = ctx->d_pbnd_fn;
+  a.face_area = ctx->d_face_area; a.vel = vel; a.count = count;
+  k_velocity<<<nblk(ctx->nf, 256), 256, 0, st>>>(a);
   HIPCHK(hipGetLastError());
   return 0;
 }

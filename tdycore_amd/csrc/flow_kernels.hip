@@ -525,4 +525,72 @@ __global__ void k_fill(int64_t n, double *__restrict__ x, double val) {
   if (i < n) x[i] = val;
 }
 
+// Velocity recovery (TDyMPFAORecoverVelocity, tdympfao_utils.c:103-217, 219-532):
+// the potential of each cell is Phi = p + rho(p) |g| z_c, and every flux row
+// q of face f adds -(T_row . Phi) / A_f to vel[f].  A boundary column takes
+// the boundary-pressure function value (no gravity term) when one is set,
+// else Phi of the face's cell (tdympfao_utils.c:314-324).  One thread per
+// face sums its rows in sub-face order, so the result is deterministic.
+struct VelArgs {
+  int nc, nf, general, has_fn, density_type;
+  int64_t nrows;
+  double gabs;  // |g|
+  const uint8_t *face_flags;
+  const int32_t *face_verts, *vcols;                          // hexahedra: rows by vertex columns
+  const double *T;                                            // hexahedra: slot-major over rows
+  const int32_t *rtoff, *rinfo, *roc, *rdc, *ccell, *vdcol;   // general meshes: per-row blocks
+  const double *Tgen;
+  const int32_t *bnd_cell;
+  const double *U, *cell_z, *pbnd_fn, *face_area;
+  double *vel;
+  int32_t *count;
+};
+
+__device__ inline double vel_phi_cell(const VelArgs &a, int c) {
+  double rho, drho, d2rho;
+  water_density(a.density_type, a.U[c], rho, drho, d2rho);
+  return a.U[c] + rho * a.gabs * a.cell_z[c];
+}
+
+__device__ inline double vel_phi_bnd(const VelArgs &a, int b) {
+  return a.has_fn ? a.pbnd_fn[b] : vel_phi_cell(a, a.bnd_cell[b]);
+}
+
+__global__ void __launch_bounds__(256) k_velocity(VelArgs a) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.nf) return;
+  double s = 0.0;
+  int n = 0;
+  if (!(a.face_flags[f] & kFaceNoFlow)) {
+    const double area = a.face_area[f];
+    for (int q = 0; q < 4; q++) {
+      const int64_t r = (int64_t)4 * f + q;
+      double tp = 0.0;
+      if (a.general) {
+        const int to = a.rtoff[r];
+        if (to < 0) continue;  // the fourth row of a triangle
+        const int ncell = a.rinfo[r] & 0xff, ndir = (a.rinfo[r] >> 8) & 0xff;
+        const int32_t *cols = a.ccell + a.roc[r];
+        const int32_t *dcols = a.vdcol + a.rdc[r];
+        const double *Tr = a.Tgen + to;
+        for (int j = 0; j < ncell; j++) tp += Tr[j] * vel_phi_cell(a, cols[j]);
+        for (int j = 0; j < ndir; j++) tp += Tr[ncell + j] * vel_phi_bnd(a, dcols[j]);
+      } else {
+        const int32_t *cols = a.vcols + (int64_t)a.face_verts[r] * kMaxCols;
+        for (int k = 0; k < kMaxCols; k++) {
+          const int col = cols[k];
+          if (col < 0) continue;
+          const double phi = col < a.nc ? vel_phi_cell(a, col) : vel_phi_bnd(a, col - a.nc);
+          tp += a.T[(int64_t)k * a.nrows + r] * phi;
+        }
+      }
+      s += -tp / area;
+      n++;
+    }
+  }
+  a.vel[f] = s;
+  if (a.count) a.count[f] = n;
+}
+
 }  // namespace tdyhip
+

@@ -135,6 +135,8 @@ struct tdyhip_ctx {
   double *d_cell_vol = nullptr, *d_phi = nullptr, *d_Sr = nullptr;
   double *d_pbnd_fn = nullptr;      // nbnd
   int32_t *d_bnd_cell = nullptr;    // nbnd
+  double *d_cell_z = nullptr;       // nc cell-centroid z (velocity recovery)
+  double *d_face_area = nullptr;    // nf
 
   // device: discretisation
   int32_t *d_vcols = nullptr;       // nv*kMaxCols

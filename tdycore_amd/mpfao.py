@@ -180,6 +180,17 @@ class MPFAO:
         self._run(_lib.load().tdyhip_diagnostics, _ptr(U), _ptr(S), _ptr(M), _ptr(W), None)
         return S, M, W
 
+    def RecoverVelocity(self, U):
+        """(vel, count) per face at U (TDyMPFAORecoverVelocity,
+        src/fv/mpfao/tdympfao_utils.c:661-688): vel[f] sums -(T_row . Phi)/A_f over
+        the face's flux rows with Phi = p + rho |g| z, count[f] is the number of
+        rows summed."""
+        import torch
+        vel = torch.empty(self.nf, dtype=torch.float64, device=U.device)
+        cnt = torch.empty(self.nf, dtype=torch.int32, device=U.device)
+        self._run(_lib.load().tdyhip_recover_velocity, _ptr(U), _ptr(vel), _ptr(cnt), None)
+        return vel, cnt
+
     def device_bytes(self) -> int:
         """Device memory held by this context (setup arrays + solver workspace)."""
         b = C.c_int64()
