@@ -9,6 +9,12 @@ continuation blocks, symbol-table groups (v1 B-trees, local heaps), and
 contiguous, compact or chunked (unfiltered) datasets of IEEE floats and
 integers of either byte order.  That is the layout h5py and PETSc write by
 default; anything else raises ``ValueError``.
+
+``write_checkpoint`` writes the reference's restart file (TDyIOOutputCheckpoint,
+src/tdyio.c:339-359): the state in natural cell order as dataset "fields/IC"
+and the step count as the root attribute "time step", in the same classic
+layout (superblock 0, symbol-table groups, a contiguous dataset), so that
+``-ic_file <file> -ic_dataset fields/IC`` restarts from it.
 """
 from __future__ import annotations
 
@@ -234,3 +240,145 @@ def list_members(path: str, group: str = "") -> list:
     """Member names of a group ("" for the root group)."""
     f = _File(open(path, "rb").read())
     return f.names(group)
+
+
+# ---- writing --------------------------------------------------------------
+
+def checkpoint_filename(time: float) -> str:
+    """The reference's checkpoint file name for a simulation time
+    (``"%11.5e_chk.h5"``, src/tdyio.c:348)."""
+    return "%11.5e_%s.h5" % (time, "chk")
+
+
+class _Writer:
+    """Bump allocator over a bytearray, 8-byte aligned."""
+
+    def __init__(self):
+        self.b = bytearray()
+
+    def alloc(self, n):
+        a = len(self.b)
+        n = (n + 7) & ~7
+        self.b += bytes(n)
+        return a
+
+    def put(self, addr, fmt, *vals):
+        struct.pack_into("<" + fmt, self.b, addr, *vals)
+
+
+_LEAF_K, _NODE_K = 4, 16
+_BTREE_SIZE = 24 + (2 * _NODE_K + 1) * 8 + 2 * _NODE_K * 8
+_SNOD_SIZE = 8 + 2 * _LEAF_K * 40
+
+
+def _pad8(b: bytes) -> bytes:
+    return b + bytes((-len(b)) % 8)
+
+
+def _msg(mtype: int, body: bytes) -> bytes:
+    body = _pad8(body)
+    return struct.pack("<HHB3x", mtype, len(body), 0) + body
+
+
+def _object_header(w: _Writer, msgs) -> int:
+    body = b"".join(msgs)
+    a = w.alloc(16 + len(body))
+    w.put(a, "BBHII", 1, 0, len(msgs), 1, len(body))
+    w.b[a + 16:a + 16 + len(body)] = body
+    return a
+
+
+def _group(w: _Writer, members) -> tuple:
+    """A symbol-table group of ``members`` = [(name, header address,
+    (btree, heap) or None)]; returns (B-tree address, heap address)."""
+    members = sorted(members)
+    if len(members) > 2 * _LEAF_K:
+        raise ValueError("too many members for one symbol-table node")
+    names = b"\0" * 8
+    offs = []
+    for nm, _, _ in members:
+        offs.append(len(names))
+        names += _pad8(nm.encode() + b"\0")
+    seg_size = len(names) + 16  # trailing free block, as the HDF5 library leaves one
+    bt = w.alloc(_BTREE_SIZE)
+    heap = w.alloc(32)
+    seg = w.alloc(seg_size)
+    snod = w.alloc(_SNOD_SIZE)
+    w.b[bt:bt + 8] = b"TREE" + bytes([0, 0]) + struct.pack("<H", 1)
+    w.put(bt + 8, "QQ", _UNDEF, _UNDEF)
+    w.put(bt + 24, "QQQ", 0, snod, offs[-1] if offs else 0)
+    w.b[heap:heap + 8] = b"HEAP" + bytes(4)
+    w.put(heap + 8, "QQQ", seg_size, len(names), seg)
+    w.b[seg:seg + len(names)] = names
+    w.put(seg + len(names), "QQ", 1, 16)  # free block: no next block (1), 16 bytes
+    w.b[snod:snod + 8] = b"SNOD" + bytes([1, 0]) + struct.pack("<H", len(members))
+    for i, ((nm, hdr, stab), off) in enumerate(zip(members, offs)):
+        e = snod + 8 + 40 * i
+        w.put(e, "QQII", off, hdr, 1 if stab else 0, 0)
+        if stab:
+            w.put(e + 24, "QQ", *stab)
+    return bt, heap
+
+
+_F64_LE = bytes.fromhex("11203f000800000000004000340b0034ff030000")
+_I32_LE = struct.pack("<BBBBIHH", 0x10, 0x08, 0, 0, 4, 0, 32)
+
+
+def write_checkpoint(path: str, values, time_step: int) -> None:
+    """Write ``values`` (the state in natural cell order) as "fields/IC" and
+    ``time_step`` as the root attribute "time step"."""
+    v = np.ascontiguousarray(np.asarray(values, dtype="<f8").ravel())
+    n = v.shape[0]
+    w = _Writer()
+    w.alloc(96)  # superblock and root symbol-table entry
+    # the dataset
+    space = struct.pack("<BBBB4xQQ", 1, 1, 1, 0, n, n)
+    fill = bytes([2, 2, 2, 1]) + struct.pack("<I", 0)
+    ds = _object_header(w, [_msg(0x01, space), _msg(0x03, _F64_LE), _msg(0x05, fill),
+                            _msg(0x08, struct.pack("<BBQQ", 3, 1, 0, n * 8))])
+    # group "fields"
+    bt, heap = _group(w, [("IC", ds, None)])
+    fields = _object_header(w, [_msg(0x11, struct.pack("<QQ", bt, heap))])
+    # root group with the attribute
+    rbt, rheap = _group(w, [("fields", fields, (bt, heap))])
+    name = b"time step\0"
+    scalar = struct.pack("<BBBB4x", 1, 0, 0, 0)
+    attr = (struct.pack("<BBHHH", 1, 0, len(name), len(_I32_LE), len(scalar)) + _pad8(name) + _pad8(_I32_LE)
+            + _pad8(scalar) + struct.pack("<i", int(time_step)))
+    root = _object_header(w, [_msg(0x11, struct.pack("<QQ", rbt, rheap)), _msg(0x0C, attr)])
+    data_addr = w.alloc(n * 8)
+    w.b[data_addr:data_addr + n * 8] = v.tobytes()
+    # the dataset's layout message points at the data
+    lay = ds + 16 + (8 + 24) + (8 + 24) + (8 + 8) + 8
+    w.put(lay + 2, "Q", data_addr)
+    # superblock
+    w.b[0:8] = _SIG
+    w.put(8, "BBBBBBBB", 0, 0, 0, 0, 0, 8, 8, 0)
+    w.put(16, "HHI", _LEAF_K, _NODE_K, 0)
+    w.put(24, "QQQQ", 0, _UNDEF, len(w.b), _UNDEF)
+    w.put(56, "QQII", 0, root, 1, 0)
+    w.put(80, "QQ", rbt, rheap)
+    with open(path, "wb") as fh:
+        fh.write(bytes(w.b))
+
+
+def read_attribute(path: str, name: str, obj: str = ""):
+    """A scalar integer or float attribute of an object ("" = root group)."""
+    f = _File(open(path, "rb").read())
+    hdr = f.root_hdr if not obj else f.lookup(obj)
+    for mtype, body, _ in f.messages(hdr):
+        if mtype != 0x0C:
+            continue
+        ver = f.d[body]
+        if ver != 1:
+            raise ValueError(f"attribute message version {ver} is not supported")
+        nsz, tsz, ssz = f.u(body + 2, 2), f.u(body + 4, 2), f.u(body + 6, 2)
+        p = body + 8
+        nm = f.d[p:p + nsz].split(b"\0")[0].decode()
+        p += (nsz + 7) & ~7
+        dt = f._datatype(p)
+        p += (tsz + 7) & ~7
+        p += (ssz + 7) & ~7
+        if nm == name:
+            return np.frombuffer(f.d, dtype=dt, count=1, offset=p)[0].item()
+    raise KeyError(f"{obj or '/'}: no attribute '{name}'")

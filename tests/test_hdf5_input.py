@@ -1,5 +1,6 @@
 """HDF5 initial-condition and material input (the reference's VecLoad path,
-src/tdyio.c:176-330) read from the reference's demo/richards/initial.h5."""
+src/tdyio.c:176-330) read from the reference's demo/richards/initial.h5, and
+the checkpoint file (TDyIOOutputCheckpoint, src/tdyio.c:339-359)."""
 import os
 
 import numpy as np
@@ -39,3 +40,24 @@ def test_missing_member_and_foreign_file_raise(tmp_path):
     p.write_bytes(b"not hdf5" * 8)
     with pytest.raises(ValueError):
         hdf5io.read_dataset(str(p), "IC")
+
+
+def test_checkpoint_round_trip_and_layout(tmp_path):
+    v = np.linspace(9.0e4, 1.0e5, 27)
+    p = str(tmp_path / hdf5io.checkpoint_filename(3153.6))
+    assert p.endswith("3.15360e+03_chk.h5")
+    hdf5io.write_checkpoint(p, v, 42)
+    assert hdf5io.list_members(p) == ["fields"] and hdf5io.list_members(p, "fields") == ["IC"]
+    r = hdf5io.read_dataset(p, "fields/IC")
+    assert r.dtype == np.float64 and np.array_equal(r, v)
+    assert hdf5io.read_attribute(p, "time step") == 42
+    # the dataset's header messages are the ones the HDF5 library wrote into initial.h5
+    # (dataspace with max dims, IEEE little-endian double, fill value, contiguous layout)
+    ref = hdf5io._File(open(H5, "rb").read())
+    mine = hdf5io._File(open(p, "rb").read())
+    kinds = lambda f, path: [t for t, _, _ in f.messages(f.lookup(path))]
+    assert kinds(mine, "fields/IC") == [t for t in kinds(ref, "IC") if t in (1, 3, 5, 8)]
+    dt = lambda f, path: [f.d[b:b + n] for t, b, n in f.messages(f.lookup(path)) if t == 3][0]
+    assert dt(mine, "fields/IC") == dt(ref, "IC")
+    with pytest.raises(KeyError):
+        hdf5io.read_attribute(p, "missing")
