@@ -39,6 +39,8 @@ struct BJLayout {
   int32_t *d_nlev = nullptr;   // nblocks*2 (L, U level counts)
   int32_t *d_nlu = nullptr;    // nblocks: L/U slots in use (the factor writes and the sweeps read only
                                // these); nullptr when every block uses all kLU
+  int32_t *d_bw = nullptr;     // nblocks: ELL slots in use (the block's longest row; the SpMV reads only
+                               // these); nullptr when every block reads all W
   uint16_t *d_cL = nullptr;    // nblocks*kLU*bs local column of L slot (bs = zero entry)
   uint16_t *d_cU = nullptr;    // nblocks*kLU*bs local column of U slot
   // ILU(0) update lists: for row i and lower slot p (row k), pairs (d, s) with
@@ -310,15 +312,15 @@ __device__ inline void ecol_addr(const int32_t *__restrict__ pat, int bs, int W,
 template <class TV, class TI>
 __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32_t *__restrict__ rowof,
                                                   const TI *__restrict__ ecol, const int32_t *__restrict__ pat,
-                                                  int bs, const TV *__restrict__ a, const TV *__restrict__ x,
-                                                  TV *__restrict__ y) {
+                                                  int bs, const int32_t *__restrict__ bw, const TV *__restrict__ a,
+                                                  const TV *__restrict__ x, TV *__restrict__ y) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NB) return;
   const int row = rowof[p];
   if (row < 0) return;
   int64_t eb, es;
   ecol_addr(pat, bs, W, NB, p, eb, es);
-  y[row] = blk_row<TV, TI>(NB, W, p, row, ecol, eb, es, a, x);
+  y[row] = blk_row<TV, TI>(NB, bw ? bw[p / bs] : W, p, row, ecol, eb, es, a, x);
 }
 
 // Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
@@ -405,7 +407,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
   // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
   int64_t eb, es;
   ecol_addr(L.d_pat, BS, W, NB, p, eb, es);
-  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, W, p, row, ecol, eb, es, a, u) : (TV)0;
+  const int wb = L.d_bw ? L.d_bw[b] : W;  // slots past the block's longest row are padding
+  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, wb, p, row, ecol, eb, es, a, u) : (TV)0;
   if (row >= 0) vj[row] = sc * u[row];
   if (t == 0) x[BS] = (TV)0;
   if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform

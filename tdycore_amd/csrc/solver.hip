@@ -25,6 +25,7 @@ struct HostBJ {
   std::vector<int32_t> nlev;
   std::vector<int32_t> nlu;   // nblocks: most strictly lower (or upper) in-block entries of a row
   std::vector<int32_t> perm;  // nc: block-layout position of each cell
+  std::vector<int32_t> bw;    // nblocks: longest row of each block (ELL slots in use)
   std::vector<int32_t> ecol;  // W*nblocks*bs: global column of slot k at block position p (k*NB + p)
   std::vector<int16_t> ecol16;  // the same as offsets from the row, when every offset fits in int16
   // Deduplicated index patterns (int16 offsets only): blocks whose L/U local
@@ -197,6 +198,11 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
   }
   for (int c = 0; c < nc; c++)
     if (out.perm[c] < 0) return "cell missing from the block-Jacobi partition";
+  out.bw.assign(out.nblocks, 0);
+  for (int64_t p = 0; p < NB; p++) {
+    const int r = out.row[p];
+    if (r >= 0) out.bw[p / S] = std::max(out.bw[p / S], jrp[r + 1] - jrp[r]);
+  }
   bool fits = true;
   for (int64_t p = 0; p < NB && fits; p++) {
     const int r = out.row[p];
@@ -374,6 +380,7 @@ struct SolverWS {
   bool coarse = false;
   int coarse_req = -1;
   double lu_mean = 0.0;          // mean per-block L/U slot count the sweeps read
+  double w_mean = 0.0;           // mean per-block ELL slot count the SpMV reads
   CoarseLayout cl;
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
@@ -447,6 +454,16 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
     double sum = 0.0;  // mean stored L/U width over the blocks (byte model)
     for (size_t b = 0; b < (size_t)hb.nblocks; b++) sum += hb.nlu[hb.pat.empty() ? b : (size_t)hb.pat[b]];
     ws.lu_mean = hb.nblocks ? sum / hb.nblocks : 0.0;
+  }
+  // per-block ELL widths pay off where rows are wide and uneven (prism meshes); hexahedral
+  // rows fill all kW slots, and their FP32 SpMV unrolls for exactly kW
+  ws.bj.d_bw = nullptr;
+  ws.w_mean = ctx->jell_w;
+  if (ctx->jell_w > kW && std::any_of(hb.bw.begin(), hb.bw.end(), [&](int32_t v) { return v < ctx->jell_w; })) {
+    if ((rc = up(&ws.bj.d_bw, hb.bw))) return rc;
+    double sum = 0.0;
+    for (int32_t v : hb.bw) sum += v;
+    ws.w_mean = hb.nblocks ? sum / hb.nblocks : 0.0;
   }
   if (!hb.pat.empty() && (rc = up(&ws.bj.d_pat, hb.pat))) return rc;
   if (!hb.ecol16.empty()) {
@@ -526,8 +543,8 @@ static int dev_norm(tdyhip_ctx *ctx, const double *x, double *out, hipStream_t s
 // W*bs offsets), not once per row.
 static double bytes_spmv(const tdyhip_ctx *ctx, const SolverWS &ws, size_t tv, bool idx16) {
   const double idx = ws.bj.npat ? (double)ws.bj.npat * ctx->jell_w * ws.bj.bs * 2 + (double)ws.bj.nblocks * 4
-                                : (double)ctx->jell_w * ctx->nc * (idx16 ? 2 : 4);
-  return (double)ctx->jell_w * ctx->nc * tv + idx + 2.0 * ctx->nc * tv;
+                                : ws.w_mean * ctx->nc * (idx16 ? 2 : 4);
+  return ws.w_mean * ctx->nc * tv + idx + 2.0 * ctx->nc * tv;
 }
 
 // y = J x with the FP64 Jacobian in the block layout
@@ -536,10 +553,10 @@ static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, dou
   const bool i16 = ws.d_ecol16 != nullptr;
   if (i16)
     KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), true),
-       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.bj.d_pat, ws.bj.bs, ws.J, x, y));
+       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.bj.d_pat, ws.bj.bs, ws.bj.d_bw, ws.J, x, y));
   else
     KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), false),
-       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, nullptr, ws.bj.bs, ws.J, x, y));
+       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, nullptr, ws.bj.bs, ws.bj.d_bw, ws.J, x, y));
 }
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
