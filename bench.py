@@ -197,7 +197,8 @@ def prisms(a):
     from tdycore_amd.mpfao import MPFAO
     torch.cuda.init()
     t = time.perf_counter()
-    P = workloads.prism_problem(n_tri=a.prism_tri, nlayers=a.prism_layers, infiltration=True)
+    per = max(1, 64 // a.prism_layers)  # block-Jacobi blocks: vertical columns of `per` triangles
+    P = workloads.prism_problem(n_tri=a.prism_tri, nlayers=a.prism_layers, infiltration=True, block_tris=per)
     m = P["mesh"]
     log(f"prism mesh: {m.nc} cells, {m.nf} faces, {m.nv} vertices, built in {time.perf_counter() - t:.1f} s")
     runs = []
@@ -215,10 +216,8 @@ def prisms(a):
     info = runs[-1][1]
     # Newton steps: SNES backward Euler, dt = 1 h, GMRES(30) + block-Jacobi ILU(0)
     # on vertical columns of 3 triangles (60 cells for 20 layers)
-    from tdycore_amd import meshgen as mg
     g = MPFAO(m, P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
-    per = max(1, 64 // a.prism_layers)
-    g.set_pc_partition(mg.column_partition(m, per))
+    g.set_pc_partition(P["pc_part"])
     opts = g.solver_options(snes_rtol=1e-10, pc_block=64)
     U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
     g.run_snes_steps(U, 3600.0, 1, opts)  # warmup step (not timed): workspace, factor, graphs

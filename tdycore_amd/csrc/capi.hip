@@ -449,8 +449,12 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
       (rc = dupload(ctx, &ctx->d_jcols, g->jcols.data(), g->jcols.size())) ||
       (rc = dupload(ctx, &ctx->d_jdiag, g->jdiag.data(), g->jdiag.size())) ||
       (rc = dalloc(ctx, &ctx->d_Dgen, (size_t)std::max<int64_t>(g->tsize, 1))) ||
+      (rc = dalloc(ctx, &ctx->d_gslot, (size_t)2 * std::max<int64_t>(g->tsize, 1))) ||
       (rc = dalloc(ctx, &ctx->d_flux, (size_t)4 * nf)))
     return done(rc);
+  k_gen_slot_table<<<nblk((int64_t)4 * nf, 256), 256, 0, st>>>((int64_t)4 * nf, ctx->d_face_cells, ctx->d_face_flags,
+                                                                ctx->d_rtoff, ctx->d_rinfo, ctx->d_roc, ctx->d_ccell,
+                                                                ctx->d_jrp, ctx->d_jcols, ctx->d_gslot);
   if (mat->p_bnd_fn) {
     ctx->has_pbnd_fn = true;
     if ((rc = dupload(ctx, &ctx->d_pbnd_fn, mat->p_bnd_fn, nbnd))) return done(rc);
@@ -502,7 +506,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     gg.ell_w = ctx->jell_w; gg.perm = perm; gg.stride = perm ? ell_stride : 1;
     gg.cell_faces = ctx->d_cell_faces; gg.face_cells = ctx->d_face_cells; gg.rtoff = ctx->d_rtoff;
     gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
-    gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags;
+    gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags; gg.gslot = ctx->d_gslot;
     gg.D = ctx->d_Dgen; gg.dM = ctx->d_dM; gg.d2M = ctx->d_d2M; gg.Udot = Udot; gg.vol = ctx->d_cell_vol; gg.J = J;
     k_gather_general_J<<<nblk(ctx->nc, kGenJBlock), kGenJBlock, 0, st>>>(gg);
   }

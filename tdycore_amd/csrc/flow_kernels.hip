@@ -332,41 +332,72 @@ struct GenGatherArgs {
   const int32_t *perm;   // ELL: position of row c (nullptr: CSR at jrp[c])
   double shift, dtime;
   const int32_t *cell_faces, *face_cells, *rtoff, *rinfo, *roc, *ccell, *jrp, *jcols, *jdiag;
-  const uint8_t *face_flags;
+  const uint8_t *face_flags, *gslot;
   const double *D, *dM, *d2M, *Udot, *vol;
   double *J;
 };
 
+// Slot table of the general Jacobian gather, built once at setup: entry
+// to + j of flux row r (its cell column j) has the column's slot in the up
+// cell's Jacobian row at gslot[2*(to+j)] and in the down cell's at
+// gslot[2*(to+j)+1] (0xff for a boundary side or a no-flow face).  One
+// thread per flux row; the slot is found by binary search in the sorted row.
+__global__ void __launch_bounds__(256) k_gen_slot_table(int64_t nrows, const int32_t *__restrict__ face_cells,
+                                                        const uint8_t *__restrict__ face_flags,
+                                                        const int32_t *__restrict__ rtoff,
+                                                        const int32_t *__restrict__ rinfo,
+                                                        const int32_t *__restrict__ roc,
+                                                        const int32_t *__restrict__ ccell,
+                                                        const int32_t *__restrict__ jrp,
+                                                        const int32_t *__restrict__ jcols,
+                                                        uint8_t *__restrict__ gslot) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int to = rtoff[r];
+  if (to < 0) return;
+  const int f = (int)(r >> 2);
+  const int ncell = rinfo[r] & 0xff;
+  const int32_t *cols = ccell + roc[r];
+  const bool noflow = (face_flags[f] & kFaceNoFlow) != 0;
+  for (int side = 0; side < 2; side++) {
+    const int c = face_cells[2 * f + side];
+    for (int j = 0; j < ncell; j++) {
+      uint8_t s = 0xff;
+      if (c >= 0 && !noflow) {
+        const int r0 = jrp[c];
+        int lo = r0, hi = jrp[c + 1] - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (jcols[mid] < cols[j]) lo = mid + 1; else hi = mid;
+        }
+        s = (uint8_t)(lo - r0);
+      }
+      gslot[2 * ((int64_t)to + j) + side] = s;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a) {
   __shared__ double acc[kGenJSlots][kGenJBlock];
-  __shared__ int32_t jcs[kGenJSlots][kGenJBlock];  // the row's sorted columns, searched in LDS
   const int t = threadIdx.x;
   const int c = blockIdx.x * kGenJBlock + t;
   if (c >= a.nc) return;
   const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
-  for (int k = 0; k < len; k++) {
-    acc[k][t] = 0.0;
-    jcs[k][t] = a.jcols[r0 + k];
-  }
+  for (int k = 0; k < len; k++) acc[k][t] = 0.0;
   for (int lf = 0; lf < 6; lf++) {
     const int f = a.cell_faces[c * 6 + lf];
     if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
     const bool is_up = a.face_cells[2 * f] == c;
+    const int side = is_up ? 0 : 1;
     for (int q = 0; q < 4; q++) {
       const int64_t r = (int64_t)f * 4 + q;
       const int to = a.rtoff[r];
       if (to < 0) continue;
       const int ncell = a.rinfo[r] & 0xff;
-      const int32_t *cols = a.ccell + a.roc[r];
+      const uint8_t *sl = a.gslot + 2 * (int64_t)to + side;
       for (int j = 0; j < ncell; j++) {
-        const int col = cols[j];
-        int lo = 0, hi = len - 1;  // the column's slot in the sorted row
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (jcs[mid][t] < col) lo = mid + 1; else hi = mid;
-        }
         const double d = a.D[to + j];
-        acc[lo][t] += is_up ? d : -d;
+        acc[sl[2 * j]][t] += is_up ? d : -d;
       }
     }
   }

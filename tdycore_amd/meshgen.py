@@ -482,21 +482,56 @@ def delaunay_square(n_tri: int, length: float, rng: np.random.Generator, jitter:
 
 
 def column_partition(mesh: Mesh, per_block: int, grid: int = 256) -> np.ndarray:
-    """Block id per cell of a prism mesh (cell = t + nt*k): the vertical
-    columns of per_block triangles that are neighbours in a grid x grid
-    ordering of their centroids, so a block holds per_block * nlayers cells
-    and its rows couple through at most a few in-block neighbours."""
-    nt = mesh.nx
-    cen = mesh.cell_cen[:nt, :2]
+    """Block id per cell of a prism mesh: the vertical columns (cells with the
+    same x-y centroid) of per_block triangles that are neighbours in a
+    grid x grid ordering of their centroids, so a block holds per_block *
+    nlayers cells and its rows couple through at most a few in-block
+    neighbours.  Any cell numbering."""
+    xy = np.ascontiguousarray(mesh.cell_cen[:, :2])
+    key = xy.view(np.dtype((np.void, 16))).ravel()
+    _, first, col_of_cell = np.unique(key, return_index=True, return_inverse=True)
+    cen = xy[first]
+    ncol = cen.shape[0]
     lo, hi = cen.min(axis=0), cen.max(axis=0)
     ij = np.clip(((cen - lo) / np.maximum(hi - lo, 1e-300) * grid).astype(np.int64), 0, grid - 1)
     # serpentine row order keeps consecutive triangles close
     row = ij[:, 1]
     col = np.where(row % 2 == 0, ij[:, 0], grid - 1 - ij[:, 0])
     order = np.lexsort((col, row))
-    blk_of_tri = np.empty(nt, dtype=np.int32)
-    blk_of_tri[order] = np.arange(nt) // per_block
-    return blk_of_tri[np.arange(mesh.nc) % nt].astype(np.int32)
+    blk_of_col = np.empty(ncol, dtype=np.int32)
+    blk_of_col[order] = np.arange(ncol) // per_block
+    return blk_of_col[col_of_cell.ravel()].astype(np.int32)
+
+
+def rcm_triangle_order(triangles: np.ndarray, npoints: int) -> np.ndarray:
+    """Triangles in reverse Cuthill-McKee order of their vertex-sharing graph
+    (the horizontal pattern of the Jacobian): entry i is the triangle placed
+    i-th.  Neighbours end up close in the numbering, which bounds the column
+    offsets of the Jacobian rows."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    T = np.asarray(triangles, dtype=np.int64)
+    nt = T.shape[0]
+    inc = sp.csr_matrix((np.ones(3 * nt), (np.repeat(np.arange(nt), 3), T.ravel())), shape=(nt, npoints))
+    adj = (inc @ inc.T).tocsr()
+    return np.asarray(reverse_cuthill_mckee(adj, symmetric_mode=True), dtype=np.int64)
+
+
+def renumber_cells(mesh: Mesh, new_of_old: np.ndarray) -> Mesh:
+    """The same mesh with cell c renamed new_of_old[c] (faces, vertices and
+    boundary indices unchanged)."""
+    new_of_old = np.asarray(new_of_old, dtype=np.int64)
+    old_of_new = np.empty_like(new_of_old)
+    old_of_new[new_of_old] = np.arange(new_of_old.shape[0])
+    fc = mesh.face_cells.copy()
+    live = fc >= 0
+    fc[live] = new_of_old[fc[live]]
+    return Mesh(nx=mesh.nx, ny=mesh.ny, nz=mesh.nz, vert_xyz=mesh.vert_xyz,
+                cell_verts=mesh.cell_verts[old_of_new], cell_faces=mesh.cell_faces[old_of_new],
+                face_verts=mesh.face_verts, face_cells=fc.astype(np.int32), cell_cen=mesh.cell_cen[old_of_new],
+                cell_vol=mesh.cell_vol[old_of_new], face_cen=mesh.face_cen, face_area=mesh.face_area,
+                face_normal=mesh.face_normal, bnd_face=mesh.bnd_face, face_dir=mesh.face_dir,
+                face_side=mesh.face_side)
 
 
 def polyhedral_mesh(V: np.ndarray, cell_verts: np.ndarray, cell_faces: np.ndarray, face_verts: np.ndarray,

@@ -82,7 +82,7 @@ def ensemble_request_specs(n=512, seed=20261206):
 
 
 def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lattice=128, gmatrix="mpfao",
-                  points=None, triangles=None, infiltration=False):
+                  points=None, triangles=None, infiltration=False, column_major=True, block_tris=3):
     """Right prisms: a random Delaunay triangulation of about n_tri triangles
     on a length x length square (jittered lattice points, scipy's Delaunay),
     extruded through nlayers layers of thickness U(0.5, 5) m.  Permeability as
@@ -96,7 +96,16 @@ def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lat
     state is an exact discrete equilibrium (flat layers, vertical walls), so
     with ``infiltration`` the top boundary is held at atmospheric pressure
     instead (ponded infiltration into the unsaturated upper half), which makes
-    every time step a genuine Newton solve."""
+    every time step a genuine Newton solve.
+
+    Cell numbering: with ``column_major`` the triangles are put in reverse
+    Cuthill-McKee order and cut into groups of ``block_tris`` consecutive
+    ones; the cells of a group are numbered together, layer by layer inside
+    the group, so that every Jacobian column offset is small (it fits int16
+    at 4M prisms) and ``pc_part`` (one block-Jacobi block per group of
+    columns) keeps each block's rows in layer order, the order whose ILU(0)
+    converges best here.  Otherwise cell = triangle + n_tri * layer and
+    ``pc_part`` is None."""
     rng = np.random.default_rng(seed)
     if points is None:
         points, triangles = mg.delaunay_square(n_tri, length, rng)
@@ -120,8 +129,23 @@ def prism_problem(n_tri=200_000, nlayers=20, seed=20261205, length=10_000.0, lat
         top = m.face_cen[m.bnd_face, 2] >= Z[-1] - 1e-9
         pb[top] = 101325.0
     U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
+    pc_part = None
+    if column_major:
+        nt, g = triangles.shape[0], int(block_tris)
+        rank = np.empty(nt, dtype=np.int64)
+        rank[mg.rcm_triangle_order(triangles, points.shape[0])] = np.arange(nt)
+        grp, j = rank // g, rank % g
+        gsize = np.minimum(g, nt - grp * g)  # the last group may be short
+        c = np.arange(m.nc)
+        t, k = c % nt, c // nt
+        new_of_old = grp[t] * g * nlayers + k * gsize[t] + j[t]
+        m = mg.renumber_cells(m, new_of_old)
+        old_of_new = np.empty_like(new_of_old)
+        old_of_new[new_of_old] = c
+        K, phi, Sr, U0 = K[old_of_new], phi[old_of_new], Sr[old_of_new], U0[old_of_new]
+        pc_part = (c // (g * nlayers)).astype(np.int32)
     kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix=gmatrix, density="constant", p_bnd_fn=pb)
-    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0, pc_part=pc_part)
 
 
 def wheel_triangulation(n_spokes=14, rings=2, radius=100.0):
