@@ -452,6 +452,34 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
       (rc = dalloc(ctx, &ctx->d_gslot, (size_t)2 * std::max<int64_t>(g->tsize, 1))) ||
       (rc = dalloc(ctx, &ctx->d_flux, (size_t)4 * nf)))
     return done(rc);
+  {
+    // vertex blocks for the flux-row kernel: T offset, rows, cells, Dirichlet
+    // columns, cell offset, Dirichlet-column offset, first block row
+    const int nv = m->nv;
+    std::vector<int32_t> vb((size_t)nv * 7), br;
+    int32_t od = 0;
+    bool fits = true;
+    for (int v = 0; v < nv && fits; v++) {
+      const int32_t *d = g->vdesc.data() + (size_t)v * 8;
+      const int nface = d[1], of = d[5], nrow = d[7];
+      int32_t *o = vb.data() + (size_t)v * 7;
+      o[0] = d[6]; o[1] = nrow; o[2] = d[0]; o[3] = d[3]; o[4] = d[4]; o[5] = od; o[6] = (int32_t)br.size();
+      if (nrow > kBlkRows || br.size() + (size_t)nrow > (size_t)INT32_MAX) fits = false;
+      br.resize(br.size() + nrow, -1);
+      for (int k = 0; k < nface; k++) {
+        const int trow = (g->fb[of + k] >> 16) - 1;
+        if (trow >= 0 && trow < nrow) br[o[6] + trow] = g->frow[of + k];
+      }
+      od += d[3];
+    }
+    if (fits && !br.empty() && std::find(br.begin(), br.end(), -1) == br.end() && od == (int32_t)g->vdcol.size()) {
+      if ((rc = dupload(ctx, &ctx->d_vblk, vb.data(), vb.size())) ||
+          (rc = dupload(ctx, &ctx->d_brow, br.data(), br.size())))
+        return done(rc);
+      // rows without a block are never written by the block kernel: their flux stays 0
+      k_fill<<<nblk((int64_t)4 * nf, 256), 256, 0, st>>>((int64_t)4 * nf, ctx->d_flux, 0.0);
+    }
+  }
   k_gen_slot_table<<<nblk((int64_t)4 * nf, 256), 256, 0, st>>>((int64_t)4 * nf, ctx->d_face_cells, ctx->d_face_flags,
                                                                 ctx->d_rtoff, ctx->d_rinfo, ctx->d_roc, ctx->d_ccell,
                                                                 ctx->d_jrp, ctx->d_jcols, ctx->d_gslot);
@@ -492,10 +520,23 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   a.T = ctx->d_Tgen; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd; a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr;
   a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr; a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
   a.flux = ctx->d_flux; a.D = ctx->d_Dgen;
-  const unsigned gr = nblk(a.nrows, 256);
-  if (R && J) k_rows_general<true, true><<<gr, 256, 0, st>>>(a);
-  else if (R) k_rows_general<true, false><<<gr, 256, 0, st>>>(a);
-  else k_rows_general<false, true><<<gr, 256, 0, st>>>(a);
+  if (ctx->d_vblk) {
+    GenBlkArgs b;
+    b.pref = ctx->pref; b.vblk = ctx->d_vblk; b.brow = ctx->d_brow;
+    b.face_cells = ctx->d_face_cells; b.face_flags = ctx->d_face_flags; b.ccell = ctx->d_ccell; b.vdcol = ctx->d_vdcol;
+    b.T = ctx->d_Tgen; b.G = ctx->d_G; b.U = U; b.Pbnd = ctx->d_Pbnd; b.rho = ctx->d_rho; b.ukvr = ctx->d_ukvr;
+    b.drho = ctx->d_drho; b.dukvr = ctx->d_dukvr; b.rho_bnd = ctx->d_rho_bnd; b.ukvr_bnd = ctx->d_ukvr_bnd;
+    b.flux = ctx->d_flux; b.D = ctx->d_Dgen;
+    const unsigned gv = (unsigned)ctx->nv;
+    if (R && J) k_rows_general_blk<true, true><<<gv, 64, 0, st>>>(b);
+    else if (R) k_rows_general_blk<true, false><<<gv, 64, 0, st>>>(b);
+    else k_rows_general_blk<false, true><<<gv, 64, 0, st>>>(b);
+  } else {
+    const unsigned gr = nblk(a.nrows, 256);
+    if (R && J) k_rows_general<true, true><<<gr, 256, 0, st>>>(a);
+    else if (R) k_rows_general<true, false><<<gr, 256, 0, st>>>(a);
+    else k_rows_general<false, true><<<gr, 256, 0, st>>>(a);
+  }
   if (R) {
     GatherArgs ga = gather_args(ctx, form, shift, Udot, R, nullptr, false, nullptr, 0);
     k_cell_gather<true, false><<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga);

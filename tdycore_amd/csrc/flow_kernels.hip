@@ -337,6 +337,113 @@ struct GenGatherArgs {
   double *J;
 };
 
+// The same flux rows, one wavefront per vertex block: first a lane per block
+// row computes the row's flux and its scalars (T.P, mean density, upwind
+// mobility and their derivatives) into LDS, then the lanes sweep the block's
+// cell entries in memory order and write the derivatives D, so T and D are
+// read and written in contiguous runs.  Rows without a block (a triangle's
+// fourth sub-face, no-flow faces) keep the zero flux written at setup.  Same
+// arithmetic as k_rows_general, entry by entry.
+constexpr int kBlkRows = 64;  // most flux rows of one vertex block (checked at setup)
+struct GenBlkArgs {
+  double pref;
+  const int32_t *vblk;  // per vertex: T offset, rows, cells, Dirichlet columns, cell offset, Dirichlet offset, row offset
+  const int32_t *brow;  // flux row of each block row, blocks in vertex order
+  const int32_t *face_cells;
+  const uint8_t *face_flags;
+  const int32_t *ccell, *vdcol;
+  const double *T, *G, *U, *Pbnd, *rho, *ukvr, *drho, *dukvr, *rho_bnd, *ukvr_bnd;
+  double *flux, *D;
+};
+
+template <bool WITH_R, bool WITH_J>
+__global__ void __launch_bounds__(64) k_rows_general_blk(GenBlkArgs a) {
+  __shared__ double s_tp[kBlkRows], s_den[kBlkRows], s_ukvr[kBlkRows], s_G[kBlkRows];
+  __shared__ double s_ddu[kBlkRows], s_ddd[kBlkRows], s_dku[kBlkRows], s_dkd[kBlkRows];
+  __shared__ int s_up[kBlkRows], s_dn[kBlkRows], s_flag[kBlkRows];  // flag: 1 no D, 2 zero D
+  const int v = blockIdx.x, t = threadIdx.x;
+  const int32_t *d = a.vblk + (int64_t)v * 7;
+  const int ot = d[0], nrow = d[1], ncell = d[2], ndir = d[3];
+  const int32_t *cols = a.ccell + d[4];
+  const int32_t *dcols = a.vdcol + d[5];
+  const int nB = ncell + ndir;
+  if (t < nrow) {
+    const int64_t r = a.brow[d[6] + t];
+    const int f = (int)(r >> 2);
+    const uint8_t fl = a.face_flags[f];
+    int flag = 0;
+    if (fl & kFaceNoFlow) {
+      if (WITH_R) a.flux[r] = 0.0;
+      flag = 1;
+    } else {
+      const double *Tr = a.T + ot + (int64_t)t * nB;
+      double TP = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < ncell; j++) TP += Tr[j] * a.U[cols[j]];
+      for (int j = 0; j < ndir; j++) TP += Tr[ncell + j] * a.Pbnd[dcols[j]];
+      const int up = a.face_cells[2 * f], dn = a.face_cells[2 * f + 1];
+      double den = 0.0;
+      den += up >= 0 ? a.rho[up] : a.rho_bnd[-up - 1];
+      den += dn >= 0 ? a.rho[dn] : a.rho_bnd[-dn - 1];
+      den *= 0.5;
+      const double G = a.G[r];
+      const bool upwind_up = (TP + den * G < 0.0);
+      const int uw = upwind_up ? up : dn;
+      double ukvr;
+      bool zero = false;
+      if (uw >= 0) {
+        ukvr = a.ukvr[uw];
+      } else {
+        ukvr = a.ukvr_bnd[-uw - 1];
+        if ((fl & kFaceSeepage) && a.Pbnd[-uw - 1] <= a.pref) zero = true;
+      }
+      if (WITH_R) {
+        double fluxm = 0.0;
+        if (!zero) {
+          fluxm = den * ukvr * (-TP);
+          fluxm += -(den * den) * ukvr * G;
+        }
+        a.flux[r] = fluxm;
+      }
+      if (WITH_J) {
+        double dukvr_up = 0.0, dukvr_dn = 0.0;
+        if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
+        else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
+        s_tp[t] = TP; s_den[t] = den; s_ukvr[t] = ukvr; s_G[t] = G;
+        s_ddu[t] = up >= 0 ? 0.5 * a.drho[up] : 0.0;
+        s_ddd[t] = dn >= 0 ? 0.5 * a.drho[dn] : 0.0;
+        s_dku[t] = dukvr_up; s_dkd[t] = dukvr_dn;
+        s_up[t] = up; s_dn[t] = dn;
+      }
+      flag = zero ? 2 : 0;
+    }
+    if (WITH_J) s_flag[t] = flag;
+  }
+  if (!WITH_J) return;
+  __syncthreads();
+  const int ne = nrow * ncell;
+  for (int e = t; e < ne; e += 64) {
+    const int i = e / ncell, j = e - i * ncell;
+    const int flag = s_flag[i];
+    if (flag & 1) continue;
+    const int64_t q = (int64_t)ot + (int64_t)i * nB + j;
+    const int col = cols[j];
+    const int up = s_up[i], dn = s_dn[i];
+    const double TP = s_tp[i], den = s_den[i], ukvr = s_ukvr[i], G = s_G[i];
+    double Jac;
+    if (flag & 2) Jac = 0.0;
+    else if (up >= 0 && col == up)
+      Jac = s_ddu[i] * ukvr * TP + den * s_dku[i] * TP + den * ukvr * a.T[q] + 2 * den * s_ddu[i] * ukvr * G +
+            (den * den) * s_dku[i] * G;
+    else if (dn >= 0 && col == dn)
+      Jac = s_ddd[i] * ukvr * TP + den * s_dkd[i] * TP + den * ukvr * a.T[q] + 2 * den * s_ddd[i] * ukvr * G +
+            (den * den) * s_dkd[i] * G;
+    else
+      Jac = den * ukvr * a.T[q];
+    a.D[q] = -Jac;  // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
+  }
+}
+
 // Slot table of the general Jacobian gather, built once at setup: entry
 // to + j of flux row r (its cell column j) has the column's slot in the up
 // cell's Jacobian row at gslot[2*(to+j)] and in the down cell's at

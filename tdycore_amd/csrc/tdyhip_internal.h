@@ -177,6 +177,8 @@ struct tdyhip_ctx {
   int32_t *d_rtoff = nullptr, *d_rinfo = nullptr, *d_roc = nullptr, *d_rdc = nullptr;  // 4nf each
   int32_t *d_ccell = nullptr, *d_vdcol = nullptr, *d_jdiag = nullptr;
   uint8_t *d_gslot = nullptr;      // 2 per block entry: Jacobian slot in the up / down cell's row
+  int32_t *d_vblk = nullptr;       // nv*7 vertex blocks for the row kernel (nullptr: row per thread)
+  int32_t *d_brow = nullptr;       // flux row of each block row
   tdyhip::HostGeneral *hgen = nullptr;
   double setup_ms[3] = {0, 0, 0};  // general setup: G matrices, bucketed local systems, gravity
 };
