@@ -98,7 +98,7 @@ def footprint_bytes(nc, nf, nv, nbnd, rows, krylov_fp32=True, restart=30):
     workspace), from the mesh counts.  Hexahedral meshes: at most 27 Jacobian
     entries per row."""
     tv = 8  # Krylov buffers are sized for FP64 (the fallback of a failed FP32 solve)
-    ctx = 525 * nc + 601 * nf + 32 * nv + 36 * nbnd
+    ctx = 533 * nc + 609 * nf + 32 * nv + 36 * nbnd
     setup_tmp = 768 * nc + 64 * nf + 264 * nv
     # Newton vectors, block-ordered Jacobian and its columns, Krylov basis,
     # block-Jacobi tables and factor (rows = the block layout's row count)
