@@ -27,7 +27,6 @@ Conventions (kept identical to the reference where they matter):
 from __future__ import annotations
 
 import dataclasses
-from typing import Optional
 
 import numpy as np
 

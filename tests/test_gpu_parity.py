@@ -8,7 +8,6 @@ import numpy as np
 import pytest
 
 from cases import cfg0_problem, gold_problem, GOLDS
-from tdycore_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 

@@ -6,7 +6,6 @@ whole columns in layer order."""
 import numpy as np
 import scipy.sparse as sp
 
-from tdycore_amd import meshgen as mg
 from tdycore_amd import workloads
 from pyoracle import Oracle
 
