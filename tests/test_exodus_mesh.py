@@ -72,3 +72,23 @@ def test_geometry_from_connectivity_matches_prism_mesh():
     np.testing.assert_allclose(g.face_area, m.face_area, rtol=1e-12)
     np.testing.assert_allclose(g.face_normal, m.face_normal, atol=1e-12)
     assert np.array_equal(g.face_cells, m.face_cells)
+
+
+def test_extruded_surface_is_the_unit_box():
+    """The reference's 3x3 quadrilateral surface (float32 coordinates)
+    extruded into 3 layers is the 3x3x3 unit box, cell c*3 + l being box cell
+    (c % 3) + 3*(c // 3) + 9*l."""
+    m = exodus.extruded_exodus_mesh(os.path.join(HERE, "golden", "3x3_quad_surface_mesh.exo"), 3)
+    b = mg.box_mesh(3, 3, 3)
+    c = np.arange(27)
+    box_of = (c // 3) % 3 + 3 * ((c // 3) // 3) + 9 * (c % 3)
+    np.testing.assert_allclose(m.cell_cen, b.cell_cen[box_of], atol=1e-7)
+    np.testing.assert_allclose(m.cell_vol, b.cell_vol[box_of], rtol=1e-6)
+    assert (m.nf, m.nbnd) == (b.nf, b.nbnd)
+    assert abs(_closed(m)).max() < 1e-12
+
+
+def test_transect_mesh_is_closed():
+    m = exodus.exodus_mesh(os.path.join(HERE, "golden", "transect_44x1x3_hex_uniform_dz_mesh.exo"))
+    assert (m.nc, m.nbnd) == (132, 2 * 44 + 2 * 44 * 3 + 2 * 3)
+    assert np.all(m.cell_vol > 0) and abs(_closed(m)).max() < 1e-6 * m.face_area.max()
