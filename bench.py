@@ -15,11 +15,20 @@ Jacobian, factorisation and Krylov iteration included.  W warmup steps run
 from the initial state, the state is reset, then exactly K steps are timed.
 
 Printed value: Newton cells/s = cells x Newton iterations / wall time of the
-K timed steps (FP64 residual, Jacobian, Krylov).  The JSON line also carries:
-  roofline      the kernel with the largest share of device time in the timed
-                region: algorithmic bytes per launch / mean launch duration
-                (HIP events on the library's stream) against 8 TB/s;
+K timed steps (FP64 residual, Jacobian, Krylov), timed with the kernel timers
+off and each GMRES restart cycle replayed as a hipGraph.  The JSON line also
+carries:
+  roofline      from a second pass over the same K steps with HIP events
+                around every launch on the library's stream (graphs off; the
+                iterations are the same, bit for bit): the kernel with the
+                largest share of device time, its algorithmic bytes per launch
+                / mean launch duration against 8 TB/s, over the launches that
+                did work (launches past a GMRES cycle's last Arnoldi step
+                return at once and are booked apart); whole_step = all
+                kernels' algorithmic bytes / the headline wall time;
   kernels       the same figures for every kernel of the path;
+  setup         device times of the transmissibility setup phases and the
+                FP64 and HBM fractions of the batched local systems;
   fp32_krylov   the same K steps with FP32 Jacobian storage and FP32 GMRES /
                 ILU(0) (FP64 residual and Newton update), its speedup and its
                 max relative pressure difference from the FP64 run, plus an
@@ -28,9 +37,10 @@ K timed steps (FP64 residual, Jacobian, Krylov).  The JSON line also carries:
                 counts;
   one_level_fp64  with the coarse correction on (default), the same K FP64
                 steps with PETSc's default one-level block Jacobi, for reference;
-  cpu_baseline  the oracle (single-threaded C restatement of the reference
-                algorithm with serial GMRES(30)+ILU(0)) timed on this host on
-                one Newton iteration of the first step of the same mesh.
+  cfg0_latency  the 16^3 jittered cube: residual, Jacobian and per-step
+                latency (cache-resident, so latency rather than bandwidth);
+  cpu_baseline  the oracle (C restatement of the reference algorithm) timed on
+                this host's cores.
 """
 import argparse
 import json
@@ -57,7 +67,8 @@ def parse():
     ap.add_argument("--brick", type=int, nargs=3, default=[8, 8, 4], help="block-Jacobi brick (cells)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-Krylov variant")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
-    ap.add_argument("--no-timers", action="store_true", help="time without per-kernel HIP events")
+    ap.add_argument("--no-timers", action="store_true", help="skip the per-kernel timer pass")
+    ap.add_argument("--no-cfg0", action="store_true", help="skip the 16^3 latency figures")
     ap.add_argument("--prisms", action="store_true",
                     help="measure the prism-mesh transmissibility setup instead (random Delaunay, extruded)")
     ap.add_argument("--prism-tri", type=int, default=200_000, help="prisms: triangles of the 2-D mesh")
@@ -82,7 +93,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+EXITED = " (exited at once)"
+
+
 def kernel_table(rep):
+    """Per-kernel rows from the library's timer report.  Launches of a GMRES
+    cycle past its last Arnoldi step return at once on the cycle's stop flag;
+    the library books them under "<kernel> (exited at once)" with no bytes, so
+    a kernel's own row averages only launches that did its work."""
     out = []
     for r in rep:
         if r["launches"] == 0 or r["ms"] <= 0:
@@ -94,6 +112,20 @@ def kernel_table(rep):
                     "GB_s": round(avg_b / (avg_ms * 1e-3) / 1e9, 1)})
     out.sort(key=lambda r: -r["total_ms"])
     return out
+
+
+def dominant(kernels):
+    """The working kernel with the largest share of device time."""
+    return next(k for k in kernels if not k["kernel"].endswith(EXITED))
+
+
+def whole_step(rep, wall_s):
+    """All kernels' algorithmic bytes over the wall time of the same steps."""
+    total_b = sum(r["bytes"] for r in rep)
+    kernel_ms = sum(r["ms"] for r in rep)
+    gbs = total_b / wall_s / 1e9
+    return {"bytes": total_b, "wall_s": round(wall_s, 4), "GB_s": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms_timer_pass": round(kernel_ms, 2)}
 
 
 def run_steps(g, U, U0, dt, K, W, opts, timers):
@@ -145,6 +177,65 @@ def accuracy_pass(g, U0, dt, K, o64, o32):
                      "krylov_fp64": s64["linear_its"], "krylov_fp32": s32["linear_its"],
                      "fp64_fallbacks": s32["krylov_fallbacks"]})
     return rows
+
+
+def cfg0_latency(reps=200):
+    """Latency on the 16^3 jittered cube (4,096 cells, cache-resident): one
+    TS residual, one TS Jacobian, and each of the configuration's 10
+    backward-Euler SNES steps (dt = 3600 s, snes_rtol 1e-10), each call
+    timed from its launch to the end of a device synchronisation."""
+    import numpy as np
+    import torch
+    from tdycore_amd import workloads
+    from tdycore_amd.mpfao import MPFAO
+    P = workloads.jittered_cube_problem(16)
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    nc = P["mesh"].nc
+    rng = np.random.default_rng(3)
+    U = torch.tensor(P["U0"] + 1e3 * (rng.random(nc) - 0.5), device="cuda")
+    Ud = torch.tensor(1e-2 * (rng.random(nc) - 0.5), device="cuda")
+    R = torch.empty_like(U)
+    J = torch.empty(g.jac_nnz, dtype=torch.float64, device="cuda")
+
+    def lat(fn):
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ts = np.array(ts) * 1e6
+        return {"median_us": round(float(np.median(ts)), 1), "p99_us": round(float(np.percentile(ts, 99)), 1)}
+
+    res = lat(lambda: g.IFunction(0.0, U, Ud, R))
+    jac = lat(lambda: g.IJacobian(0.0, U, Ud, 1.0 / 3600.0, J))
+    opts = g.solver_options(snes_rtol=1e-10, ksp_rtol=1e-5, pc_block=64, pc_coarse=1)
+    Us = torch.tensor(P["U0"], device="cuda")
+    g.run_snes_steps(Us, 3600.0, 1, opts)  # warm-up step: workspace, graphs
+    Us = torch.tensor(P["U0"], device="cuda")
+    torch.cuda.synchronize()
+    steps, newton, lin = [], 0, 0
+    for _ in range(10):
+        t0 = time.perf_counter()
+        st = g.run_snes_steps(Us, 3600.0, 1, opts)
+        torch.cuda.synchronize()
+        steps.append(time.perf_counter() - t0)
+        newton += st["newton_its"]
+        lin += st["linear_its"]
+        if st["status"] != 0:
+            raise RuntimeError(f"16^3 step failed: {st}")
+    g.close()
+    steps = np.array(steps) * 1e3
+    return {"workload": "16x16x16 jittered unit cube (4096 cells), rotated tensor K, 10 backward-Euler SNES steps "
+                        "of 3600 s, snes_rtol 1e-10, GMRES(30) + bjacobi ILU(0) 64-row blocks + coarse correction",
+            "residual": res, "jacobian": jac,
+            "step_ms": {"median": round(float(np.median(steps)), 3), "max": round(float(steps.max()), 3),
+                        "total_10_steps": round(float(steps.sum()), 3)},
+            "newton_its": newton, "linear_its": lin,
+            "newton_iteration_ms": round(float(steps.sum()) / newton, 3)}
 
 
 def stream_copy_gbs(nbytes=1 << 30, reps=10):
@@ -407,20 +498,29 @@ def main():
     U0 = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
     U = U0.clone()
     timers = not a.no_timers
-
     pcc = 1 if a.pc_coarse else 0
     o64 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=0, pc_coarse=pcc)
-    el, st, rep = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o64, timers)
+    # headline: kernel timers off, GMRES cycles replayed as hipGraphs
+    el, st, _ = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o64, False)
     if st["status"] != 0:
         raise RuntimeError(f"FP64 Newton stepping failed: {st}")
     log(f"fp64: {a.steps} steps {el:.3f} s newton {st['newton_its']} linear {st['linear_its']}")
     value = nc * st["newton_its"] / el
-    kernels = kernel_table(rep) if rep else []
+    U64 = U.clone()
+    # per-kernel figures from a second pass over the same steps with HIP events
+    # around every launch (graphs off); the iterations are the same, bit for bit
+    kernels, roof, wstep, timer_pass = [], None, None, None
     copy_gbs = stream_copy_gbs()
     log(f"stream copy {copy_gbs:.0f} GB/s")
-    roof = None
-    if kernels:
-        k0 = kernels[0]
+    if timers:
+        elt, stt, rep = run_steps(g, U, U0, a.dt, a.steps, 0, o64, True)
+        same = (stt["newton_its"], stt["linear_its"]) == (st["newton_its"], st["linear_its"])
+        timer_pass = {"wall_s": round(elt, 4), "same_iterations": same,
+                      "same_state": bool(torch.equal(U, U64))}
+        log(f"timer pass: {elt:.3f} s, same iterations {same}")
+        kernels = kernel_table(rep)
+        wstep = whole_step(rep, el)
+        k0 = dominant(kernels)
         ach = k0["GB_s"]
         traffic = None
         if (nx, ny, nz) == (128, 128, 64) and os.path.exists(TRAFFIC_FILE):
@@ -428,18 +528,32 @@ def main():
             traffic = pmc["traffic_bytes"] if pmc else None
         roof = {"bound": "hbm", "kernel": k0["kernel"], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": k0["bytes_per_launch"], "avg_us": k0["avg_us"],
-                "stream_copy_GB_s": round(copy_gbs, 1), "frac_of_copy": round(ach / copy_gbs, 4)}
-    U64 = U.clone()
+                "bytes_per_launch": k0["bytes_per_launch"], "avg_us": k0["avg_us"], "launches": k0["launches"],
+                "stream_copy_GB_s": round(copy_gbs, 1), "frac_of_copy": round(ach / copy_gbs, 4),
+                "whole_step": wstep}
+
+    info = g.setup_info()
+    ls_s = info["local_systems_ms"] * 1e-3
+    setup = {"gmatrix_ms": round(info["gmatrix_ms"], 3), "local_systems_ms": round(info["local_systems_ms"], 3),
+             "gravity_ms": round(info["gravity_ms"], 3), "regions": info["buckets"][0],
+             "local_systems_fp64_TFLOP_s": round(info["local_systems_flops"] / ls_s / 1e12, 3) if ls_s else None,
+             "local_systems_fp64_frac": round(info["local_systems_flops"] / ls_s / 78.6e12, 4) if ls_s else None,
+             "local_systems_GB_s": round(info["local_systems_bytes"] / ls_s / 1e9, 1) if ls_s else None,
+             "local_systems_hbm_frac": round(info["local_systems_bytes"] / ls_s / 8e12, 4) if ls_s else None,
+             "wall_s": round(setup_s, 3)}
 
     fp32 = None
     if not a.no_fp32:
         o32 = g.solver_options(snes_rtol=1e-10, pc_block=bs, krylov_fp32=1, pc_coarse=pcc)
-        el32, st32, rep32 = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o32, timers)
+        el32, st32, _ = run_steps(g, U, U0, a.dt, a.steps, a.warmup, o32, False)
         if st32["status"] != 0:
             raise RuntimeError(f"FP32-Krylov Newton stepping failed: {st32}")
         rel = ((U - U64).abs() / U64.abs().clamp_min(1e-300)).max().item()
         v32 = nc * st32["newton_its"] / el32
+        k32 = []
+        if timers:
+            _, _, rep32 = run_steps(g, U, U0, a.dt, a.steps, 0, o32, True)
+            k32 = kernel_table(rep32)[:8]
         per_step = accuracy_pass(g, U0, a.dt, a.steps, o64, o32)
         log("per-step FP32 vs FP64: max rel p diff %.2e, mass balance fp64 %.2e fp32 %.2e" % (
             max(r["max_rel_p_diff"] for r in per_step), max(r["mass_balance_fp64"] for r in per_step),
@@ -447,7 +561,7 @@ def main():
         fp32 = {"value": v32, "ms_per_step": 1e3 * el32 / a.steps, "speedup_vs_fp64": round(v32 / value, 3),
                 "wall_speedup_vs_fp64": round(el / el32, 3),
                 "newton_its": st32["newton_its"], "linear_its": st32["linear_its"], "max_rel_p_diff_vs_fp64": rel,
-                "kernels": kernel_table(rep32)[:6] if rep32 else [], "per_step": per_step}
+                "kernels": k32, "per_step": per_step}
         log(f"fp32 krylov: {el32:.3f} s newton {st32['newton_its']} linear {st32['linear_its']} rel {rel:.3e}")
 
     one_level = None
@@ -458,6 +572,14 @@ def main():
                      "newton_its": st1["newton_its"], "linear_its": st1["linear_its"],
                      "note": "FP64, PETSc-default one-level block-Jacobi ILU(0), no kernel timers"}
         log(f"one-level fp64: {el1:.3f} s newton {st1['newton_its']} linear {st1['linear_its']}")
+    g.close()
+    del g
+
+    small = None
+    if not a.no_cfg0:
+        small = cfg0_latency()
+        log(f"16^3 latency: residual {small['residual']['median_us']} us, Jacobian {small['jacobian']['median_us']} us, "
+            f"step {small['step_ms']['median']} ms")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -475,8 +597,9 @@ def main():
                                + (" + coarse correction" if pcc else ""),
                    "pc_coarse": pcc,
                    "cells": nc, "dt": a.dt, "newton_its": st["newton_its"], "linear_its": st["linear_its"],
-                   "setup_s": round(setup_s, 3), "kernel_timers": timers},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "fp32_krylov": fp32, "one_level_fp64": one_level,
+                   "setup_s": round(setup_s, 3), "headline_kernel_timers": False, "hip_graphs": True},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "timer_pass": timer_pass, "setup": setup,
+        "fp32_krylov": fp32, "one_level_fp64": one_level, "cfg0_latency": small,
     }
     print(json.dumps(line), flush=True)
 

@@ -205,15 +205,18 @@ int tdyhip_trans_nnz(const tdyhip_ctx *ctx, int64_t *nnz);
 int tdyhip_trans_csr(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals);
 int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
 /* Setup of a general mesh (any cell other than a hexahedron: right prisms, given in the
- * hexahedral arrays padded with -1).  Replaces ComputeTransmissibilityMatrix and
- * ComputeGravityDiscretization for vertices of any degree (src/fv/mpfao/tdympfao.c:1801-1860,
- * 1126-1513, 2169-2274); tdyhip_create runs it.  Every other call works on such a context
- * (rows of up to 64 Jacobian entries); its Newton solver needs a block-Jacobi partition with
- * at most 27 in-block entries per row (tdyhip_set_pc_partition, e.g. vertical columns).  ms[3] = device times of the G matrices, the size-bucketed local
- * systems and the gravity term (HIP events, general meshes only); counts[11] = interaction
- * regions in the 16-, 32- and 64-lane buckets, the most cells and faces at one vertex, the
- * transmissibility entries, the algorithmic FP64 flops and bytes of the local systems, and
- * the team width the autotuner chose for each bucket (0 for an empty bucket). */
+ * hexahedral arrays padded with -1; and hexahedral meshes with more than 8 cells, 12 faces or
+ * 8 transmissibility columns at a vertex or more than 27 Jacobian entries in a row).  Replaces
+ * ComputeTransmissibilityMatrix and ComputeGravityDiscretization for vertices of any degree
+ * (src/fv/mpfao/tdympfao.c:1801-1860, 1126-1513, 2169-2274); tdyhip_create runs it.  Every other
+ * call works on such a context (rows of up to 64 Jacobian entries); its Newton solver needs a
+ * block-Jacobi partition with at most 27 in-block entries per row (tdyhip_set_pc_partition, e.g.
+ * vertical columns).
+ * Setup report of any mesh: ms[3] = device times (HIP events) of the G matrices, the local
+ * systems and the gravity term; counts[11] = interaction regions in the 16-, 32- and 64-lane
+ * buckets (hexahedral meshes: all in the first), the most cells and faces at one vertex, the
+ * transmissibility entries, the algorithmic FP64 flops and bytes of the local systems, and the
+ * team width used for each bucket (0 for an empty bucket). */
 int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms /* 3 */, int64_t *counts /* 11 */);
 
 #ifdef __cplusplus

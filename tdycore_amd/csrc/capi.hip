@@ -99,16 +99,46 @@ static void kt_end(tdyhip_ctx *ctx, hipStream_t st, const char *name, double byt
   KernelTimers::Rec &r = ctx->kt.get(name);
   r.launches++;
   r.bytes += bytes;
-  r.pending.emplace_back(e0, e1);
+  r.pending.push_back({e0, e1, ctx->kt.tag, bytes});
+}
+// A GMRES cycle's launches are settled once its step count is known: those
+// tagged with a step >= `working` returned at once on the stop flag and move
+// to "<kernel> (exited at once)" with no bytes; the rest lose their tag.
+static void kt_settle(tdyhip_ctx *ctx, int working) {
+  if (!ctx->kt.on) return;
+  std::vector<std::pair<std::string, KernelTimers::Pending>> moved;
+  for (auto &nr : ctx->kt.recs) {
+    auto &pend = nr.second.pending;
+    size_t keep = 0;
+    for (size_t i = 0; i < pend.size(); i++) {
+      KernelTimers::Pending pe = pend[i];
+      if (pe.tag >= working) {
+        nr.second.launches--;
+        nr.second.bytes -= pe.bytes;
+        pe.tag = -1;
+        pe.bytes = 0.0;
+        moved.emplace_back(nr.first + " (exited at once)", pe);
+      } else {
+        pe.tag = -1;
+        pend[keep++] = pe;
+      }
+    }
+    pend.resize(keep);
+  }
+  for (auto &mp : moved) {
+    KernelTimers::Rec &r = ctx->kt.get(mp.first);
+    r.launches++;
+    r.pending.push_back(mp.second);
+  }
 }
 static void kt_collect(tdyhip_ctx *ctx) {
   for (auto &nr : ctx->kt.recs) {
     for (auto &pe : nr.second.pending) {
       float ms = 0.f;
-      if (hipEventSynchronize(pe.second) == hipSuccess && hipEventElapsedTime(&ms, pe.first, pe.second) == hipSuccess)
+      if (hipEventSynchronize(pe.e1) == hipSuccess && hipEventElapsedTime(&ms, pe.e0, pe.e1) == hipSuccess)
         nr.second.ms += ms;
-      ctx->kt.pool.push_back(pe.first);
-      ctx->kt.pool.push_back(pe.second);
+      ctx->kt.pool.push_back(pe.e0);
+      ctx->kt.pool.push_back(pe.e1);
     }
     nr.second.pending.clear();
   }
@@ -612,12 +642,21 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   *out = nullptr;
   if (m->nc <= 0 || m->nf <= 0 || m->nv <= 0 || m->nbnd < 0) { set_error("empty mesh"); return TDYHIP_ERR_ARG; }
   if ((int64_t)m->nf * 4 * kMaxCols > ((int64_t)1 << 40)) { set_error("mesh too large"); return TDYHIP_ERR_ARG; }
-  // hexahedral meshes take the full path; other cells (prisms) the general setup only
-  const bool hex = mesh_is_hex(*m);
+  // Hexahedral meshes take the fixed-size kernels (at most 8 cells, 12 faces and
+  // 8 transmissibility columns at a vertex, 27 Jacobian entries in a row);
+  // other cells (prisms), and hexahedral meshes beyond those limits, take the
+  // general path (regions of up to 64 unknowns, rows of up to 64 entries).
+  bool hex = mesh_is_hex(*m);
   HostTopology topo;
   if (hex) {
     std::string msg = build_topology(*m, topo);
-    if (!msg.empty()) { set_error(msg); return TDYHIP_ERR_ARG; }
+    if (!msg.empty() && !topo.over_limit) { set_error(msg); return TDYHIP_ERR_ARG; }
+    if (!msg.empty()) {
+      hex = false;
+    } else {
+      for (int c = 0; c < m->nc && hex; c++) hex = topo.jrp[c + 1] - topo.jrp[c] <= kJacSlots;
+    }
+    if (!hex) topo = HostTopology();
   }
 
   tdyhip_ctx *ctx = new tdyhip_ctx();
@@ -733,15 +772,25 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   hipStream_t st = ctx->stream;
   k_fill<<<nblk((int64_t)kMaxCols * nrows, 256), 256, 0, st>>>((int64_t)kMaxCols * nrows, ctx->d_T, 0.0);
   if (hipMemsetAsync(d_status, 0, sizeof(int), st) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
+  // device times of the three setup phases (G matrices, local systems, gravity term)
+  hipEvent_t sev[6] = {};
+  for (auto &e : sev)
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  auto rec = [&](int i) { if (sev[i]) (void)hipEventRecord(sev[i], st); };
+  rec(0);
   k_gmatrix<<<nblk((int64_t)nc * 8, 256), 256, 0, st>>>(nc, ctx->gmatrix, d_scf, ctx->d_face_cells, d_cc, d_fc, d_fa,
                                                         d_fn, d_K, nullptr, d_Gsub);
-  KT(ctx, st, "k_local_systems", (double)nv * sizeof(IRDesc) + (double)nc * 72 * 8 + (double)ctx->trans_nnz * 8,
-     k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
-                                                                                 d_status));
+  rec(1);
+  rec(2);
+  k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
+                                                                               d_status);
+  rec(3);
   k_gather_table<<<nblk(nc, 256), 256, 0, st>>>(nc, ctx->d_cell_faces, ctx->d_face_cells, ctx->d_face_verts,
                                                 ctx->d_face_flags, d_rul, ctx->d_vcols, d_jsl, ctx->d_gtab);
+  rec(4);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
                                            d_fc, d_fa, d_fn, d_K, nullptr, ctx->d_G);
+  rec(5);
   // boundary rho, mu frozen at (Pref, Tref) (AllocateMemoryForBoundaryValues, tdympfao.c:297-336)
   {
     double rb, d1, d2;
@@ -750,7 +799,25 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
     set_error("setup kernels failed");
+    for (auto &e : sev)
+      if (e) (void)hipEventDestroy(e);
     return tmp_fail(TDYHIP_ERR_HIP);
+  }
+  for (int i = 0; i < 3; i++) {
+    float ms = 0.f;
+    if (sev[2 * i] && sev[2 * i + 1] && hipEventElapsedTime(&ms, sev[2 * i], sev[2 * i + 1]) == hipSuccess)
+      ctx->setup_ms[i] = ms;
+  }
+  for (auto &e : sev)
+    if (e) (void)hipEventDestroy(e);
+  // algorithmic work of the local systems (the same model as the general path's)
+  for (int v = 0; v < nv; v++) {
+    const IRDesc &d = topo.ir[v];
+    const double nU = d.nint + d.nneu, nB = d.ncell + d.ndir, nrow = d.nint + d.ndir;
+    ctx->ls_flops += 2.0 / 3.0 * nU * nU * nU + 3.0 * nU * nU * nB + 6.0 * nrow * nB;
+    ctx->ls_bytes += (double)sizeof(IRDesc) + d.ncell * (3 * 9 * 8.0) + nrow * nB * 8.0;
+    ctx->ir_max_cells = std::max(ctx->ir_max_cells, (int)d.ncell);
+    ctx->ir_max_faces = std::max(ctx->ir_max_faces, (int)d.nface);
   }
   int status = 0;
   if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
@@ -976,6 +1043,15 @@ int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms, int64_t *counts) {
     counts[6] = (int64_t)fl;
     counts[7] = (int64_t)by;
     for (int b = 0; b < 3; b++) counts[8 + b] = g.lanes[b];
+  } else {
+    // hexahedral meshes: one 16-lane team per region (k_local_systems)
+    counts[0] = ctx->nv;
+    counts[3] = ctx->ir_max_cells;
+    counts[4] = ctx->ir_max_faces;
+    counts[5] = ctx->trans_nnz;
+    counts[6] = (int64_t)ctx->ls_flops;
+    counts[7] = (int64_t)ctx->ls_bytes;
+    counts[8] = kTeam;
   }
   return 0;
 }

@@ -71,6 +71,8 @@ struct GmresDev {
   int its;         // total Arnoldi steps
   int jcur;        // basis vectors completed in this cycle
   int maxit;
+  int finished;    // the solve is over: converged, iteration cap, or a non-finite residual
+  int diverged;    // a non-finite residual estimate (KSP_DIVERGED_NANORINF)
   int pad;
 };
 
@@ -349,9 +351,12 @@ __device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *
   gd->jcur = j + 1;
   gd->resid = fabs(g[j + 1]);
   gd->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
-  const int conv = gd->resid <= gd->tol || hn == 0.0;
+  const bool bad = !isfinite(gd->resid) || !isfinite(hn);
+  const int conv = !bad && (gd->resid <= gd->tol || hn == 0.0);
   gd->converged = conv;
-  if (conv || gd->its >= gd->maxit || j + 1 == m) gd->done = 1;
+  if (bad) gd->diverged = 1;
+  if (conv || bad || gd->its >= gd->maxit) gd->finished = 1;
+  if (gd->finished || j + 1 == m) gd->done = 1;
 }
 
 // ---------------------------------------------------------------- fused Arnoldi matvec
@@ -640,6 +645,8 @@ __global__ void k_gmres_init(GmresDev *gd, int *status, int maxit) {
   gd->its = 0;
   gd->done = 0;
   gd->converged = 0;
+  gd->finished = 0;
+  gd->diverged = 0;
   gd->jcur = 0;
   gd->maxit = maxit;
   *status = 0;
@@ -655,9 +662,12 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
   gd->resid = beta;
   gd->jcur = 0;
   gd->inv_h = beta > 0.0 ? 1.0 / beta : 0.0;
-  const int conv = !(beta > gd->tol);
+  const bool bad = !isfinite(beta);
+  const int conv = !bad && !(beta > gd->tol);
   gd->converged = conv;
-  gd->done = conv || gd->its >= gd->maxit;
+  if (bad) gd->diverged = 1;
+  gd->finished = conv || bad || gd->its >= gd->maxit;
+  gd->done = gd->finished;
 }
 
 __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,

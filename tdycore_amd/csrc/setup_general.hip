@@ -15,8 +15,9 @@
 // same order, as LU factorisation followed by the permuted forward solve),
 // back-substitutes one column of B per lane, and writes the region's T block
 // (rows: internal faces, then Dirichlet faces; columns: cells, then Dirichlet
-// faces) row-major, one column per lane.  The sums follow the oracle's order,
-// so only fused multiply-adds separate the results.
+// faces) row-major, one column per lane.  The sums follow the oracle's order
+// and nothing here is contracted into fused multiply-adds, so the results
+// agree with the oracle's bit for bit.
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
@@ -45,6 +46,7 @@ __device__ inline void team_argmax(double &v, int &i) {
 // F = -G0 - G1 - G2 (ComputeCandFmatrix, tdympfao.c:1000-1076).
 __device__ inline double gen_crow(const GenArgs &a, int oc, int of, int nU, int i, int l, int (&col)[3],
                                   double (&val)[3]) {
+#pragma clang fp contract(off)
   const double *G = a.Gsub + (size_t)a.csub[oc + i] * 9 + l * 3;
   const int pk = a.csck[oc + i];
   for (int m = 0; m < 3; m++) {
@@ -68,6 +70,7 @@ __device__ inline double crow_at(const int (&col)[3], const double (&val)[3], in
 
 template <int LANES>
 __global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
+#pragma clang fp contract(off)
   // [A | B] of a region in LDS, rows of a.pitch doubles (odd: column reads
   // spread over the banks), sized by the bucket's largest region
   extern __shared__ double lds[];

@@ -1,5 +1,12 @@
 // Transmissibility setup on the device (subsystem (a)).
 //
+// Every setup kernel and helper below is compiled without floating-point
+// contraction (no fused multiply-adds) and performs the oracle's operations
+// in the oracle's order (multipliers as a * (1 / pivot), the sums of C X in
+// column order, the sub-cell volume as |det| / 6 * 6), so the results agree
+// bit for bit, also on high-contrast meshes whose local systems amplify any
+// rounding difference by their condition number.
+//
 //  k_gmatrix        one thread per subcell: the 3x3 sub-cell G matrix
 //                   (ComputeGMatrix_MPFAO / _TPF, src/fv/mpfao/tdympfao.c:39-126, 128-274)
 //  k_local_systems  one 16-lane team per vertex: assemble the interaction-region system
@@ -12,6 +19,7 @@
 namespace tdyhip {
 
 __device__ inline void cross3(const double *a, const double *b, double *c) {
+#pragma clang fp contract(off)
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
@@ -22,6 +30,7 @@ __global__ void k_gmatrix(int nc, int gmatrix, const int32_t *__restrict__ sc_fa
                           const double *__restrict__ face_cen, const double *__restrict__ face_area,
                           const double *__restrict__ face_normal, const double *__restrict__ K0,
                           const uint8_t *__restrict__ face_nv, double *__restrict__ Gsub) {
+#pragma clang fp contract(off)
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= nc * 8) return;
   const int c = sc >> 3;
@@ -36,7 +45,8 @@ __global__ void k_gmatrix(int nc, int gmatrix, const int32_t *__restrict__ sc_fa
     for (int d = 0; d < 3; d++) e[k][d] = face_cen[3 * fid[k] + d] - cc[d];
   double x[3];
   cross3(e[0], e[1], x);
-  const double Tsub = fabs(x[0] * e[2][0] + x[1] * e[2][1] + x[2] * e[2][2]);
+  // six times the volume of the tetrahedron (cell centroid, three face centroids)
+  const double Tsub = fabs(x[0] * e[2][0] + x[1] * e[2][1] + x[2] * e[2][2]) / 6.0 * 6.0;
   double nu[3][3];
   for (int k = 0; k < 3; k++) cross3(e[(k + 1) % 3], e[(k + 2) % 3], nu[k]);
   double *G = Gsub + (size_t)sc * 9;
@@ -84,6 +94,7 @@ __global__ void k_gmatrix(int nc, int gmatrix, const int32_t *__restrict__ sc_fa
 // (ComputeCandFmatrix, tdympfao.c:1000-1076).
 __device__ inline double crow(const IRDesc &d, const double *__restrict__ Gsub, int nU, int i, int l,
                               double *out) {
+#pragma clang fp contract(off)
   const double *G = Gsub + (size_t)d.subcell[i] * 9 + l * 3;
   for (int q = 0; q < nU + d.ndir; q++) out[q] = 0.0;
   for (int m = 0; m < 3; m++) {
@@ -109,6 +120,7 @@ __device__ inline double team_bcast(double x, int src) { return __shfl(x, src, k
 __global__ void __launch_bounds__(kTeam * kTeamsPerBlock)
     k_local_systems(int nv, const IRDesc *__restrict__ irs, const double *__restrict__ Gsub, double *__restrict__ T,
                     int64_t nrows, int *__restrict__ status) {
+#pragma clang fp contract(off)
   __shared__ double Xs[kTeamsPerBlock][kMaxIRFaces][kMaxCols];
   const int team = threadIdx.x / kTeam, lane = threadIdx.x % kTeam;
   const int v = blockIdx.x * kTeamsPerBlock + team;
@@ -144,6 +156,10 @@ __global__ void __launch_bounds__(kTeam * kTeamsPerBlock)
     }
   }
   const double gus = gu[0] + gu[1] + gu[2], gds = gd[0] + gd[1] + gd[2];
+  // the up-side pairs in column order: C X sums its terms by column
+  if (cu[0] > cu[1]) { int t = cu[0]; cu[0] = cu[1]; cu[1] = t; double w = gu[0]; gu[0] = gu[1]; gu[1] = w; }
+  if (cu[1] > cu[2]) { int t = cu[1]; cu[1] = cu[2]; cu[2] = t; double w = gu[1]; gu[1] = gu[2]; gu[2] = w; }
+  if (cu[0] > cu[1]) { int t = cu[0]; cu[0] = cu[1]; cu[1] = t; double w = gu[0]; gu[0] = gu[1]; gu[1] = w; }
   // equation of an internal (role 0) or no-flow (role 1) face: A row and B row
   const bool eq = has_face && role != 2;
   double a[kMaxIRFaces], b[kMaxCols];
@@ -195,7 +211,8 @@ __global__ void __launch_bounds__(kTeam * kTeamsPerBlock)
       const double pk = team_bcast(a[k], who);
       if (lane == who) kpiv = k;
       const bool elim = eq && kpiv < 0;
-      const double l = elim ? a[k] / pk : 0.0;
+      const double inv = 1.0 / pk;
+      const double l = elim ? a[k] * inv : 0.0;
 #pragma unroll
       for (int j = k + 1; j < kMaxIRFaces; j++) {
         const double pj = team_bcast(a[j], who);
@@ -253,6 +270,7 @@ __global__ void __launch_bounds__(kTeam * kTeamsPerBlock)
 }
 
 __device__ inline void inv3(const double *a, double *r) {
+#pragma clang fp contract(off)
   const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
                      a[2] * (a[3] * a[7] - a[4] * a[6]);
   r[0] = +1.0 / det * (a[4] * a[8] - a[7] * a[5]);
@@ -271,6 +289,7 @@ __global__ void k_gravdis(int nf, int gmatrix, double gravity, const int32_t *__
                           const double *__restrict__ face_cen, const double *__restrict__ face_area,
                           const double *__restrict__ face_normal, const double *__restrict__ K0,
                           const uint8_t *__restrict__ face_nv, double *__restrict__ Gout) {
+#pragma clang fp contract(off)
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   double val = 0.0;

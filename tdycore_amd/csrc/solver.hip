@@ -11,6 +11,7 @@
 
 #include "krylov.hip"
 #include "coarse.hip"
+#include "linesearch.h"
 #include "tdyhip_internal.h"
 
 namespace tdyhip {
@@ -525,7 +526,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
 
 static int dev_dot(tdyhip_ctx *ctx, const double *x, const double *y, double *out, hipStream_t st) {
   SolverWS &ws = *ctx->ws;
-  k_dot<double, double><<<ws.nchunk, kRedBlock, 0, st>>>(ws.n, x, y, ws.dpartial);
+  KT(ctx, st, "k_dot", (double)ws.n * (x == y ? 8 : 16), k_dot<double, double><<<ws.nchunk, kRedBlock, 0, st>>>(ws.n, x, y, ws.dpartial));
   k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
   SCHK(hipMemcpyAsync(out, ws.h, sizeof(double), hipMemcpyDeviceToHost, st));
   SCHK(hipStreamSynchronize(st));
@@ -586,7 +587,8 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   const int *stop = &gd->done;
   const unsigned g256 = (unsigned)((N + 255) / 256);
   if (!std::is_same<TV, double>::value)
-    k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk);
+    KT(ctx, st, "k_convert_J", (double)ws.nnz * (8 + sizeof(TV)),
+       k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk));
   k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
   if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
   else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
@@ -694,7 +696,11 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       // (At 1M rows, 2048 partials per workgroup cost as much as they save.)
       const NormArgs na{part, nch, j - 1, m, ws.h, dH, dcs, dsn, dg};
       const bool fuse_norm = nch <= kFuseNormChunks;
+      // timer tags: the launches of step j work only if the cycle reached
+      // column j; a fused matvec also finishes column j-1, so it counts with it
+      ctx->kt.tag = (fuse_norm && j > 0) ? j - 1 : j;
       spmv_apply(u, V + (int64_t)j * N, o, stop, (fuse_norm && j > 0) ? &na : nullptr);
+      ctx->kt.tag = j;
       // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
       // kernel per reduction with a last-block hand-off is much slower here:
       // the device-scope fences it needs write back the per-XCD L2s.)
@@ -706,8 +712,10 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
       KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
       if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
     }
+    ctx->kt.tag = -1;
     k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
-    k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec);
+    // (booked at the solution's read and write only: the basis vectors it reads depend on the cycle's length)
+    KT(ctx, st, "k_gemv_update", (double)N * 16, k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec));
   };
   // Replay the cycle as a hipGraph (captured on first use) unless the kernel
   // timers are on (they time every launch) or the vectors are not the
@@ -730,12 +738,13 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     SCHK(hipMemcpyAsync(&hg, gd, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
     if (cycle == 0) SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
+    kt_settle(ctx, hg.jcur);
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
-    if (hg.converged || hg.its >= o.ksp_max_it) break;
+    if (hg.finished) break;
     if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
     // restart: w = M^-1 (b - A x), the residual in FP64
     spmv_blk64(ctx, ws, yvec, ws.JY, st);
-    k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z);
+    KT(ctx, st, "k_sub_to", (double)N * (16 + sizeof(TV)), k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z));
     pc_apply(z, w, nullptr);
     norm2_to_h0(w);
     k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 0);
@@ -744,6 +753,7 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
   *its_out = hg.its;
   // KSP_DIVERGED_ITS: PETSc's SNES counts it as a linear-solve failure
   // (SNESCheckKSPSolve, maxLinearSolveFailures 1 -> SNES_DIVERGED_LINEAR_SOLVE)
+  if (hg.diverged) { set_error("GMRES residual is not finite"); return TDYHIP_ERR_NOCONV; }
   if (!hg.converged) { set_error("GMRES reached ksp_max_it"); return TDYHIP_ERR_NOCONV; }
   return 0;
 }
@@ -813,14 +823,14 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
   SolverWS &ws = *ctx->ws;
   const int n = ctx->nc;
   const unsigned g = nblk(n, 256);
-  const double alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8, divtol = 1e4;
+  const double alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8, divtol = 1e4;  // PETSc 3.17 defaults
   int rc;
   if ((rc = eval_F(ctx, fm, X, ws.F, st))) return rc;
   double fnorm;
   if ((rc = dev_norm(ctx, ws.F, &fnorm, st))) return rc;
   const double fnorm0 = fnorm, ttol = o.snes_rtol * fnorm;
   int status = 0, it = 0;
-  if (!(fnorm == fnorm)) { stats.status = -4; return 0; }
+  if (!std::isfinite(fnorm)) { stats.status = -4; return 0; }  // SNES_DIVERGED_FNORM_NAN
   if (fnorm < o.snes_atol) { stats.status = 0; stats.last_fnorm = fnorm; return 0; }
   for (it = 1; it <= o.snes_max_it; it++) {
     if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
@@ -832,52 +842,42 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
     if (rc) return rc;
     double ynorm;
     if ((rc = dev_norm(ctx, ws.Y, &ynorm, st))) return rc;
+    // a non-finite Newton step is a failed linear solve (KSP_DIVERGED_NANORINF)
+    if (!std::isfinite(ynorm)) { status = -3; break; }
     if (ynorm == 0.0) break;
     if (ynorm > maxstep) {
       k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
       ynorm = maxstep;
     }
     spmv_blk64(ctx, ws, ws.Y, ws.JY, st);
-    double initslope;
-    if ((rc = dev_dot(ctx, ws.F, ws.JY, &initslope, st))) return rc;
-    if (initslope > 0.0) initslope = -initslope;
-    if (initslope == 0.0) initslope = -1.0;
-    double lambda = 1.0, gnorm;
-    k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
-    if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
-    if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
-    if (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
-      double lambdaprev = lambda, gnormprev = gnorm;
-      double lt = -initslope / (gnorm * gnorm - fnorm * fnorm - 2.0 * lambda * initslope);
-      if (lt > 0.5 * lambda) lt = 0.5 * lambda;
-      lambda = (lt <= 0.1 * lambda) ? 0.1 * lambda : lt;
-      k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
-      if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
-      if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
-      bool failed = false;
-      while (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
-        if (lambda <= minlambda) { failed = true; break; }
-        const double t1 = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lambda * initslope;
-        const double t2 = 0.5 * (gnormprev * gnormprev - fnorm * fnorm) - lambdaprev * initslope;
-        const double a = (t1 / (lambda * lambda) - t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
-        const double bb = (-lambdaprev * t1 / (lambda * lambda) + lambda * t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
-        double d = bb * bb - 3 * a * initslope;
-        if (d < 0.0) d = 0.0;
-        lt = (a == 0.0) ? -initslope / (2.0 * bb) : (-bb + std::sqrt(d)) / (3.0 * a);
-        lambdaprev = lambda; gnormprev = gnorm;
-        if (lt > 0.5 * lambda) lt = 0.5 * lambda;
-        lambda = (lt <= 0.1 * lambda) ? 0.1 * lambda : lt;
-        k_waxpy<<<g, 256, 0, st>>>(n, lambda, X, ws.Y, ws.W);
-        if ((rc = eval_F(ctx, fm, ws.W, ws.G, st))) return rc;
-        if ((rc = dev_norm(ctx, ws.G, &gnorm, st))) return rc;
-      }
-      if (failed) {
-        double xnorm;
-        if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
-        // SNES_CONVERGED_SNORM_RELATIVE when the step is negligible
-        status = (ynorm < o.snes_stol * xnorm) ? 0 : -6;
-        break;
-      }
+    double dfjy;
+    if ((rc = dev_dot(ctx, ws.F, ws.JY, &dfjy, st))) return rc;
+    const double initslope = bt_initslope(dfjy);
+    // backtracking line search from the full step; a NaN or Inf residual ends
+    // it as a failure (SNES_LINESEARCH_FAILED_NANORINF)
+    auto trial = [&](double lam, double *gn) -> int {
+      KT(ctx, st, "k_waxpy", (double)n * 24, k_waxpy<<<g, 256, 0, st>>>(n, lam, X, ws.Y, ws.W));
+      int r2 = eval_F(ctx, fm, ws.W, ws.G, st);
+      return r2 ? r2 : dev_norm(ctx, ws.G, gn, st);
+    };
+    auto stop = [&]() -> int {
+      if (!ctx->cancel.load(std::memory_order_relaxed)) return 0;
+      set_error("solve cancelled");
+      return TDYHIP_ERR_CANCELLED;
+    };
+    BTParams btp;
+    btp.alpha = alpha;
+    btp.minlambda = minlambda;
+    BTResult bt;
+    if ((rc = bt_search(fnorm, initslope, btp, false, 0.0, trial, stop, bt))) return rc;
+    const double lambda = bt.lambda, gnorm = bt.gnorm;
+    if (!bt.accepted) {
+      double xnorm;
+      if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
+      // SNES_CONVERGED_SNORM_RELATIVE when the step is negligible, else
+      // SNES_DIVERGED_LINE_SEARCH (X keeps the last accepted iterate)
+      status = (ynorm < o.snes_stol * xnorm) ? 0 : -6;
+      break;
     }
     SCHK(hipMemcpyAsync(X, ws.W, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     std::swap(ws.F, ws.G);
@@ -885,7 +885,7 @@ static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solve
     double xnorm;
     if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
     ynorm *= lambda;
-    if (!(fnorm == fnorm)) { status = -4; break; }
+    if (!std::isfinite(fnorm)) { status = -4; break; }
     if (fnorm < o.snes_atol) break;
     if (fnorm <= ttol) break;
     if (ynorm < o.snes_stol * xnorm) break;

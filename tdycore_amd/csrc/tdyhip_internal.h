@@ -48,9 +48,12 @@ struct HostTopology {
   std::vector<int32_t> jrp, jcols, jdiag;
   std::vector<int32_t> bnd_cell;  // nbnd
   std::vector<uint8_t> face_flags;// nf
+  bool over_limit = false;        // set with the error when a vertex exceeds the fixed-size limits above
 };
 
-// Builds the host topology; returns an error message or empty string.
+// Builds the host topology; returns an error message or empty string.  A mesh
+// that only exceeds the hexahedral kernels' fixed limits (over_limit) is
+// valid: it takes the general path.
 std::string build_topology(const tdyhip_mesh &m, HostTopology &t);
 
 // Interaction regions of a general mesh (hexahedra and right prisms, padded
@@ -93,18 +96,29 @@ void set_error(const std::string &msg);
 struct SolverWS;
 
 // Per-kernel timers: HIP events around each launch on the launching stream,
-// plus the launch's algorithmic bytes (each array counted once).
+// plus the launch's algorithmic bytes (each array counted once).  Launches
+// inside a GMRES restart cycle carry the Arnoldi step they belong to (tag);
+// once the cycle's step count is known, the launches of steps past it (which
+// returned at once on the cycle's stop flag) are moved to a record of their
+// own, "<kernel> (exited at once)", with no bytes, so that a kernel's record
+// holds only launches that did its work.
 struct KernelTimers {
   bool on = false;
+  int tag = -1;  // Arnoldi step of the launches being recorded (-1: none)
+  struct Pending {
+    hipEvent_t e0, e1;
+    int tag;
+    double bytes;
+  };
   struct Rec {
     int64_t launches = 0;
     double bytes = 0.0;
     double ms = 0.0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<Pending> pending;
   };
   std::vector<std::pair<std::string, Rec>> recs;
   std::vector<hipEvent_t> pool;
-  Rec &get(const char *name) {
+  Rec &get(const std::string &name) {
     for (auto &r : recs)
       if (r.first == name) return r.second;
     recs.emplace_back(name, Rec());
@@ -180,5 +194,7 @@ struct tdyhip_ctx {
   int32_t *d_vblk = nullptr;       // nv*7 vertex blocks for the row kernel (nullptr: row per thread)
   int32_t *d_brow = nullptr;       // flux row of each block row
   tdyhip::HostGeneral *hgen = nullptr;
-  double setup_ms[3] = {0, 0, 0};  // general setup: G matrices, bucketed local systems, gravity
+  double setup_ms[3] = {0, 0, 0};  // setup device times: G matrices, local systems, gravity term
+  double ls_flops = 0.0, ls_bytes = 0.0;  // hexahedral local systems: algorithmic FP64 flops and bytes
+  int ir_max_cells = 0, ir_max_faces = 0;
 };

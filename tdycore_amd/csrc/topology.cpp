@@ -80,7 +80,7 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
     for (int s = 0; s < 8; s++) {
       int v = cv[c * 8 + s];
       if (v < 0 || v >= nv) return "cell vertex out of range";
-      if (vc_n[v] >= kMaxIRCells) return "more than 8 cells at a vertex";
+      if (vc_n[v] >= kMaxIRCells) { t.over_limit = true; return "more than 8 cells at a vertex"; }
       vcell[(size_t)v * kMaxIRCells + vc_n[v]] = c;
       vsub[(size_t)v * kMaxIRCells + vc_n[v]] = s;
       vc_n[v]++;
@@ -90,7 +90,7 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
       if (((t.face_flags[f] & kFaceBoundary) != 0) != (pass == 1)) continue;
       for (int q = 0; q < 4; q++) {
         int v = fv[f * 4 + q];
-        if (vf_n[v] >= kMaxIRFaces) return "more than 12 faces at a vertex";
+        if (vf_n[v] >= kMaxIRFaces) { t.over_limit = true; return "more than 12 faces at a vertex"; }
         vface[(size_t)v * kMaxIRFaces + vf_n[v]] = f;
         vsubf[(size_t)v * kMaxIRFaces + vf_n[v]] = q;
         vf_n[v]++;
@@ -119,7 +119,7 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
       d.fup_i[k] = d.fdn_i[k] = 255;
     }
     d.nint = (uint8_t)nint; d.nneu = (uint8_t)nneu; d.ndir = (uint8_t)ndir;
-    if (ncell + ndir > kMaxCols) return "more than 8 transmissibility columns at a vertex";
+    if (ncell + ndir > kMaxCols) { t.over_limit = true; return "more than 8 transmissibility columns at a vertex"; }
     // columns: cells, then Dirichlet faces (boundary index)
     for (int i = 0; i < ncell; i++) {
       t.vcols[(size_t)v * kMaxCols + i] = vcell[(size_t)v * kMaxIRCells + i];

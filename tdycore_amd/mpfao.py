@@ -135,11 +135,11 @@ class MPFAO:
         return out
 
     def setup_info(self):
-        """Device times (ms) of the general-mesh setup phases and its bucket
+        """Device times (ms) of the setup phases and the interaction-region
         counts: {'gmatrix_ms', 'local_systems_ms', 'gravity_ms', 'buckets'
-        (regions per 16/32/64-lane bucket), 'max_cells', 'max_faces',
-        'trans_nnz', 'local_systems_flops', 'local_systems_bytes', 'lanes'
-        (team width the autotuner chose per bucket)}.  Times are zero for hexahedral meshes."""
+        (regions per 16/32/64-lane bucket; hexahedral meshes: all in the
+        first), 'max_cells', 'max_faces', 'trans_nnz', 'local_systems_flops',
+        'local_systems_bytes', 'lanes' (team width used per bucket)}."""
         ms = np.zeros(3)
         cnt = np.zeros(11, np.int64)
         _lib.check(_lib.load().tdyhip_setup_info(self._h, ms.ctypes.data, cnt.ctypes.data))

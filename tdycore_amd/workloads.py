@@ -160,3 +160,35 @@ def wheel_triangulation(n_spokes=14, rings=2, radius=100.0):
         pts += list(np.stack([r * radius * np.cos(a), r * radius * np.sin(a)], 1))
     P = np.array(pts)
     return P, Delaunay(P).simplices.astype(np.int64)
+
+
+def star_hex_problem(n_quads=6, layers=2, seed=16, radius=100.0, thickness=10.0):
+    """Hexahedra around one vertex of high valence: n_quads quadrilaterals
+    (centre, A_i, B_i, A_i+1) around a centre point, A_i on a circle, B_i
+    outside between them, extruded into ``layers`` layers of ``thickness``.
+    The centre vertex of an inner layer has 2 n_quads cells and 3 n_quads
+    faces (12 and 18 for n_quads = 6), more than the hexahedral kernels'
+    8 cells and 12 faces, so such a mesh takes the general path.  Rotated
+    log-normal tensors (sigma 0.5), phi ~ U(0.2, 0.4), Sr 0.1, van Genuchten
+    n = 1.5, alpha 1e-4; Dirichlet on every boundary face at the hydrostatic
+    pressure of a water table at mid-height."""
+    from . import exodus
+    rng = np.random.default_rng(seed)
+    a = 2 * np.pi * np.arange(n_quads) / n_quads
+    b = a + np.pi / n_quads
+    P = np.concatenate([[[0.0, 0.0]], radius * np.column_stack([np.cos(a), np.sin(a)]),
+                        1.4 * radius * np.column_stack([np.cos(b), np.sin(b)])])
+    quads = [[0, 1 + i, 1 + n_quads + i, 1 + (i + 1) % n_quads] for i in range(n_quads)]
+    ns = P.shape[0]
+    X = np.concatenate([np.column_stack([P, np.full(ns, l * thickness)]) for l in range(layers + 1)])
+    cells = [[v + l * ns for v in q] + [v + (l + 1) * ns for v in q] for l in range(layers) for q in quads]
+    m = exodus.connectivity_mesh(X, cells)
+    K = mg.random_rotated_K(m.nc, rng, -12.0, 0.5)
+    phi = rng.uniform(0.2, 0.4, size=m.nc)
+    Sr = np.full(m.nc, 0.1)
+    fbc = m.face_bc(default=mg.BC_DIRICHLET)
+    zt = 0.5 * layers * thickness
+    pb = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], zt))
+    U0 = mg.hydrostatic_pressure(m.cell_cen[:, 2], zt)
+    kw = dict(vg_m=1.0 / 3.0, vg_alpha=1e-4, gmatrix="mpfao", density="constant", p_bnd_fn=pb)
+    return dict(mesh=m, face_bc=fbc, K=K, porosity=phi, Sr=Sr, kw=kw, U0=U0)

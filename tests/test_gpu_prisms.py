@@ -29,10 +29,13 @@ def _problem(name):
         pr["face_bc"] = m.face_bc(default=mg.BC_DIRICHLET)
         pr["kw"]["p_bnd_fn"] = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 2.0))
         return pr
+    if name == "hex_star":  # hexahedra beyond the fixed-size kernels' limits take the general path
+        return workloads.star_hex_problem(n_quads=6, layers=3)
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["delaunay", "delaunay_tuned", "delaunay_tpf", "wheel", "wheel_all_dirichlet"])
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tuned", "delaunay_tpf", "wheel", "wheel_all_dirichlet",
+                                  "hex_star"])
 def test_prism_setup_matches_oracle(name):
     import torch
     from pyoracle import Oracle
@@ -55,6 +58,8 @@ def test_prism_setup_matches_oracle(name):
     assert sum(info["buckets"]) == P["mesh"].nv and info["trans_nnz"] == len(v0)
     if name.startswith("wheel"):
         assert info["buckets"][2] > 0  # a region of more than 32 unknowns
+    if name == "hex_star":
+        assert info["max_cells"] == 12 and info["max_faces"] == 18  # took the general path
     for b, w in zip(info["buckets"], info["lanes"]):
         assert (w == 0) == (b == 0) and w in (0, 16, 32, 64)
     if name == "delaunay_tuned":
@@ -62,7 +67,7 @@ def test_prism_setup_matches_oracle(name):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet"])
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet", "hex_star"])
 def test_prism_residual_and_jacobian_match_oracle(name):
     """TS and SNES residuals and Jacobians on prisms, through the C ABI,
     against the oracle at 1e-12 (residual: of its largest entry; Jacobian:

@@ -61,3 +61,28 @@ def test_checkpoint_round_trip_and_layout(tmp_path):
     assert dt(mine, "fields/IC") == dt(ref, "IC")
     with pytest.raises(KeyError):
         hdf5io.read_attribute(p, "missing")
+
+
+H5DUMP = "/opt/conda/bin/h5dump"
+
+
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump (the HDF5 library's reader) is not installed")
+def test_checkpoint_reads_back_with_the_hdf5_library(tmp_path):
+    """The checkpoint as the HDF5 library sees it: h5dump lists the root
+    attribute "time step" (int32 scalar) and "fields/IC" (float64 vector), and
+    its binary dump of fields/IC equals the written values bit for bit."""
+    import subprocess
+    rng = np.random.default_rng(5)
+    v = np.concatenate([rng.normal(1e5, 3e4, 997), [0.0, -0.0, 1.0 / 3.0]])
+    p = str(tmp_path / "ck.h5")
+    hdf5io.write_checkpoint(p, v, 42)
+    head = subprocess.run([H5DUMP, "-H", p], check=True, capture_output=True, text=True).stdout
+    assert 'ATTRIBUTE "time step"' in head and "H5T_STD_I32LE" in head and "SCALAR" in head
+    assert 'GROUP "fields"' in head and 'DATASET "IC"' in head and "H5T_IEEE_F64LE" in head
+    assert "( 1000 ) / ( 1000 )" in head
+    att = subprocess.run([H5DUMP, "-a", "/time step", p], check=True, capture_output=True, text=True).stdout
+    assert "(0): 42" in att
+    out = str(tmp_path / "ic.bin")
+    subprocess.run([H5DUMP, "-d", "/fields/IC", "-b", "LE", "-o", out, p], check=True, capture_output=True)
+    back = np.fromfile(out, dtype="<f8")
+    assert back.shape == v.shape and np.array_equal(back.view(np.uint64), v.view(np.uint64))
