@@ -103,8 +103,11 @@ static void kt_end(tdyhip_ctx *ctx, hipStream_t st, const char *name, double byt
 }
 // A GMRES cycle's launches are settled once its step count is known: those
 // tagged with a step >= `working` returned at once on the stop flag and move
-// to "<kernel> (exited at once)" with no bytes; the rest lose their tag.
-static void kt_settle(tdyhip_ctx *ctx, int working) {
+// to "<kernel> (exited at once)" with no bytes; so do the launches of the
+// next cycle's start (tag kTagRestart) when the solve had finished.  The
+// rest lose their tag.
+constexpr int kTagRestart = 1 << 20;
+static void kt_settle(tdyhip_ctx *ctx, int working, bool restart_worked) {
   if (!ctx->kt.on) return;
   std::vector<std::pair<std::string, KernelTimers::Pending>> moved;
   for (auto &nr : ctx->kt.recs) {
@@ -112,7 +115,8 @@ static void kt_settle(tdyhip_ctx *ctx, int working) {
     size_t keep = 0;
     for (size_t i = 0; i < pend.size(); i++) {
       KernelTimers::Pending pe = pend[i];
-      if (pe.tag >= working) {
+      const bool exited = pe.tag == kTagRestart ? !restart_worked : pe.tag >= working;
+      if (exited) {
         nr.second.launches--;
         nr.second.bytes -= pe.bytes;
         pe.tag = -1;
@@ -178,6 +182,7 @@ static StateParams state_params(const tdyhip_ctx *ctx, int form, double shift) {
   sp.pref = ctx->pref; sp.dtime = ctx->dtime; sp.shift = shift;
   sp.c0 = ctx->mualem_c[0]; sp.c1 = ctx->mualem_c[1]; sp.c2 = ctx->mualem_c[2]; sp.c3 = ctx->mualem_c[3];
   sp.density_type = ctx->density_type; sp.form = form;
+  sp.dyn = ctx->launch_dyn; sp.stop = ctx->launch_stop;
   return sp;
 }
 
@@ -229,6 +234,7 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
   a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
   a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
   a.flux = ctx->d_flux; a.Jrow = ctx->d_jrow;
+  a.stop = ctx->launch_stop;
   return a;
 }
 
@@ -243,6 +249,7 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
   a.perm = ell ? perm : nullptr;
   a.R = R; a.J = J;
+  a.dyn = ctx->launch_dyn; a.stop = ctx->launch_stop;
   return a;
 }
 
@@ -550,6 +557,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   a.T = ctx->d_Tgen; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd; a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr;
   a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr; a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
   a.flux = ctx->d_flux; a.D = ctx->d_Dgen;
+  a.stop = ctx->launch_stop;
   if (ctx->d_vblk) {
     GenBlkArgs b;
     b.pref = ctx->pref; b.vblk = ctx->d_vblk; b.brow = ctx->d_brow;
@@ -557,6 +565,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     b.T = ctx->d_Tgen; b.G = ctx->d_G; b.U = U; b.Pbnd = ctx->d_Pbnd; b.rho = ctx->d_rho; b.ukvr = ctx->d_ukvr;
     b.drho = ctx->d_drho; b.dukvr = ctx->d_dukvr; b.rho_bnd = ctx->d_rho_bnd; b.ukvr_bnd = ctx->d_ukvr_bnd;
     b.flux = ctx->d_flux; b.D = ctx->d_Dgen;
+    b.stop = ctx->launch_stop;
     const unsigned gv = (unsigned)ctx->nv;
     if (R && J) k_rows_general_blk<true, true><<<gv, 64, 0, st>>>(b);
     else if (R) k_rows_general_blk<true, false><<<gv, 64, 0, st>>>(b);
@@ -579,6 +588,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
     gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags; gg.gslot = ctx->d_gslot;
     gg.D = ctx->d_Dgen; gg.dM = ctx->d_dM; gg.d2M = ctx->d_d2M; gg.Udot = Udot; gg.vol = ctx->d_cell_vol; gg.J = J;
+    gg.dyn = ctx->launch_dyn; gg.stop = ctx->launch_stop;
     k_gather_general_J<<<nblk(ctx->nc, kGenJBlock), kGenJBlock, 0, st>>>(gg);
   }
   return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;

@@ -20,10 +20,16 @@
 
 namespace tdyhip {
 
+// Launch-time inputs that a replayed hipGraph must not freeze: when `dyn` is
+// set, dtime = dyn[0] and shift = dyn[1] are read from device memory (the
+// solver updates them between replays); when `stop` is set and nonzero the
+// launch returns at once (the Newton solver's device-side gating).
 struct StateParams {
   double vg_m, vg_alpha, x0, x1, dx, pref, dtime, shift;
   double c0, c1, c2, c3;
   int density_type, form;  // form: 0 TS, 1 SNES
+  const double *dyn;
+  const int *stop;
 };
 
 __global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ U,
@@ -33,6 +39,7 @@ __global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ 
                              double *__restrict__ ukvr_o, double *__restrict__ drho_o,
                              double *__restrict__ dukvr_o, double *__restrict__ dM_o,
                              double *__restrict__ d2M_o, double *__restrict__ acc_o) {
+  if (sp.stop && *sp.stop) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const double p = U[c];
@@ -59,7 +66,7 @@ __global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ 
   if (sp.form == 0) {
     acc_o[c] = dM * Udot[c] * vol[c];
   } else {
-    acc_o[c] = rho * ph * S * vol[c] / sp.dtime - accum_prev[c];
+    acc_o[c] = rho * ph * S * vol[c] / (sp.dyn ? sp.dyn[0] : sp.dtime) - accum_prev[c];
   }
 }
 
@@ -97,6 +104,7 @@ __global__ void k_bnd_state(int nbnd, StateParams sp, const double *__restrict__
                             const int32_t *__restrict__ bnd_cell, const double *__restrict__ Sr,
                             const double *__restrict__ pbnd_fn, double *__restrict__ Pbnd,
                             double *__restrict__ ukvr_bnd) {
+  if (sp.stop && *sp.stop) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbnd) return;
   const int c = bnd_cell[b];
@@ -128,6 +136,7 @@ struct RowArgs {
   const double *rho_bnd, *ukvr_bnd;
   double *flux;
   double *Jrow;  // nrows * kMaxCols: d(flux row r)/d(p of column k) at Jrow[r*kMaxCols + k]
+  const int *stop;
 };
 
 constexpr int kRowBlock = 256;  // k_rows workgroup size (the Jrow staging below assumes it)
@@ -218,6 +227,7 @@ __device__ inline void row_eval(const RowArgs &a, int64_t r, double (&jr)[kMaxCo
 // rows are staged in LDS and written out as one contiguous, coalesced run.
 template <bool WITH_R, bool WITH_J>
 __global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
+  if (a.stop && *a.stop) return;  // uniform over the workgroup
   const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
   double jr[kMaxCols];
   row_eval<WITH_R, WITH_J>(a, r, jr);
@@ -250,10 +260,12 @@ struct GenRowArgs {
   const int32_t *rtoff, *rinfo, *roc, *rdc, *ccell, *vdcol;
   const double *T, *G, *U, *Pbnd, *rho, *ukvr, *drho, *dukvr, *rho_bnd, *ukvr_bnd;
   double *flux, *D;
+  const int *stop;
 };
 
 template <bool WITH_R, bool WITH_J>
 __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
+  if (a.stop && *a.stop) return;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.nrows) return;
   const int f = (int)(r >> 2);
@@ -335,6 +347,8 @@ struct GenGatherArgs {
   const uint8_t *face_flags, *gslot;
   const double *D, *dM, *d2M, *Udot, *vol;
   double *J;
+  const double *dyn;  // device dtime / shift (see StateParams), or nullptr
+  const int *stop;
 };
 
 // The same flux rows, one wavefront per vertex block: first a lane per block
@@ -354,10 +368,12 @@ struct GenBlkArgs {
   const int32_t *ccell, *vdcol;
   const double *T, *G, *U, *Pbnd, *rho, *ukvr, *drho, *dukvr, *rho_bnd, *ukvr_bnd;
   double *flux, *D;
+  const int *stop;
 };
 
 template <bool WITH_R, bool WITH_J>
 __global__ void __launch_bounds__(64) k_rows_general_blk(GenBlkArgs a) {
+  if (a.stop && *a.stop) return;  // uniform over the workgroup
   __shared__ double s_tp[kBlkRows], s_den[kBlkRows], s_ukvr[kBlkRows], s_G[kBlkRows];
   __shared__ double s_ddu[kBlkRows], s_ddd[kBlkRows], s_dku[kBlkRows], s_dkd[kBlkRows];
   __shared__ int s_up[kBlkRows], s_dn[kBlkRows], s_flag[kBlkRows];  // flag: 1 no D, 2 zero D
@@ -486,6 +502,7 @@ __global__ void __launch_bounds__(256) k_gen_slot_table(int64_t nrows, const int
 
 __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a) {
   __shared__ double acc[kGenJSlots][kGenJBlock];
+  if (a.stop && *a.stop) return;
   const int t = threadIdx.x;
   const int c = blockIdx.x * kGenJBlock + t;
   if (c >= a.nc) return;
@@ -508,8 +525,9 @@ __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a
       }
     }
   }
-  const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
-                                  : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
+  const double shift = a.dyn ? a.dyn[1] : a.shift, dtime = a.dyn ? a.dyn[0] : a.dtime;
+  const double dv = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
+                                  : a.dM[c] * a.vol[c] * (1.0 / dtime);
   acc[a.jdiag[c]][t] += dv;
   const int64_t st = a.stride;
   double *row = a.J + (a.perm ? (int64_t)a.perm[c] : (int64_t)r0);
@@ -565,6 +583,8 @@ struct GatherArgs {
   int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or the ELL row count
   int ell_w;           // ELL width (slots written per row in the ELL layout)
   double *R, *J;
+  const double *dyn;   // device dtime / shift (see StateParams), or nullptr
+  const int *stop;
 };
 
 constexpr int kGatherBlock = 128;
@@ -579,6 +599,7 @@ constexpr int kJacSlots = 27;  // vertex-adjacency row length of a hexahedral me
 template <bool WITH_R, bool WITH_J>
 __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
   __shared__ double acc[WITH_J ? kJacSlots : 1][kGatherBlock];
+  if (a.stop && *a.stop) return;
   const int t = threadIdx.x;
   const int c = blockIdx.x * kGatherBlock + t;
   if (c >= a.nc) return;
@@ -614,8 +635,9 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
         }
       }
     }
-    const double dv = (a.form == 0) ? (a.shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
-                                    : a.dM[c] * a.vol[c] * (1.0 / a.dtime);
+    const double shift = a.dyn ? a.dyn[1] : a.shift, dtime = a.dyn ? a.dyn[0] : a.dtime;
+    const double dv = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
+                                    : a.dM[c] * a.vol[c] * (1.0 / dtime);
     acc[a.jdslot[c]][t] += dv;
     const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
     const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;

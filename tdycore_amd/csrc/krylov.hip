@@ -73,7 +73,7 @@ struct GmresDev {
   int maxit;
   int finished;    // the solve is over: converged, iteration cap, or a non-finite residual
   int diverged;    // a non-finite residual estimate (KSP_DIVERGED_NANORINF)
-  int pad;
+  int jlast;       // columns of the last cycle that ended (its Hessenberg solve)
 };
 
 // ---------------------------------------------------------------- ILU(0) factor
@@ -315,7 +315,9 @@ template <class TV, class TI>
 __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32_t *__restrict__ rowof,
                                                   const TI *__restrict__ ecol, const int32_t *__restrict__ pat,
                                                   int bs, const int32_t *__restrict__ bw, const TV *__restrict__ a,
-                                                  const TV *__restrict__ x, TV *__restrict__ y) {
+                                                  const TV *__restrict__ x, TV *__restrict__ y,
+                                                  const int *__restrict__ stop = nullptr) {
+  if (stop && *stop) return;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NB) return;
   const int row = rowof[p];
@@ -578,8 +580,9 @@ __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, con
 // partial[chunk] = <x, y>
 template <class TX, class TY>
 __global__ void __launch_bounds__(kRedBlock) k_dot(int64_t n, const TX *__restrict__ x, const TY *__restrict__ y,
-                                                   double *__restrict__ partial) {
+                                                   double *__restrict__ partial, const int *__restrict__ stop = nullptr) {
   __shared__ double sh[kRedBlock / 64];
+  if (stop && *stop) return;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
   double s = 0;
 #pragma unroll
@@ -653,8 +656,14 @@ __global__ void k_gmres_init(GmresDev *gd, int *status, int maxit) {
 }
 
 // Start of a cycle: beta = sqrt(h[0]) from a norm reduction; sets tol on the first cycle.
+// After the solve has finished, a later cycle is empty: no columns, stop flag set.
 __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double *__restrict__ g, int m, double rtol,
                               double atol, int first) {
+  if (!first && gd->finished) {
+    gd->jcur = 0;
+    gd->done = 1;
+    return;
+  }
   const double beta = sqrt(h[0]);
   if (first) gd->tol = fmax(rtol * beta, atol);
   for (int i = 0; i <= m; i++) g[i] = 0.0;
@@ -695,9 +704,10 @@ __global__ void __launch_bounds__(kRedBlock) k_norm_arnoldi(int nchunk, const TV
 }
 
 // y = H(1:j,1:j)^-1 g(1:j) for the j = jcur columns of the cycle
-__global__ void k_hess_solve(const GmresDev *gd, int m, const double *__restrict__ H, const double *__restrict__ g,
+__global__ void k_hess_solve(GmresDev *gd, int m, const double *__restrict__ H, const double *__restrict__ g,
                              double *__restrict__ y) {
   const int ld = m + 1, j = gd->jcur;
+  gd->jlast = j;
   for (int i = j - 1; i >= 0; i--) {
     double s = g[i];
     for (int k = i + 1; k < j; k++) s -= H[i + k * ld] * y[k];
@@ -719,14 +729,17 @@ __global__ void k_gemv_update_dev(int64_t n, int mmax, const TV *__restrict__ V,
 
 // out = (TV)(a - b), the difference taken in FP64
 template <class TV>
-__global__ void k_sub_to(int64_t n, const double *__restrict__ a, const double *__restrict__ b, TV *__restrict__ out) {
+__global__ void k_sub_to(int64_t n, const double *__restrict__ a, const double *__restrict__ b, TV *__restrict__ out,
+                         const int *__restrict__ stop = nullptr) {
+  if (stop && *stop) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (TV)(a[i] - b[i]);
 }
 
 // w = x - lambda*y
 __global__ void k_waxpy(int64_t n, double lambda, const double *__restrict__ x, const double *__restrict__ y,
-                        double *__restrict__ w) {
+                        double *__restrict__ w, const int *__restrict__ stop = nullptr) {
+  if (stop && *stop) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) w[i] = x[i] - lambda * y[i];
 }

@@ -386,11 +386,26 @@ struct SolverWS {
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
+  // Newton iterations (and tails) captured as graphs
+  struct NewtonGraph {
+    const double *X;
+    int type, fp32, cycles;
+    hipGraphExec_t exec;
+  };
+  std::vector<NewtonGraph> newton_graphs;
+  int cycles_hint = 1;            // restart cycles the last Newton iteration's solve needed
+  struct NewtonDev *nd = nullptr; // the Newton iteration's outcome on the device
+  double *nh = nullptr;           // 4 norms / dots of the Newton tail
+  double *dyn = nullptr;          // dtime, shift (1 / dt of the TS form) read by the solver's launches
 };
+
+constexpr int kMaxGraphCycles = 8;    // restart cycles inside one Newton-iteration graph
+constexpr size_t kMaxNewtonGraphs = 16;
 
 static void ws_free(SolverWS &ws) {
   for (hipGraphExec_t &e : ws.cycle_exec)
     if (e) (void)hipGraphExecDestroy(e);
+  for (auto &e : ws.newton_graphs) (void)hipGraphExecDestroy(e.exec);
   for (void *p : ws.allocs) (void)hipFree(p);
   ws.allocs.clear();
   ws = SolverWS();
@@ -493,6 +508,9 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if ((rc = ws_alloc(ws, &ws.hs, (size_t)(o.ksp_restart + 1) * o.ksp_restart + 4 * (o.ksp_restart + 1)))) return rc;
   if ((rc = ws_alloc(ws, &ws.gdev, 1))) return rc;
   if ((rc = ws_alloc(ws, &ws.status, 1))) return rc;
+  if ((rc = ws_alloc(ws, (char **)&ws.nd, 256))) return rc;  // a NewtonDev (defined with the solver below)
+  if ((rc = ws_alloc(ws, &ws.nh, 4))) return rc;
+  if ((rc = ws_alloc(ws, &ws.dyn, 2))) return rc;
   ws.coarse_req = o.pc_coarse ? 1 : 0;
   if (o.pc_coarse) {
     HostCoarse hc;
@@ -549,15 +567,16 @@ static double bytes_spmv(const tdyhip_ctx *ctx, const SolverWS &ws, size_t tv, b
 }
 
 // y = J x with the FP64 Jacobian in the block layout
-static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, double *y, hipStream_t st) {
+static void spmv_blk64(tdyhip_ctx *ctx, const SolverWS &ws, const double *x, double *y, hipStream_t st,
+                       const int *stop = nullptr) {
   const unsigned g = (unsigned)((ws.NB + 255) / 256);
   const bool i16 = ws.d_ecol16 != nullptr;
   if (i16)
     KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), true),
-       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.bj.d_pat, ws.bj.bs, ws.bj.d_bw, ws.J, x, y));
+       k_spmv_blk<double, int16_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol16, ws.bj.d_pat, ws.bj.bs, ws.bj.d_bw, ws.J, x, y, stop));
   else
     KT(ctx, st, "k_spmv_blk", bytes_spmv(ctx, ws, sizeof(double), false),
-       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, nullptr, ws.bj.bs, ws.bj.d_bw, ws.J, x, y));
+       k_spmv_blk<double, int32_t><<<g, 256, 0, st>>>(ws.NB, ctx->jell_w, ws.bj.d_row, ws.d_ecol, nullptr, ws.bj.bs, ws.bj.d_bw, ws.J, x, y, stop));
 }
 static double bytes_apply(const SolverWS &ws, size_t tv) {
   const double rows = (double)ws.bj.nblocks * ws.bj.bs;
@@ -570,33 +589,125 @@ static double bytes_factor(const SolverWS &ws, size_t tv) {
   return rows * (kW * 8 + (2 * kLU + 1) * tv) + idx_rows * (kW * (4 + 2) + kLU + kPair * 4 + 1 + 1 + 2);
 }
 
-// Factor the preconditioner from ws.J and solve J y = b (FP64 in/out) with
-// GMRES(m), left preconditioned, classical Gram-Schmidt.  Each restart cycle is
-// launched as one batch of kernels; Givens rotations and the stopping test run
-// on the device, and the host synchronises once per cycle.
+// ---------------------------------------------------------------- GMRES in launch pieces
+// GMRES(m), left preconditioned, classical Gram-Schmidt, on ws.J (FP64) or its
+// FP32 copy, solving J y = b with b and y FP64.  The solve is launched in
+// pieces that the host-driven path and the captured Newton iteration share:
+//   gmres_prep   factor the preconditioner, y = 0, r0 = M^-1 b and the first
+//                cycle's start (a solve may be converged before any step);
+//   gmres_cycle  one restart cycle (m Arnoldi steps that stop early on the
+//                device, the Hessenberg solve, the update of y), then the next
+//                cycle's start: w = M^-1 (b - J y) with the FP64 Jacobian and
+//                its norm.  Once the solve has finished (converged, iteration
+//                cap or a non-finite residual) every kernel of a later cycle
+//                returns at once, so any number of cycles may run in a row.
 template <class TV>
-static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec, int *its_out,
-                 hipStream_t st) {
+struct KBufs {
+  TV *V, *w, *z, *b, *fac, *Jk, *part;
+  double *dH, *dcs, *dsn, *dg, *dy, *rc1;
+  GmresDev *gd;
+  int m, nch;
+  int64_t N;
+};
+
+template <class TV>
+static KBufs<TV> kbufs(tdyhip_ctx *ctx, int m) {
   SolverWS &ws = *ctx->ws;
-  const int n = ws.n, m = o.ksp_restart;
-  const int64_t N = n;
-  TV *V = (TV *)ws.V, *w = (TV *)ws.w, *z = (TV *)ws.z, *b = (TV *)ws.b, *x = (TV *)ws.x, *fac = (TV *)ws.fac;
-  TV *Jk = std::is_same<TV, double>::value ? (TV *)ws.J : (TV *)ws.Jk, *part = (TV *)ws.partial;
-  GmresDev *gd = ws.gdev;
-  double *dH = ws.hs, *dcs = dH + (size_t)(m + 1) * m, *dsn = dcs + m, *dg = dsn + m, *dy = dg + (m + 1);
-  const int *stop = &gd->done;
-  const unsigned g256 = (unsigned)((N + 255) / 256);
+  KBufs<TV> k;
+  k.V = (TV *)ws.V; k.w = (TV *)ws.w; k.z = (TV *)ws.z; k.b = (TV *)ws.b; k.fac = (TV *)ws.fac;
+  k.Jk = std::is_same<TV, double>::value ? (TV *)ws.J : (TV *)ws.Jk;
+  k.part = (TV *)ws.partial;
+  k.dH = ws.hs; k.dcs = k.dH + (size_t)(m + 1) * m; k.dsn = k.dcs + m; k.dg = k.dsn + m; k.dy = k.dg + (m + 1);
+  k.rc1 = ws.coarse ? ws.cl.rc1 : nullptr;
+  k.gd = ws.gdev;
+  k.m = m;
+  k.N = ws.n;
+  k.nch = (int)((k.N + Red<TV>::chunk - 1) / Red<TV>::chunk);  // chunks of the multi-vector kernels
+  return k;
+}
+
+// level-1/2 solve of the coarse correction: z1 per block
+static void coarse_solve(tdyhip_ctx *ctx, const int *stp, hipStream_t st) {
+  const CoarseLayout &C = ctx->ws->cl;
+  KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24 + (double)C.ng * C.ng * 8,
+     k_coarse_level1<<<C.ng, kL1Threads, 0, st>>>(C, stp));
+}
+
+// out = M^-1 in (block-Jacobi ILU(0), plus the coarse correction when on)
+template <class TV>
+static void pc_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *in, TV *out, const int *stp, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, k.fac, in, out, stp, k.rc1));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, k.fac, in, out, stp, k.rc1));
+  else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, k.fac, in, out, stp, k.rc1));
+  if (!ws.coarse) return;
+  const CoarseLayout &C = ws.cl;
+  coarse_solve(ctx, stp, st);
+  const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * 8;
+  if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp));
+  else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
+}
+
+// V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused
+// into the ILU sweeps); with na, ||u|| comes from the previous step's norm
+// partials and the launch also finishes that step's Arnoldi column
+template <class TV>
+static void spmv_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *u, TV *vj, TV *out, const int *stp,
+                       const NormArgs *na, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const int W = ctx->jell_w;
+  const double by = bytes_spmv(ctx, ws, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
+  const unsigned nb = ws.bj.nblocks;
+  const NormArgs nv = na ? *na : NormArgs{};
+  GmresDev *gd = k.gd;
+  const TV *Jk = k.Jk, *fac = k.fac;
+  double *rc1 = k.rc1;
+#define TDY_SPMV(TI, EC, NRM)                                                                                      \
+  do {                                                                                                             \
+    if (ws.bj.bs == 64)                                                                                            \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, TI, NRM><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+    else if (ws.bj.bs == 256)                                                                                      \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, TI, NRM><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+    else                                                                                                           \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, TI, NRM><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+  } while (0)
+  if (ws.d_ecol16) {
+    if (na) TDY_SPMV(int16_t, ws.d_ecol16, true);
+    else TDY_SPMV(int16_t, ws.d_ecol16, false);
+  } else {
+    if (na) TDY_SPMV(int32_t, ws.d_ecol, true);
+    else TDY_SPMV(int32_t, ws.d_ecol, false);
+  }
+#undef TDY_SPMV
+  // the prolongation of the coarse correction is added by the multi-dot that follows
+  if (ws.coarse) coarse_solve(ctx, stp, st);
+}
+
+// ws.h[0] = ||v||^2
+template <class TV>
+static void norm2_to_h0(tdyhip_ctx *ctx, const TV *v, const int *stp, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(ws.n, v, v, ws.dpartial, stp);
+  k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h, stp);
+}
+
+template <class TV>
+static void gmres_prep(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec,
+                       hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const KBufs<TV> k = kbufs<TV>(ctx, o.ksp_restart);
+  const unsigned g256 = (unsigned)((k.N + 255) / 256);
   if (!std::is_same<TV, double>::value)
     KT(ctx, st, "k_convert_J", (double)ws.nnz * (8 + sizeof(TV)),
-       k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, Jk));
-  k_gmres_init<<<1, 1, 0, st>>>(gd, ws.status, o.ksp_max_it);
-  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
-  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
-  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, FactorLanes<512>::NT, 0, st>>>(ws.bj, ws.J, fac, ws.status));
+       k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, k.Jk));
+  k_gmres_init<<<1, 1, 0, st>>>(k.gd, ws.status, o.ksp_max_it);
+  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
+  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, FactorLanes<512>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
   // coarse correction: Galerkin rows, group inverses and the level-2 inverse
   // for this Jacobian; each application then adds its prolongation to the
   // block-Jacobi result (the restriction is fused into the ILU sweeps)
-  double *rc1 = ws.coarse ? ws.cl.rc1 : nullptr;
   if (ws.coarse) {
     const CoarseLayout &C = ws.cl;
     const int W = ctx->jell_w;
@@ -607,149 +718,120 @@ static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bve
     KT(ctx, st, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C));
     KT(ctx, st, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, st>>>(C));
   }
-  // level-1/2 solve of the coarse correction: z1 per block
-  auto coarse_solve = [&](const int *stp) {
-    const CoarseLayout &C = ws.cl;
-    KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24 + (double)C.ng * C.ng * 8,
-       k_coarse_level1<<<C.ng, kL1Threads, 0, st>>>(C, stp));
-  };
-  auto coarse_add = [&](TV *out, const int *stp) {
-    if (!ws.coarse) return;
-    const CoarseLayout &C = ws.cl;
-    coarse_solve(stp);
-    const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * 8;
-    if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp));
-    else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp));
-    else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
-  };
-  auto pc_apply = [&](const TV *in, TV *out, const int *stp) {
-    if (ws.bj.bs == 64) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 64><<<ws.bj.nblocks, 64, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
-    else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 256><<<ws.bj.nblocks, 256, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
-    else KT(ctx, st, "k_bj_apply", bytes_apply(ws, sizeof(TV)), k_bj_apply<TV, 512><<<ws.bj.nblocks, 512, 0, st>>>(ws.bj, fac, in, out, stp, rc1));
-    coarse_add(out, stp);
-  };
-  // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused
-  // into the ILU sweeps); with na, ||u|| comes from the previous step's norm
-  // partials and the launch also finishes that step's Arnoldi column
-  auto spmv_apply = [&](const TV *u, TV *vj, TV *out, const int *stp, const NormArgs *na) {
-    const int W = ctx->jell_w;
-    const double by = bytes_spmv(ctx, ws, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
-    const unsigned nb = ws.bj.nblocks;
-    const NormArgs nv = na ? *na : NormArgs{};
-#define TDY_SPMV(TI, EC, NRM)                                                                                      \
-  do {                                                                                                             \
-    if (ws.bj.bs == 64)                                                                                            \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, TI, NRM><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
-    else if (ws.bj.bs == 256)                                                                                      \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, TI, NRM><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
-    else                                                                                                           \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, TI, NRM><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
-  } while (0)
-    if (ws.d_ecol16) {
-      if (na) TDY_SPMV(int16_t, ws.d_ecol16, true);
-      else TDY_SPMV(int16_t, ws.d_ecol16, false);
-    } else {
-      if (na) TDY_SPMV(int32_t, ws.d_ecol, true);
-      else TDY_SPMV(int32_t, ws.d_ecol, false);
-    }
-#undef TDY_SPMV
-    // the prolongation of the coarse correction is added by the multi-dot that follows
-    if (ws.coarse) coarse_solve(stp);
-  };
-  auto norm2_to_h0 = [&](const TV *v) {
-    k_dot<TV, TV><<<ws.nchunk, kRedBlock, 0, st>>>(N, v, v, ws.dpartial);
-    k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.h);
-  };
-  k_convert<double, TV><<<g256, 256, 0, st>>>(N, bvec, b);
+  k_convert<double, TV><<<g256, 256, 0, st>>>(k.N, bvec, k.b);
   // The solution accumulates in FP64 in place (yvec) in both precisions, and
-  // every restart recomputes the true residual b - J x with the FP64 Jacobian
+  // every restart recomputes the true residual b - J y with the FP64 Jacobian
   // (mixed-precision restarted GMRES): the FP32 Arnoldi cycles only ever
   // solve for a correction, so the FP32 form stops on the same residual as
   // the FP64 one.
-  (void)x;
-  k_scale_copy<double><<<g256, 256, 0, st>>>(N, 0.0, bvec, yvec);
-  // r0 = M^-1 b  (x0 = 0)
-  pc_apply(b, w, nullptr);
-  norm2_to_h0(w);
-  k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 1);
-  SCHK(hipGetLastError());
-  GmresDev hg;
-  int status = 0;
-  // One restart cycle: m Arnoldi steps (each stops early on the device once
-  // the cycle has converged), the Hessenberg solve and the update of x.
-  // The unnormalised Arnoldi vector alternates between w and z: step j reads
-  // it from one and leaves the next in the other.
-  const int nch = (int)((N + Red<TV>::chunk - 1) / Red<TV>::chunk);  // chunks of the multi-vector kernels
-  constexpr int kFuseNormChunks = 512;
+  k_scale_copy<double><<<g256, 256, 0, st>>>(k.N, 0.0, bvec, yvec);
+  pc_apply<TV>(ctx, k, k.b, k.w, nullptr, st);  // r0 = M^-1 b  (y0 = 0)
+  norm2_to_h0<TV>(ctx, k.w, nullptr, st);
+  k_cycle_start<<<1, 1, 0, st>>>(k.gd, ws.h, k.dg, k.m, o.ksp_rtol, o.ksp_atol, 1);
+}
+
+// On meshes of up to this many reduction chunks, step j > 0 finishes column
+// j-1 (its norm and Givens update) inside the matvec: every workgroup re-reads
+// the norm partials, which costs less than a launch while they are few.  (At
+// 1M rows, 2048 partials per workgroup cost as much as they save.)
+constexpr int kFuseNormChunks = 512;
+
+template <class TV>
+static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec,
+                        hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const KBufs<TV> k = kbufs<TV>(ctx, o.ksp_restart);
+  const int m = k.m, nch = k.nch;
+  const int64_t N = k.N;
+  const unsigned g256 = (unsigned)((N + 255) / 256);
+  GmresDev *gd = k.gd;
+  const int *stop = &gd->done;
   Prolong pro;  // coarse prolongation folded into the multi-dot
   if (ws.coarse) {
     pro.z1 = ws.cl.z1;
     pro.pos = ws.d_perm;
     pro.bs_log2 = ws.bj.bs == 64 ? 6 : ws.bj.bs == 256 ? 8 : 9;
   }
-  auto cycle_body = [&]() {
-    for (int j = 0; j < m; j++) {
-      TV *u = (j % 2 == 0) ? w : z, *o = (j % 2 == 0) ? z : w;
-      // On meshes of up to 512 chunks, step j > 0 finishes column j-1 (its
-      // norm and Givens update) inside the matvec: every workgroup re-reads
-      // the norm partials, which costs less than a launch while they are few.
-      // (At 1M rows, 2048 partials per workgroup cost as much as they save.)
-      const NormArgs na{part, nch, j - 1, m, ws.h, dH, dcs, dsn, dg};
-      const bool fuse_norm = nch <= kFuseNormChunks;
-      // timer tags: the launches of step j work only if the cycle reached
-      // column j; a fused matvec also finishes column j-1, so it counts with it
-      ctx->kt.tag = (fuse_norm && j > 0) ? j - 1 : j;
-      spmv_apply(u, V + (int64_t)j * N, o, stop, (fuse_norm && j > 0) ? &na : nullptr);
-      ctx->kt.tag = j;
-      // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
-      // kernel per reduction with a last-block hand-off is much slower here:
-      // the device-scope fences it needs write back the per-XCD L2s.)
-      const double bmd = (double)(j + 2) * N * sizeof(TV) + (pro.z1 ? (double)N * (4 + sizeof(TV)) : 0.0);
-      if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
-      else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
-      else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, o, part, nch, pro, stop));
-      k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, part, ws.h, stop);
-      KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, V, N, ws.h, o, part, stop));
-      if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, part, gd, j, m, ws.h, dH, dcs, dsn, dg);
-    }
-    ctx->kt.tag = -1;
-    k_hess_solve<<<1, 1, 0, st>>>(gd, m, dH, dg, dy);
-    // (booked at the solution's read and write only: the basis vectors it reads depend on the cycle's length)
-    KT(ctx, st, "k_gemv_update", (double)N * 16, k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, V, N, dy, gd, yvec));
-  };
-  // Replay the cycle as a hipGraph (captured on first use) unless the kernel
-  // timers are on (they time every launch) or the vectors are not the
-  // workspace's own.
-  const bool graph = o.hip_graphs && !ctx->kt.on && bvec == ws.F && yvec == ws.Y;
-  hipGraphExec_t &cyc = ws.cycle_exec[std::is_same<TV, double>::value ? 0 : 1];
-  if (graph && !cyc) {
-    hipGraph_t gr = nullptr;
-    SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    cycle_body();
-    SCHK(hipStreamEndCapture(st, &gr));
-    const hipError_t ie = hipGraphInstantiate(&cyc, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (ie != hipSuccess) { cyc = nullptr; SCHK(ie); }
+  // The unnormalised Arnoldi vector alternates between w and z: step j reads
+  // it from one and leaves the next in the other.
+  for (int j = 0; j < m; j++) {
+    TV *u = (j % 2 == 0) ? k.w : k.z, *ov = (j % 2 == 0) ? k.z : k.w;
+    const NormArgs na{k.part, nch, j - 1, m, ws.h, k.dH, k.dcs, k.dsn, k.dg};
+    const bool fuse_norm = nch <= kFuseNormChunks;
+    // timer tags: the launches of step j work only if the cycle reached
+    // column j; a fused matvec also finishes column j-1, so it counts with it
+    ctx->kt.tag = (fuse_norm && j > 0) ? j - 1 : j;
+    spmv_apply<TV>(ctx, k, u, k.V + (int64_t)j * N, ov, stop, (fuse_norm && j > 0) ? &na : nullptr, st);
+    ctx->kt.tag = j;
+    // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
+    // kernel per reduction with a last-block hand-off is much slower here:
+    // the device-scope fences it needs write back the per-XCD L2s.)
+    const double bmd = (double)(j + 2) * N * sizeof(TV) + (pro.z1 ? (double)N * (4 + sizeof(TV)) : 0.0);
+    if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
+    else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
+    else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
+    k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, k.part, ws.h, stop);
+    KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ws.h, ov, k.part, stop));
+    if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, k.part, gd, j, m, ws.h, k.dH, k.dcs, k.dsn, k.dg);
   }
+  ctx->kt.tag = -1;
+  k_hess_solve<<<1, 1, 0, st>>>(gd, m, k.dH, k.dg, k.dy);
+  // (booked at the solution's read and write only: the basis vectors it reads depend on the cycle's length)
+  KT(ctx, st, "k_gemv_update", (double)N * 16, k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, k.V, N, k.dy, gd, yvec));
+  // the next cycle's start, w = M^-1 (b - J y) in FP64 and its norm; nothing
+  // once the solve has finished
+  const int *fin = &gd->finished;
+  ctx->kt.tag = kTagRestart;
+  spmv_blk64(ctx, ws, yvec, ws.JY, st, fin);
+  KT(ctx, st, "k_sub_to", (double)N * (16 + sizeof(TV)), k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, k.z, fin));
+  pc_apply<TV>(ctx, k, k.z, k.w, fin, st);
+  norm2_to_h0<TV>(ctx, k.w, fin, st);
+  ctx->kt.tag = -1;
+  k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, k.dg, m, o.ksp_rtol, o.ksp_atol, 0);
+}
+
+// The one-cycle graph of precision TV (captured on first use, replayed by
+// the host-driven loop); its kernels only touch workspace arrays.
+template <class TV>
+static hipGraphExec_t cycle_graph(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  hipGraphExec_t &cyc = ws.cycle_exec[std::is_same<TV, double>::value ? 0 : 1];
+  if (cyc) return cyc;
+  hipGraph_t gr = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return nullptr;
+  gmres_cycle<TV>(ctx, o, ws.F, ws.Y, st);
+  if (hipStreamEndCapture(st, &gr) != hipSuccess) return nullptr;
+  if (hipGraphInstantiate(&cyc, gr, nullptr, nullptr, 0) != hipSuccess) cyc = nullptr;
+  (void)hipGraphDestroy(gr);
+  return cyc;
+}
+
+// Host-driven solve: the preparation, then one cycle at a time (replayed as
+// the one-cycle graph when allowed) with a read of the cycle's state after
+// each, until the solve has finished.
+template <class TV>
+static int gmres(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *bvec, double *yvec, int *its_out,
+                 hipStream_t st, bool allow_graph) {
+  SolverWS &ws = *ctx->ws;
+  gmres_prep<TV>(ctx, o, bvec, yvec, st);
+  SCHK(hipGetLastError());
+  const bool graph = allow_graph && o.hip_graphs && !ctx->kt.on && bvec == ws.F && yvec == ws.Y;
+  hipGraphExec_t cyc = graph ? cycle_graph<TV>(ctx, o, st) : nullptr;
+  if (graph && !cyc) { set_error("GMRES cycle graph capture failed"); return TDYHIP_ERR_HIP; }
+  GmresDev hg;
+  int status = 0;
   for (int cycle = 0;; cycle++) {
-    if (graph) SCHK(hipGraphLaunch(cyc, st));
-    else cycle_body();
+    if (cyc) SCHK(hipGraphLaunch(cyc, st));
+    else gmres_cycle<TV>(ctx, o, bvec, yvec, st);
     SCHK(hipGetLastError());
-    SCHK(hipMemcpyAsync(&hg, gd, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+    SCHK(hipMemcpyAsync(&hg, ws.gdev, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
     if (cycle == 0) SCHK(hipMemcpyAsync(&status, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
-    kt_settle(ctx, hg.jcur);
+    kt_settle(ctx, hg.jlast, !hg.finished);
     if (status) { set_error("zero pivot in block-Jacobi ILU(0)"); return TDYHIP_ERR_SINGULAR; }
     if (hg.finished) break;
     if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
-    // restart: w = M^-1 (b - A x), the residual in FP64
-    spmv_blk64(ctx, ws, yvec, ws.JY, st);
-    KT(ctx, st, "k_sub_to", (double)N * (16 + sizeof(TV)), k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, z));
-    pc_apply(z, w, nullptr);
-    norm2_to_h0(w);
-    k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, dg, m, o.ksp_rtol, o.ksp_atol, 0);
   }
-  SCHK(hipGetLastError());
   *its_out = hg.its;
   // KSP_DIVERGED_ITS: PETSc's SNES counts it as a linear-solve failure
   // (SNESCheckKSPSolve, maxLinearSolveFailures 1 -> SNES_DIVERGED_LINEAR_SOLVE)
@@ -768,122 +850,352 @@ struct Form {
   double dt;            // TS only
 };
 
-__global__ void k_udot(int64_t n, double inv_dt, const double *__restrict__ U, const double *__restrict__ Up,
-                       double *__restrict__ Ud) {
+// Udot = (U - Uprev) / dt with 1 / dt read from device memory (dyn[1])
+__global__ void k_udot(int64_t n, const double *__restrict__ dyn, const double *__restrict__ U,
+                       const double *__restrict__ Up, double *__restrict__ Ud, const int *__restrict__ stop) {
+  if (stop && *stop) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) Ud[i] = (U[i] - Up[i]) * inv_dt;
+  if (i < n) Ud[i] = (U[i] - Up[i]) * dyn[1];
 }
 
-static int eval_F(tdyhip_ctx *ctx, const Form &fm, const double *X, double *F, hipStream_t st) {
+// The solver's residual and Jacobian launches read dtime and shift from
+// ws.dyn (so a replayed graph follows dt) and honour the given stop flag.
+static int eval_F(tdyhip_ctx *ctx, const Form &fm, const double *X, double *F, hipStream_t st,
+                  const int *stop = nullptr) {
   SolverWS &ws = *ctx->ws;
+  ctx->launch_dyn = ws.dyn;
+  ctx->launch_stop = stop;
+  int rc;
   if (fm.type == 0) {
-    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
-    int rc = launch_state(ctx, 0, X, ws.Udot, st);
-    if (rc) return rc;
-    return launch_flux(ctx, 0, 0.0, X, ws.Udot, F, nullptr, st);
+    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, ws.dyn, X, fm.Uprev, ws.Udot, stop);
+    rc = launch_state(ctx, 0, X, ws.Udot, st);
+    if (!rc) rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, F, nullptr, st);
+  } else {
+    rc = launch_state(ctx, 1, X, nullptr, st);
+    if (!rc) rc = launch_flux(ctx, 1, 0.0, X, nullptr, F, nullptr, st);
   }
-  int rc = launch_state(ctx, 1, X, nullptr, st);
-  if (rc) return rc;
-  return launch_flux(ctx, 1, 0.0, X, nullptr, F, nullptr, st);
+  ctx->launch_dyn = nullptr;
+  ctx->launch_stop = nullptr;
+  return rc;
 }
 
 static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, hipStream_t st) {
   SolverWS &ws = *ctx->ws;
+  ctx->launch_dyn = ws.dyn;
+  int rc;
   if (fm.type == 0) {
-    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, 1.0 / fm.dt, X, fm.Uprev, ws.Udot);
-    return launch_flux(ctx, 0, 1.0 / fm.dt, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB);
+    k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, ws.dyn, X, fm.Uprev, ws.Udot, nullptr);
+    rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB);
+  } else {
+    rc = launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB);
   }
-  return launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB);
+  ctx->launch_dyn = nullptr;
+  return rc;
+}
+
+// ---------------------------------------------------------------- Newton tail on the device
+// After the linear solve of a Newton iteration: the step length checks, the
+// line search's full step and its sufficient-decrease test, and the step
+// itself, with no host round trip (PETSc 3.17 SNESNEWTONLS + SNESLINESEARCHBT,
+// the first trial of the search).  The host reads NewtonDev once per
+// iteration; a rejected full step is backtracked on the host.
+struct NewtonDev {
+  double fnorm;      // ||F|| at the current iterate, kept on the device across iterations
+  double ynorm;      // ||Y|| after the maxstep cut
+  double yscale;     // factor applied to Y (1, or maxstep / ||Y||)
+  double initslope;  // -(F, J Y) with PETSc's guards
+  double gnorm;      // ||F(X - Y)||
+  double xnorm;      // ||X|| once the iteration is over
+  int skip;          // the tail does nothing: the linear solve failed, or a zero or non-finite step
+  int not_accepted;  // 0 once the full step passed the sufficient-decrease test and was taken
+  int lin_ok;        // the linear solve converged
+  int ystate;        // 0 a step, 1 a zero step, 2 a non-finite step
+  int its, finished, converged, diverged, pc_status;  // the linear solve's outcome
+  int pad;
+};
+static_assert(sizeof(NewtonDev) <= 256, "NewtonDev must fit the workspace buffer allocated for it");
+
+__global__ void k_nt_gate(const GmresDev *__restrict__ gd, const int *__restrict__ status, NewtonDev *nd) {
+  nd->lin_ok = gd->finished && gd->converged && *status == 0;
+  nd->skip = !nd->lin_ok;
+  nd->not_accepted = 1;
+  nd->ystate = 0;
+  nd->yscale = 1.0;
+}
+
+__global__ void k_nt_ynorm(NewtonDev *nd, const double *__restrict__ h, double maxstep) {
+  if (nd->skip) return;
+  const double yn = sqrt(h[0]);
+  if (!isfinite(yn)) { nd->ystate = 2; nd->skip = 1; return; }
+  if (yn == 0.0) { nd->ystate = 1; nd->skip = 1; nd->ynorm = 0.0; return; }
+  nd->yscale = yn > maxstep ? maxstep / yn : 1.0;
+  nd->ynorm = yn > maxstep ? maxstep : yn;
+}
+
+__global__ void k_nt_scale(int64_t n, const NewtonDev *__restrict__ nd, double *__restrict__ Y) {
+  if (nd->skip || nd->yscale == 1.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Y[i] = nd->yscale * Y[i];
+}
+
+__global__ void k_nt_slope(NewtonDev *nd, const double *__restrict__ h) {
+  if (nd->skip) return;
+  nd->initslope = bt_initslope(h[0]);
+}
+
+__global__ void k_nt_accept(NewtonDev *nd, const double *__restrict__ h, double alpha) {
+  if (nd->skip) return;
+  const double g = sqrt(h[0]);
+  nd->gnorm = g;
+  if (bt_sufficient(nd->fnorm, g, 1.0, nd->initslope, alpha)) {
+    nd->not_accepted = 0;
+    nd->fnorm = g;
+  }
+}
+
+__global__ void k_copy_gated(int64_t n, const double *__restrict__ src, double *__restrict__ dst,
+                             const int *__restrict__ stop) {
+  if (*stop) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_nt_final(NewtonDev *nd, const double *__restrict__ h, const GmresDev *__restrict__ gd,
+                           const int *__restrict__ status) {
+  nd->xnorm = sqrt(h[0]);
+  nd->its = gd->its;
+  nd->finished = gd->finished;
+  nd->converged = gd->converged;
+  nd->diverged = gd->diverged;
+  nd->pc_status = *status;
+}
+
+__global__ void k_nt_set_fnorm(NewtonDev *nd, const double *__restrict__ h) { nd->fnorm = sqrt(h[0]); }
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
+__global__ void k_set_dyn(double *dyn, double d0, double d1) {
+  dyn[0] = d0;
+  dyn[1] = d1;
+}
+
+// nh[slot] = <x, y> (skipped with the stop flag)
+static void dot_to(tdyhip_ctx *ctx, const double *x, const double *y, int slot, const int *stop, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  KT(ctx, st, "k_dot", (double)ws.n * (x == y ? 8 : 16), k_dot<double, double><<<ws.nchunk, kRedBlock, 0, st>>>(ws.n, x, y, ws.dpartial, stop));
+  k_reduce_partials<double><<<1, kRedBlock, 0, st>>>(1, ws.nchunk, ws.dpartial, ws.nh + slot, stop);
+}
+
+constexpr double kMaxStep = 1e8, kBTAlpha = 1e-4;  // PETSc 3.17 SNESLINESEARCHBT defaults
+
+static int newton_tail(tdyhip_ctx *ctx, const Form &fm, double *X, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  NewtonDev *nd = ws.nd;
+  const int64_t n = ctx->nc;
+  const unsigned g = nblk(n, 256);
+  k_nt_gate<<<1, 1, 0, st>>>(ws.gdev, ws.status, nd);
+  dot_to(ctx, ws.Y, ws.Y, 0, &nd->skip, st);
+  k_nt_ynorm<<<1, 1, 0, st>>>(nd, ws.nh, kMaxStep);
+  k_nt_scale<<<g, 256, 0, st>>>(n, nd, ws.Y);
+  spmv_blk64(ctx, ws, ws.Y, ws.JY, st, &nd->skip);
+  dot_to(ctx, ws.F, ws.JY, 1, &nd->skip, st);
+  k_nt_slope<<<1, 1, 0, st>>>(nd, ws.nh + 1);
+  KT(ctx, st, "k_waxpy", (double)n * 24, k_waxpy<<<g, 256, 0, st>>>(n, 1.0, X, ws.Y, ws.W, &nd->skip));
+  int rc = eval_F(ctx, fm, ws.W, ws.G, st, &nd->skip);
+  if (rc) return rc;
+  dot_to(ctx, ws.G, ws.G, 2, &nd->skip, st);
+  k_nt_accept<<<1, 1, 0, st>>>(nd, ws.nh + 2, kBTAlpha);
+  k_copy_gated<<<g, 256, 0, st>>>(n, ws.W, X, &nd->not_accepted);
+  k_copy_gated<<<g, 256, 0, st>>>(n, ws.G, ws.F, &nd->not_accepted);
+  dot_to(ctx, X, X, 3, nullptr, st);
+  k_nt_final<<<1, 1, 0, st>>>(nd, ws.nh + 3, ws.gdev, ws.status);
+  return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
+}
+
+// The whole Newton iteration as one launch sequence: Jacobian, preparation,
+// `cycles` restart cycles (those past the end of the solve return at once)
+// and the tail.
+static int newton_iter_launch(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solver_opts &ok, int fp32,
+                              int cycles, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  int rc = eval_J(ctx, fm, X, ws.J, st);
+  if (rc) return rc;
+  if (fp32) {
+    gmres_prep<float>(ctx, ok, ws.F, ws.Y, st);
+    for (int c = 0; c < cycles; c++) gmres_cycle<float>(ctx, ok, ws.F, ws.Y, st);
+  } else {
+    gmres_prep<double>(ctx, ok, ws.F, ws.Y, st);
+    for (int c = 0; c < cycles; c++) gmres_cycle<double>(ctx, ok, ws.F, ws.Y, st);
+  }
+  return newton_tail(ctx, fm, X, st);
+}
+
+// Captured graphs keyed by the iterate's pointer, the form, the Krylov
+// precision and the cycle count (cycles = 0: the tail alone).
+static hipGraphExec_t newton_graph(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solver_opts &ok,
+                                   int fp32, int cycles, bool tail_only, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const int key_c = tail_only ? 0 : cycles;
+  for (auto &e : ws.newton_graphs)
+    if (e.X == X && e.type == fm.type && e.fp32 == fp32 && e.cycles == key_c) return e.exec;
+  if (ws.newton_graphs.size() >= kMaxNewtonGraphs) {
+    (void)hipStreamSynchronize(st);  // the oldest graph may still be in flight
+    (void)hipGraphExecDestroy(ws.newton_graphs.front().exec);
+    ws.newton_graphs.erase(ws.newton_graphs.begin());
+  }
+  hipGraph_t gr = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return nullptr;
+  const int rc = tail_only ? newton_tail(ctx, fm, X, st) : newton_iter_launch(ctx, fm, X, ok, fp32, cycles, st);
+  const hipError_t ec = hipStreamEndCapture(st, &gr);
+  if (rc || ec != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return nullptr;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(gr);
+  if (ex) ws.newton_graphs.push_back({X, fm.type, fp32, key_c, ex});
+  return ex;
+}
+
+static int read_newton(tdyhip_ctx *ctx, NewtonDev &h, hipStream_t st) {
+  SCHK(hipMemcpyAsync(&h, ctx->ws->nd, sizeof(NewtonDev), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  return 0;
 }
 
 // FP32 Krylov: the FP32 attempt gets at most kFp32MaxIt iterations; if it
 // does not converge (or its ILU(0) meets a zero pivot), the same Newton
 // correction is solved again in FP64 from scratch.
 constexpr int kFp32MaxIt = 1000;
-static int linsolve(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const double *b, double *y, int *its,
-                    int *fallbacks, hipStream_t st) {
-  if (!o.krylov_fp32) return gmres<double>(ctx, o, b, y, its, st);
-  tdyhip_solver_opts o32 = o;
-  o32.ksp_max_it = std::min(o.ksp_max_it, kFp32MaxIt);
-  int its32 = 0;
-  int rc = gmres<float>(ctx, o32, b, y, &its32, st);
-  if (rc != TDYHIP_ERR_NOCONV && rc != TDYHIP_ERR_SINGULAR) { *its = its32; return rc; }
-  (*fallbacks)++;
-  int its64 = 0;
-  rc = gmres<double>(ctx, o, b, y, &its64, st);
-  *its = its32 + its64;
-  return rc;
+
+// One Newton iteration up to and including the full step's test: on return h
+// holds the linear solve's outcome, the step length and, when the solve
+// converged, the residual norm at the full step and whether it was taken.
+static int newton_iteration(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solver_opts &o,
+                            tdyhip_solver_stats &stats, hipStream_t st, NewtonDev &h) {
+  SolverWS &ws = *ctx->ws;
+  const int fp32 = o.krylov_fp32 ? 1 : 0;
+  tdyhip_solver_opts ok = o;
+  if (fp32) ok.ksp_max_it = std::min(o.ksp_max_it, kFp32MaxIt);
+  const bool graphs = o.hip_graphs && !ctx->kt.on;
+  int rc;
+  if (graphs) {
+    // Jacobian, preparation, the cycles the last iteration needed and the
+    // tail in one replay; a solve that needs more cycles continues one cycle
+    // graph at a time, then replays the tail alone.
+    const int C = std::max(1, std::min(ws.cycles_hint, kMaxGraphCycles));
+    hipGraphExec_t ex = newton_graph(ctx, fm, X, ok, fp32, C, false, st);
+    if (!ex) { set_error("Newton iteration graph capture failed"); return TDYHIP_ERR_HIP; }
+    SCHK(hipGraphLaunch(ex, st));
+    if ((rc = read_newton(ctx, h, st))) return rc;
+    if (!h.finished && !h.pc_status) {
+      hipGraphExec_t cyc = fp32 ? cycle_graph<float>(ctx, ok, st) : cycle_graph<double>(ctx, ok, st);
+      if (!cyc) { set_error("GMRES cycle graph capture failed"); return TDYHIP_ERR_HIP; }
+      GmresDev hg;
+      do {
+        if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
+        SCHK(hipGraphLaunch(cyc, st));
+        SCHK(hipMemcpyAsync(&hg, ws.gdev, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+      } while (!hg.finished);
+      hipGraphExec_t tail = newton_graph(ctx, fm, X, ok, fp32, 0, true, st);
+      if (!tail) { set_error("Newton tail graph capture failed"); return TDYHIP_ERR_HIP; }
+      SCHK(hipGraphLaunch(tail, st));
+      if ((rc = read_newton(ctx, h, st))) return rc;
+    }
+  } else {
+    if ((rc = eval_J(ctx, fm, X, ws.J, st))) return rc;
+    int its = 0;
+    rc = fp32 ? gmres<float>(ctx, ok, ws.F, ws.Y, &its, st, false) : gmres<double>(ctx, ok, ws.F, ws.Y, &its, st, false);
+    if (rc && rc != TDYHIP_ERR_SINGULAR && rc != TDYHIP_ERR_NOCONV) return rc;
+    if ((rc = newton_tail(ctx, fm, X, st)) || (rc = read_newton(ctx, h, st))) return rc;
+  }
+  int lits = h.its;
+  ws.cycles_hint = std::max(1, (h.its + o.ksp_restart - 1) / o.ksp_restart);
+  if (fp32 && !h.lin_ok) {
+    // the FP32 attempt failed: the same correction in FP64, host-driven
+    stats.krylov_fallbacks++;
+    int its = 0;
+    rc = gmres<double>(ctx, o, ws.F, ws.Y, &its, st, true);
+    if (rc && rc != TDYHIP_ERR_SINGULAR && rc != TDYHIP_ERR_NOCONV) return rc;
+    if (graphs) {
+      hipGraphExec_t tail = newton_graph(ctx, fm, X, o, 0, 0, true, st);
+      if (!tail) { set_error("Newton tail graph capture failed"); return TDYHIP_ERR_HIP; }
+      SCHK(hipGraphLaunch(tail, st));
+    } else if ((rc = newton_tail(ctx, fm, X, st))) {
+      return rc;
+    }
+    if ((rc = read_newton(ctx, h, st))) return rc;
+    lits += its;
+  }
+  stats.linear_its += lits;
+  return 0;
 }
 
 // SNESSolve_NEWTONLS with SNESLINESEARCHBT and SNESConvergedDefault
 // (PETSc 3.17 defaults: atol 1e-50, stol 1e-8, maxit 50, divtol 1e4,
-// line-search alpha 1e-4, minlambda 1e-12, maxstep 1e8).
+// line-search alpha 1e-4, minlambda 1e-12, maxstep 1e8).  Each iteration is
+// one replayed graph and one read of its outcome (newton_iteration); only a
+// full step that fails the sufficient-decrease test is backtracked from the
+// host.
 static int newton(tdyhip_ctx *ctx, const Form &fm, double *X, const tdyhip_solver_opts &o, tdyhip_solver_stats &stats,
                   hipStream_t st) {
   SolverWS &ws = *ctx->ws;
   const int n = ctx->nc;
   const unsigned g = nblk(n, 256);
-  const double alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8, divtol = 1e4;  // PETSc 3.17 defaults
+  const double minlambda = 1e-12, divtol = 1e4;
   int rc;
+  k_set_dyn<<<1, 1, 0, st>>>(ws.dyn, ctx->dtime, fm.type == 0 ? 1.0 / fm.dt : 0.0);
   if ((rc = eval_F(ctx, fm, X, ws.F, st))) return rc;
-  double fnorm;
-  if ((rc = dev_norm(ctx, ws.F, &fnorm, st))) return rc;
+  dot_to(ctx, ws.F, ws.F, 0, nullptr, st);
+  k_nt_set_fnorm<<<1, 1, 0, st>>>(ws.nd, ws.nh);
+  double fnorm2;
+  SCHK(hipMemcpyAsync(&fnorm2, ws.nh, sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  double fnorm = std::sqrt(fnorm2);
   const double fnorm0 = fnorm, ttol = o.snes_rtol * fnorm;
   int status = 0, it = 0;
   if (!std::isfinite(fnorm)) { stats.status = -4; return 0; }  // SNES_DIVERGED_FNORM_NAN
   if (fnorm < o.snes_atol) { stats.status = 0; stats.last_fnorm = fnorm; return 0; }
   for (it = 1; it <= o.snes_max_it; it++) {
     if (ctx->cancel.load(std::memory_order_relaxed)) { set_error("solve cancelled"); return TDYHIP_ERR_CANCELLED; }
-    if ((rc = eval_J(ctx, fm, X, ws.J, st))) return rc;
-    int lits = 0;
-    rc = linsolve(ctx, o, ws.F, ws.Y, &lits, &stats.krylov_fallbacks, st);
-    stats.linear_its += lits;
-    if (rc == TDYHIP_ERR_SINGULAR || rc == TDYHIP_ERR_NOCONV) { status = -3; break; }
-    if (rc) return rc;
-    double ynorm;
-    if ((rc = dev_norm(ctx, ws.Y, &ynorm, st))) return rc;
-    // a non-finite Newton step is a failed linear solve (KSP_DIVERGED_NANORINF)
-    if (!std::isfinite(ynorm)) { status = -3; break; }
-    if (ynorm == 0.0) break;
-    if (ynorm > maxstep) {
-      k_scale_copy<double><<<g, 256, 0, st>>>(n, maxstep / ynorm, ws.Y, ws.Y);
-      ynorm = maxstep;
-    }
-    spmv_blk64(ctx, ws, ws.Y, ws.JY, st);
-    double dfjy;
-    if ((rc = dev_dot(ctx, ws.F, ws.JY, &dfjy, st))) return rc;
-    const double initslope = bt_initslope(dfjy);
-    // backtracking line search from the full step; a NaN or Inf residual ends
-    // it as a failure (SNES_LINESEARCH_FAILED_NANORINF)
-    auto trial = [&](double lam, double *gn) -> int {
-      KT(ctx, st, "k_waxpy", (double)n * 24, k_waxpy<<<g, 256, 0, st>>>(n, lam, X, ws.Y, ws.W));
-      int r2 = eval_F(ctx, fm, ws.W, ws.G, st);
-      return r2 ? r2 : dev_norm(ctx, ws.G, gn, st);
-    };
-    auto stop = [&]() -> int {
-      if (!ctx->cancel.load(std::memory_order_relaxed)) return 0;
-      set_error("solve cancelled");
-      return TDYHIP_ERR_CANCELLED;
-    };
-    BTParams btp;
-    btp.alpha = alpha;
-    btp.minlambda = minlambda;
-    BTResult bt;
-    if ((rc = bt_search(fnorm, initslope, btp, false, 0.0, trial, stop, bt))) return rc;
-    const double lambda = bt.lambda, gnorm = bt.gnorm;
-    if (!bt.accepted) {
-      double xnorm;
+    NewtonDev h;
+    if ((rc = newton_iteration(ctx, fm, X, o, stats, st, h))) return rc;
+    // a failed or non-finite linear solve (KSP_DIVERGED_*: SNES_DIVERGED_LINEAR_SOLVE)
+    if (!h.lin_ok || h.ystate == 2) { status = -3; break; }
+    if (h.ystate == 1) break;  // a zero step: converged
+    double ynorm = h.ynorm, lambda = 1.0, gnorm = h.gnorm, xnorm = h.xnorm;
+    if (h.not_accepted) {
+      // backtracking from the full step; a NaN or Inf residual ends it as a
+      // failure (SNES_LINESEARCH_FAILED_NANORINF)
+      auto trial = [&](double lam, double *gn) -> int {
+        KT(ctx, st, "k_waxpy", (double)n * 24, k_waxpy<<<g, 256, 0, st>>>(n, lam, X, ws.Y, ws.W));
+        int r2 = eval_F(ctx, fm, ws.W, ws.G, st);
+        return r2 ? r2 : dev_norm(ctx, ws.G, gn, st);
+      };
+      auto stop = [&]() -> int {
+        if (!ctx->cancel.load(std::memory_order_relaxed)) return 0;
+        set_error("solve cancelled");
+        return TDYHIP_ERR_CANCELLED;
+      };
+      BTParams btp;
+      btp.alpha = kBTAlpha;
+      btp.minlambda = minlambda;
+      BTResult bt;
+      if ((rc = bt_search(fnorm, h.initslope, btp, true, h.gnorm, trial, stop, bt))) return rc;
+      if (!bt.accepted) {
+        // SNES_CONVERGED_SNORM_RELATIVE when the step is negligible, else
+        // SNES_DIVERGED_LINE_SEARCH (X keeps the last accepted iterate)
+        status = (ynorm < o.snes_stol * h.xnorm) ? 0 : -6;
+        break;
+      }
+      lambda = bt.lambda;
+      gnorm = bt.gnorm;
+      SCHK(hipMemcpyAsync(X, ws.W, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      SCHK(hipMemcpyAsync(ws.F, ws.G, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      k_set_scalar<<<1, 1, 0, st>>>(&ws.nd->fnorm, gnorm);
       if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
-      // SNES_CONVERGED_SNORM_RELATIVE when the step is negligible, else
-      // SNES_DIVERGED_LINE_SEARCH (X keeps the last accepted iterate)
-      status = (ynorm < o.snes_stol * xnorm) ? 0 : -6;
-      break;
     }
-    SCHK(hipMemcpyAsync(X, ws.W, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    std::swap(ws.F, ws.G);
     fnorm = gnorm;
-    double xnorm;
-    if ((rc = dev_norm(ctx, X, &xnorm, st))) return rc;
     ynorm *= lambda;
     if (!std::isfinite(fnorm)) { status = -4; break; }
     if (fnorm < o.snes_atol) break;

@@ -181,6 +181,11 @@ struct tdyhip_ctx {
   std::vector<int32_t> pc_part;    // caller-given block-Jacobi partition (empty: breadth-first)
   int pc_part_version = 0;
   tdyhip::KernelTimers kt;
+  // Set by the Newton solver around its own residual / Jacobian launches (all
+  // on ctx->stream): device-resident dtime and shift, and a device stop flag
+  // that makes the launches return at once.  Null for the ABI callbacks.
+  const double *launch_dyn = nullptr;
+  const int *launch_stop = nullptr;
   // algorithmic-byte bookkeeping
   int64_t trans_nnz = 0;           // transmissibility entries (cells + Dirichlet columns)
   int64_t active_rows = 0;         // flux rows of faces that carry flux

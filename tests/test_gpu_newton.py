@@ -94,3 +94,33 @@ def test_device_newton_is_deterministic():
             outs.append(U.cpu().numpy())
         assert np.array_equal(outs[0], outs[1]), fp32
         assert np.array_equal(outs[0], outs[2]), fp32
+
+
+def test_newton_iteration_graph_matches_direct_launches():
+    """Each Newton iteration is one replayed graph (Jacobian, preconditioner,
+    GMRES cycles, the line search's full step and its test) with dt and the
+    TS shift read from device memory.  Replays must equal direct launches bit
+    for bit when dt changes from step to step (TDyTimeIntegratorRunToTime's
+    growth) and in the TS form, also where a solve needs more cycles than the
+    graph holds and where a full step is backtracked on the host."""
+    print("[case] Newton iteration graphs cfg0 n=10", flush=True)
+    import torch
+    from tdycore_amd.mpfao import MPFAO
+    P = cfg0_problem(n=10)
+    g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    for fp32 in (0, 1):
+        outs, stats = [], []
+        for graphs in (1, 0, 1):
+            U = torch.tensor(P["U0"], device="cuda")
+            o = g.solver_options(snes_rtol=1e-10, krylov_fp32=fp32, hip_graphs=graphs, pc_block=64, pc_coarse=1)
+            st = g.run_tdyti(U, 600.0, 1.5, 7200.0, 4 * 3600.0, o)
+            assert st["status"] == 0 and st["steps"] >= 4
+            U2 = U.clone()
+            st2 = g.run_ts_be(U2, 1800.0, 2, o)
+            assert st2["status"] == 0
+            outs.append((U.cpu().numpy(), U2.cpu().numpy()))
+            stats.append((st["newton_its"], st["linear_its"], st2["newton_its"], st2["linear_its"]))
+        for k in (1, 2):
+            assert np.array_equal(outs[0][0], outs[k][0]) and np.array_equal(outs[0][1], outs[k][1]), (fp32, k)
+            assert stats[0] == stats[k]
+    g.close()
