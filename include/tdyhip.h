@@ -108,6 +108,14 @@ int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *U
 int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot,
                                double shift, double *R, double *Jvals, tdyhip_stream stream);
 
+/* The flow equation's source_sink per cell (mpfao->source_sink, allocated zero by
+ * AllocateMemoryForSourceSinkValues, src/fv/mpfao/tdympfao.c:364-384): the TS residual
+ * subtracts Q*V (tdympfao_ts.c:211), the SNES residual Q (tdympfao_snes.c:122).  HOST array
+ * of nc entries, copied; NULL restores zero.  The reference fills it only through a
+ * forcing function (TDySetForcingFunction, src/tdycore.c:1314); manufactured solutions
+ * need it. */
+int tdyhip_set_source_sink(tdyhip_ctx *ctx, const double *Q);
+
 /* SNES form: TDySetDtimeForSNESSolver (src/tdycore.c:1575), TDyMPFAOSNESPreSolve,
  * TDyMPFAOSNESFunction, TDyMPFAOSNESJacobian (src/fv/mpfao/tdympfao_snes.c:32,67,143). */
 int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt);

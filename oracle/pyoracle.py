@@ -47,6 +47,8 @@ def lib():
                                     C.c_double, P]
         L.ora_newton_snes.argtypes = [P, P, C.c_int, C.c_double, P]
         L.ora_liquid_mass.argtypes = [P, P, P]
+        L.ora_set_source_sink.argtypes = [P, P]
+        L.ora_set_threads.argtypes = [P, C.c_int]
         _LIB = L
     return _LIB
 
@@ -54,6 +56,8 @@ def lib():
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
+
+_LIN = {"lu": 0, "gmres": 1, "gmres_bjacobi": 2}
 
 STATE = {"S": 0, "dS_dP": 1, "d2S_dP2": 2, "Kr": 3, "dKr_dS": 4, "rho": 5, "drho_dP": 6,
          "d2rho_dP2": 7, "vis": 8}
@@ -185,21 +189,25 @@ class Oracle:
     def run_ts_be(self, U, dt, nsteps, lin="lu", rtol=1e-8):
         U = np.array(U, dtype=np.float64, copy=True)
         st = np.zeros(4, np.int32)
-        lib().ora_run_ts_be(self.h, _p(U), float(dt), int(nsteps), 0 if lin == "lu" else 1, float(rtol), _p(st))
+        lib().ora_run_ts_be(self.h, _p(U), float(dt), int(nsteps), _LIN[lin], float(rtol), _p(st))
         return U, st
 
     def run_snes_steps(self, U, dt, nsteps, lin="lu", rtol=1e-8):
         U = np.array(U, dtype=np.float64, copy=True)
         st = np.zeros(4, np.int32)
-        lib().ora_run_snes_steps(self.h, _p(U), float(dt), int(nsteps), 0 if lin == "lu" else 1, float(rtol), _p(st))
+        lib().ora_run_snes_steps(self.h, _p(U), float(dt), int(nsteps), _LIN[lin], float(rtol), _p(st))
         return U, st
 
     def run_tdyti(self, U, dt_init, growth, dt_max, final_time, lin="lu", rtol=1e-8):
         U = np.array(U, dtype=np.float64, copy=True)
         st = np.zeros(4, np.int32)
         lib().ora_run_tdyti(self.h, _p(U), float(dt_init), float(growth), float(dt_max), float(final_time),
-                            0 if lin == "lu" else 1, float(rtol), _p(st))
+                            _LIN[lin], float(rtol), _p(st))
         return U, st
+
+    def set_threads(self, n):
+        """Threads of every parallel loop (OpenMP)."""
+        lib().ora_set_threads(self.h, int(n))
 
     def set_newton_maxit(self, maxit):
         lib().ora_set_newton_maxit(self.h, int(maxit))
@@ -207,8 +215,17 @@ class Oracle:
     def newton_snes(self, U, lin="lu", rtol=1e-8):
         U = np.array(U, dtype=np.float64, copy=True)
         st = np.zeros(4, np.int32)
-        s = lib().ora_newton_snes(self.h, _p(U), 0 if lin == "lu" else 1, float(rtol), _p(st))
+        s = lib().ora_newton_snes(self.h, _p(U), _LIN[lin], float(rtol), _p(st))
         return U, st, s
+
+    def set_source_sink(self, Q=None):
+        """Per-cell source_sink (None: zero): the TS residual subtracts Q*V, the
+        SNES residual Q (tdympfao_ts.c:211, tdympfao_snes.c:122)."""
+        if Q is None:
+            lib().ora_set_source_sink(self.h, None)
+        else:
+            self._src = np.ascontiguousarray(Q, np.float64)
+            lib().ora_set_source_sink(self.h, _p(self._src))
 
     def liquid_mass(self, U):
         U = np.ascontiguousarray(U, np.float64)

@@ -41,6 +41,7 @@
 #include "tdy_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -98,7 +99,18 @@ struct ora_ctx {
   int *bnd_cell;
   double *Pvec, *TP;
   double *accum_prev;
+  double *source; /* per-cell source_sink (tdympfao.c:364-384; zero unless set) */
   double dtime;
+
+  /* threads of every parallel loop (default 1).  The residual and Jacobian
+   * are formed per flux row and then summed per cell in a fixed order, so
+   * their values do not depend on the thread count. */
+  int nthreads;
+  int *ir_rowstart;  /* nv+1: first T row of each vertex (index into ir_rowface) */
+  int *row_T;        /* 4nf: offset of flux row face*4+q's T row in Tvals, -1: no T row */
+  int *row_v;        /* 4nf: vertex of the flux row */
+  double *flux_row;  /* 4nf fluxes of the last residual */
+  double *D;         /* Tvals-sized: d(flux row)/d(cell column) of the last Jacobian */
 };
 
 /* ------------------------------------------------------------------ */
@@ -746,6 +758,33 @@ ora_ctx *ora_create(int nc, int nf, int nv, int nbnd, const int *cell_verts, con
   if (build_subcells(ctx) || build_trans(ctx)) { ora_destroy(ctx); return NULL; }
   build_gravdis(ctx);
   build_jac_pattern(ctx);
+  ctx->nthreads = 1;
+  ctx->ir_rowstart = malloc(sizeof(int) * (nv + 1));
+  ctx->row_T = malloc(sizeof(int) * 4 * (size_t)nf);
+  ctx->row_v = malloc(sizeof(int) * 4 * (size_t)nf);
+  ctx->flux_row = calloc((size_t)4 * nf, sizeof(double));
+  for (size_t i = 0; i < (size_t)4 * nf; i++) { ctx->row_T[i] = -1; ctx->row_v[i] = -1; }
+  {
+    int rowcur = 0;
+    size_t tsz = 0;
+    for (int v = 0; v < nv; v++) {
+      const vtx_ir *r = &ctx->ir[v];
+      ctx->ir_rowstart[v] = rowcur;
+      for (int k = 0; k < r->nface; k++) {
+        int f = ctx->ir_faces[r->off_face + k], q = ctx->ir_subface[r->off_face + k];
+        ctx->row_v[f * 4 + q] = v;
+      }
+      for (int t = 0; t < r->nrow; t++) {
+        int k = ctx->ir_rowface[rowcur + t];
+        int f = ctx->ir_faces[r->off_face + k], q = ctx->ir_subface[r->off_face + k];
+        ctx->row_T[f * 4 + q] = r->off_T + t * r->ncol;
+      }
+      rowcur += r->nrow;
+      tsz = (size_t)r->off_T + (size_t)r->nrow * r->ncol > tsz ? (size_t)r->off_T + (size_t)r->nrow * r->ncol : tsz;
+    }
+    ctx->ir_rowstart[nv] = rowcur;
+    ctx->D = calloc(tsz ? tsz : 1, sizeof(double));
+  }
   size_t n = nc;
   ctx->S = calloc(n, 8); ctx->dS = calloc(n, 8); ctx->d2S = calloc(n, 8); ctx->Kr = calloc(n, 8);
   ctx->dKr = calloc(n, 8); ctx->rho = calloc(n, 8); ctx->drho = calloc(n, 8); ctx->d2rho = calloc(n, 8);
@@ -755,6 +794,7 @@ ora_ctx *ora_create(int nc, int nf, int nv, int nbnd, const int *cell_verts, con
   ctx->bnd_cell = calloc(nbnd + 1, sizeof(int));
   ctx->Pvec = calloc(nc + nbnd, 8); ctx->TP = calloc((size_t)4 * nf, 8);
   ctx->accum_prev = calloc(n, 8);
+  ctx->source = calloc(n, 8);
   ctx->dtime = 1.0;
   for (int f = 0; f < nf; f++) {
     int up = ctx->face_cells[2 * f], dn = ctx->face_cells[2 * f + 1];
@@ -779,7 +819,8 @@ void ora_destroy(ora_ctx *ctx) {
                 ctx->tr_rowptr, ctx->tr_cols, ctx->tr_vals, ctx->gravdis, ctx->j_rowptr, ctx->j_cols,
                 ctx->S, ctx->dS, ctx->d2S, ctx->Kr, ctx->dKr, ctx->rho, ctx->drho, ctx->d2rho,
                 ctx->vis, ctx->dvis, ctx->P_bnd, ctx->Kr_bnd, ctx->S_bnd, ctx->rho_bnd, ctx->vis_bnd,
-                ctx->bnd_cell, ctx->Pvec, ctx->TP, ctx->accum_prev};
+                ctx->bnd_cell, ctx->Pvec, ctx->TP, ctx->accum_prev, ctx->source, ctx->ir_rowstart,
+                ctx->row_T, ctx->row_v, ctx->flux_row, ctx->D};
   for (size_t i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
   free(ctx);
 }
@@ -809,6 +850,7 @@ void ora_jac_pattern(const ora_ctx *ctx, int *rowptr, int *cols) {
 
 /* TDyUpdateState_Richards_MPFAO, tdympfao.c:2527-2594 */
 void ora_update_state(ora_ctx *ctx, const double *p) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
   for (int c = 0; c < ctx->nc; c++) {
     double Pc = ctx->pref - p[c];
     vg_sat(ctx->vg_m, ctx->vg_alpha, ctx->Sr[c], Pc, &ctx->S[c], &ctx->dS[c], &ctx->d2S[c]);
@@ -831,6 +873,7 @@ void ora_boundary_pressure(const ora_ctx *ctx, double *out) { memcpy(out, ctx->P
 /* TDyMPFAO_SetBoundaryPressure (tdympfao_utils.c:690-759) and
  * TDyMPFAOUpdateBoundaryState (tdympfao_utils.c:24-99) */
 static void boundary_state(ora_ctx *ctx, const double *p) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
   for (int b = 0; b < ctx->nbnd; b++) {
     int c = ctx->bnd_cell[b];
     ctx->P_bnd[b] = ctx->p_bnd_fn ? ctx->p_bnd_fn[b] : p[c];
@@ -846,6 +889,7 @@ static void boundary_state(ora_ctx *ctx, const double *p) {
 /* MatMult(Trans_mat, P_vec, TtimesP_vec), tdympfao_ts.c:184 */
 static void trans_matmult(ora_ctx *ctx) {
   int nr = 4 * ctx->nf;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
   for (int i = 0; i < nr; i++) {
     double s = 0.0;
     for (int k = ctx->tr_rowptr[i]; k < ctx->tr_rowptr[i + 1]; k++) s += ctx->tr_vals[k] * ctx->Pvec[ctx->tr_cols[k]];
@@ -853,8 +897,12 @@ static void trans_matmult(ora_ctx *ctx) {
   }
 }
 
-/* TDyMPFAOIFunction_Vertices, tdympfao_ts.c:12-150 */
+/* TDyMPFAOIFunction_Vertices, tdympfao_ts.c:12-150: the flux of every
+ * sub-face row of every vertex (each row belongs to one vertex), then each
+ * cell sums the rows of its faces (+ for the up cell, - for the down cell) in
+ * a fixed (face, sub-face) order. */
 static void flux_vertices(ora_ctx *ctx, double *R) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(dynamic, 256)
   for (int v = 0; v < ctx->nv; v++) {
     vtx_ir *r = &ctx->ir[v];
     for (int k = 0; k < r->nface; k++) {
@@ -886,9 +934,20 @@ static void flux_vertices(ora_ctx *ctx, double *R) {
         fluxm = den * ukvr * (-TP);
         fluxm += -pow(den, 2.0) * ukvr * G;
       }
-      if (up >= 0) R[up] += fluxm;
-      if (dn >= 0) R[dn] -= fluxm;
+      ctx->flux_row[f * 4 + q] = fluxm;
     }
+  }
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+  for (int c = 0; c < ctx->nc; c++) {
+    double s = 0.0;
+    for (int lf = 0; lf < 6; lf++) {
+      int f = ctx->cell_faces[c * 6 + lf];
+      if (f < 0) continue;
+      const double *fr = ctx->flux_row + (size_t)f * 4;
+      double fs = ((fr[0] + fr[1]) + fr[2]) + fr[3];
+      s += (ctx->face_cells[2 * f] == c) ? fs : -fs;
+    }
+    R[c] += s;
   }
 }
 
@@ -907,7 +966,7 @@ int ora_ifunction(ora_ctx *ctx, const double *U, const double *Udot, double *R) 
     double rho = ctx->rho[c], drho = ctx->drho[c], phi = ctx->phi[c], dphi = 0.0, S = ctx->S[c], dS = ctx->dS[c];
     double dm = rho * dphi * S + drho * phi * S + rho * phi * dS;
     R[c] += dm * Udot[c] * ctx->cvol[c];
-    R[c] -= 0.0 * ctx->cvol[c];
+    R[c] -= ctx->source[c] * ctx->cvol[c]; /* tdympfao_ts.c:211 */
   }
   return 0;
 }
@@ -926,9 +985,10 @@ static int jpos(const ora_ctx *ctx, int row, int col) {
 /* TDyMPFAOIJacobian_Vertices, tdympfao_ts.c:235-456 (uses the state of the
  * preceding residual evaluation, like the reference) */
 static void jac_vertices(ora_ctx *ctx, double *J) {
-  int rowcur = 0;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(dynamic, 256)
   for (int v = 0; v < ctx->nv; v++) {
     vtx_ir *r = &ctx->ir[v];
+    const int rowcur = ctx->ir_rowstart[v];
     /* map face k -> T row */
     for (int t = 0; t < r->nrow; t++) {
       int k = ctx->ir_rowface[rowcur + t];
@@ -971,12 +1031,29 @@ static void jac_vertices(ora_ctx *ctx, double *J) {
           Jac = dden_dn * ukvr * TP + den * dukvr_dn * TP + den * ukvr * T + 2 * den * dden_dn * ukvr * G +
                 pow(den, 2.0) * dukvr_dn * G;
         else Jac = den * ukvr * T;
-        Jac = -Jac;
-        if (up >= 0) J[jpos(ctx, up, cell)] += Jac;
-        if (dn >= 0) J[jpos(ctx, dn, cell)] += -Jac;
+        ctx->D[r->off_T + (size_t)t * r->ncol + icol] = -Jac;
       }
     }
-    rowcur += r->nrow;
+  }
+  /* each cell sums its faces' rows: + for the up cell, - for the down cell
+   * (the reference's MatSetValuesLocal calls, tdympfao_ts.c:439-444) */
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+  for (int c = 0; c < ctx->nc; c++) {
+    for (int lf = 0; lf < 6; lf++) {
+      int f = ctx->cell_faces[c * 6 + lf];
+      if (f < 0) continue;
+      const int is_up = ctx->face_cells[2 * f] == c;
+      for (int q = 0; q < 4; q++) {
+        const int rt = ctx->row_T[f * 4 + q];
+        if (rt < 0) continue;
+        const vtx_ir *r = &ctx->ir[ctx->row_v[f * 4 + q]];
+        for (int icol = 0; icol < r->ncell; icol++) {
+          int cell = ctx->ir_cells[r->off_cell + icol];
+          double d = ctx->D[rt + icol];
+          J[jpos(ctx, c, cell)] += is_up ? d : -d;
+        }
+      }
+    }
   }
 }
 
@@ -1010,7 +1087,7 @@ int ora_snes_function(ora_ctx *ctx, const double *U, double *R) {
   for (int c = 0; c < ctx->nc; c++) {
     double acc = ctx->rho[c] * ctx->phi[c] * ctx->S[c] * ctx->cvol[c] / ctx->dtime;
     R[c] += acc - ctx->accum_prev[c];
-    R[c] -= 0.0;
+    R[c] -= ctx->source[c]; /* not scaled by the volume, tdympfao_snes.c:122 */
   }
   return 0;
 }
@@ -1026,6 +1103,12 @@ int ora_snes_jacobian(ora_ctx *ctx, const double *U, double *J) {
   return 0;
 }
 
+/* the flow equation's source_sink per cell (zero by default, as the
+ * reference's AllocateMemoryForSourceSinkValues leaves it); NULL clears it */
+void ora_set_source_sink(ora_ctx *ctx, const double *Q) {
+  for (int c = 0; c < ctx->nc; c++) ctx->source[c] = Q ? Q[c] : 0.0;
+}
+
 void ora_liquid_mass(ora_ctx *ctx, const double *U, double *out) {
   ora_update_state(ctx, U);
   for (int c = 0; c < ctx->nc; c++) out[c] = ctx->rho[c] * ctx->cvol[c];
@@ -1034,7 +1117,13 @@ void ora_liquid_mass(ora_ctx *ctx, const double *U, double *out) {
 /* ------------------------------------------------------------------ */
 /* linear solvers                                                       */
 /* ------------------------------------------------------------------ */
-static double nrm2(int n, const double *x) { double s = 0.0; for (int i = 0; i < n; i++) s += x[i] * x[i]; return sqrt(s); }
+static double vdot(const ora_ctx *ctx, int n, const double *x, const double *y) {
+  double s = 0.0;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static) reduction(+ : s)
+  for (int i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+static double nrm2c(const ora_ctx *ctx, int n, const double *x) { return sqrt(vdot(ctx, n, x, x)); }
 
 static int dense_solve(const ora_ctx *ctx, const double *J, const double *F, double *Y) {
   int n = ctx->nc;
@@ -1087,7 +1176,62 @@ static void ilu_apply(const ora_ctx *ctx, const double *LU, const int *diag, con
     x[i] = s / LU[diag[i]];
   }
 }
+/* Block-Jacobi ILU(0) (PETSc's parallel default, PCBJACOBI with PCILU in
+ * each block): B contiguous row blocks, entries coupling two blocks dropped,
+ * ILU(0) of each block in natural order; blocks are factored and applied in
+ * parallel. */
+static int bj_ilu0(const ora_ctx *ctx, int B, double *LU, int *diag) {
+  int n = ctx->nc;
+  const int *rp = ctx->j_rowptr, *cj = ctx->j_cols;
+  for (int i = 0; i < n; i++) diag[i] = jpos(ctx, i, i);
+  int bad = 0;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static, 1) reduction(| : bad)
+  for (int b = 0; b < B; b++) {
+    int lo = (int)((long long)b * n / B), hi = (int)((long long)(b + 1) * n / B);
+    int *pos = malloc(sizeof(int) * (size_t)(hi - lo + 1));
+    for (int i = 0; i < hi - lo; i++) pos[i] = -1;
+    for (int i = lo; i < hi; i++) {
+      for (int k = rp[i]; k < rp[i + 1]; k++) if (cj[k] >= lo && cj[k] < hi) pos[cj[k] - lo] = k;
+      for (int k = rp[i]; k < rp[i + 1] && cj[k] < i; k++) {
+        int col = cj[k];
+        if (col < lo) continue;
+        double l = LU[k] / LU[diag[col]];
+        LU[k] = l;
+        for (int kk = diag[col] + 1; kk < rp[col + 1]; kk++) {
+          int c2 = cj[kk];
+          if (c2 < lo || c2 >= hi) continue;
+          int p = pos[c2 - lo];
+          if (p >= 0) LU[p] -= l * LU[kk];
+        }
+      }
+      for (int k = rp[i]; k < rp[i + 1]; k++) if (cj[k] >= lo && cj[k] < hi) pos[cj[k] - lo] = -1;
+      if (LU[diag[i]] == 0.0) bad = 1;
+    }
+    free(pos);
+  }
+  return bad;
+}
+static void bj_apply(const ora_ctx *ctx, int B, const double *LU, const int *diag, const double *bv, double *x) {
+  int n = ctx->nc;
+  const int *rp = ctx->j_rowptr, *cj = ctx->j_cols;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static, 1)
+  for (int b = 0; b < B; b++) {
+    int lo = (int)((long long)b * n / B), hi = (int)((long long)(b + 1) * n / B);
+    for (int i = lo; i < hi; i++) {
+      double s = bv[i];
+      for (int k = rp[i]; k < diag[i]; k++) if (cj[k] >= lo) s -= LU[k] * x[cj[k]];
+      x[i] = s;
+    }
+    for (int i = hi - 1; i >= lo; i--) {
+      double s = x[i];
+      for (int k = diag[i] + 1; k < rp[i + 1]; k++) if (cj[k] < hi) s -= LU[k] * x[cj[k]];
+      x[i] = s / LU[diag[i]];
+    }
+  }
+}
+
 static void csr_mv(const ora_ctx *ctx, const double *A, const double *x, double *y) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
   for (int i = 0; i < ctx->nc; i++) {
     double s = 0.0;
     for (int k = ctx->j_rowptr[i]; k < ctx->j_rowptr[i + 1]; k++) s += A[k] * x[ctx->j_cols[k]];
@@ -1095,50 +1239,55 @@ static void csr_mv(const ora_ctx *ctx, const double *A, const double *x, double 
   }
 }
 
-/* GMRES(30), left preconditioned with ILU(0), classical Gram-Schmidt,
- * stopping on the preconditioned residual (PETSc KSP defaults: rtol 1e-5,
- * atol 1e-50, max_it 10000) */
-static int gmres_ilu(const ora_ctx *ctx, const double *J, const double *F, double *Y, int *its_out) {
+/* GMRES(30), left preconditioned with ILU(0) (B = 0: global, natural order;
+ * B > 0: block Jacobi with B blocks), classical Gram-Schmidt, stopping on the
+ * preconditioned residual (PETSc KSP defaults: rtol 1e-5, atol 1e-50,
+ * max_it 10000) */
+static int gmres_ilu(const ora_ctx *ctx, const double *J, const double *F, double *Y, int *its_out, int B) {
   int n = ctx->nc, m = 30;
   size_t nnz = ctx->j_rowptr[n];
   double *LU = malloc(nnz * sizeof(double));
   int *diag = malloc(n * sizeof(int));
   memcpy(LU, J, nnz * sizeof(double));
-  if (ilu0(ctx, LU, diag)) { free(LU); free(diag); return 1; }
+  if (B > 0 ? bj_ilu0(ctx, B, LU, diag) : ilu0(ctx, LU, diag)) { free(LU); free(diag); return 1; }
+#define PC_APPLY(b_, x_) (B > 0 ? bj_apply(ctx, B, LU, diag, (b_), (x_)) : ilu_apply(ctx, LU, diag, (b_), (x_)))
   double *V = malloc((size_t)(m + 1) * n * sizeof(double));
   double *H = calloc((size_t)(m + 1) * m, sizeof(double));
   double *cs = malloc(m * sizeof(double)), *sn = malloc(m * sizeof(double)), *g = malloc((m + 1) * sizeof(double));
   double *w = malloc(n * sizeof(double)), *t = malloc(n * sizeof(double));
   memset(Y, 0, n * sizeof(double));
-  ilu_apply(ctx, LU, diag, F, t);
-  double r0 = nrm2(n, t), tol = 1e-5 * r0;
+  PC_APPLY(F, t);
+  double r0 = nrm2c(ctx, n, t), tol = 1e-5 * r0;
   int its = 0, status = 0;
   if (r0 == 0.0) goto done;
   for (;;) {
     /* r = M^-1 (F - J Y) */
     csr_mv(ctx, J, Y, w);
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
     for (int i = 0; i < n; i++) w[i] = F[i] - w[i];
-    ilu_apply(ctx, LU, diag, w, t);
-    double beta = nrm2(n, t);
+    PC_APPLY(w, t);
+    double beta = nrm2c(ctx, n, t);
     if (beta <= tol || beta < 1e-50) break;
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
     for (int i = 0; i < n; i++) V[i] = t[i] / beta;
     memset(g, 0, (m + 1) * sizeof(double)); g[0] = beta;
     int j;
     for (j = 0; j < m; j++) {
       csr_mv(ctx, J, V + (size_t)j * n, w);
-      ilu_apply(ctx, LU, diag, w, t);
-      for (int i = 0; i <= j; i++) {
-        double h = 0.0; const double *vi = V + (size_t)i * n;
-        for (int q = 0; q < n; q++) h += t[q] * vi[q];
-        H[i + j * (m + 1)] = h;
+      PC_APPLY(w, t);
+      for (int i = 0; i <= j; i++) H[i + j * (m + 1)] = vdot(ctx, n, t, V + (size_t)i * n);
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+      for (int q = 0; q < n; q++) {
+        double tq = t[q];
+        for (int i = 0; i <= j; i++) tq -= H[i + j * (m + 1)] * V[(size_t)i * n + q];
+        t[q] = tq;
       }
-      for (int i = 0; i <= j; i++) {
-        double h = H[i + j * (m + 1)]; const double *vi = V + (size_t)i * n;
-        for (int q = 0; q < n; q++) t[q] -= h * vi[q];
-      }
-      double hn = nrm2(n, t);
+      double hn = nrm2c(ctx, n, t);
       H[j + 1 + j * (m + 1)] = hn;
-      if (hn > 0) for (int q = 0; q < n; q++) V[(size_t)(j + 1) * n + q] = t[q] / hn;
+      if (hn > 0) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+        for (int q = 0; q < n; q++) V[(size_t)(j + 1) * n + q] = t[q] / hn;
+      }
       for (int i = 0; i < j; i++) {
         double a = H[i + j * (m + 1)], b = H[i + 1 + j * (m + 1)];
         H[i + j * (m + 1)] = cs[i] * a + sn[i] * b;
@@ -1159,7 +1308,12 @@ static int gmres_ilu(const ora_ctx *ctx, const double *J, const double *F, doubl
       for (int k = i + 1; k < j; k++) s -= H[i + k * (m + 1)] * yk[k];
       yk[i] = s / H[i + i * (m + 1)];
     }
-    for (int i = 0; i < j; i++) { const double *vi = V + (size_t)i * n; for (int q = 0; q < n; q++) Y[q] += yk[i] * vi[q]; }
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+    for (int q = 0; q < n; q++) {
+      double yq = Y[q];
+      for (int i = 0; i < j; i++) yq += yk[i] * V[(size_t)i * n + q];
+      Y[q] = yq;
+    }
     free(yk);
     if (fabs(g[j]) <= tol) break;
     /* KSP_DIVERGED_ITS is a linear-solve failure for SNES (SNESCheckKSPSolve) */
@@ -1169,6 +1323,7 @@ done:
   *its_out = its;
   free(LU); free(diag); free(V); free(H); free(cs); free(sn); free(g); free(w); free(t);
   return status;
+#undef PC_APPLY
 }
 
 /* ------------------------------------------------------------------ */
@@ -1206,42 +1361,47 @@ static int newton(ora_ctx *ctx, double *X, fun_t fun, jac_t jac, void *aux, int 
   const double atol = 1e-50, stol = 1e-8, divtol = 1e4, alpha = 1e-4, minlambda = 1e-12, maxstep = 1e8;
   const int maxit = ctx->newton_maxit;
   fun(ctx, X, F, aux);
-  double fnorm = nrm2(n, F), fnorm0 = fnorm, ttol = rtol * fnorm;
+  double fnorm = nrm2c(ctx, n, F), fnorm0 = fnorm, ttol = rtol * fnorm;
   int status = 0, it = 0, linits = 0;
   if (!(fnorm == fnorm)) { status = -4; goto out; }
   if (fnorm < atol) goto out;
   for (it = 1; it <= maxit; it++) {
     jac(ctx, X, Jv, aux);
     int li = 0;
-    int err = (lin == 0) ? dense_solve(ctx, Jv, F, Y) : gmres_ilu(ctx, Jv, F, Y, &li);
+    int err = (lin == 0) ? dense_solve(ctx, Jv, F, Y) : gmres_ilu(ctx, Jv, F, Y, &li, lin == 2 ? ctx->nthreads : 0);
     linits += (lin == 0) ? 1 : li;
     if (err) { status = -3; break; }
     /* line search bt */
-    double ynorm = nrm2(n, Y);
+    double ynorm = nrm2c(ctx, n, Y);
     if (ynorm == 0.0) break; /* SNESLineSearchApply_BT: zero step -> converged on snorm */
-    if (ynorm > maxstep) { for (int i = 0; i < n; i++) Y[i] *= maxstep / ynorm; ynorm = maxstep; }
-    /* initslope = -(F, J Y) */
-    for (int i = 0; i < n; i++) {
-      double s = 0.0;
-      for (int k = ctx->j_rowptr[i]; k < ctx->j_rowptr[i + 1]; k++) s += Jv[k] * Y[ctx->j_cols[k]];
-      JY[i] = s;
+    if (ynorm > maxstep) {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+      for (int i = 0; i < n; i++) Y[i] *= maxstep / ynorm;
+      ynorm = maxstep;
     }
-    double initslope = 0.0;
-    for (int i = 0; i < n; i++) initslope += F[i] * JY[i];
+    /* initslope = -(F, J Y) */
+    csr_mv(ctx, Jv, Y, JY);
+    double initslope = vdot(ctx, n, F, JY);
     if (initslope > 0.0) initslope = -initslope;
     if (initslope == 0.0) initslope = -1.0;
     double lambda = 1.0;
-    for (int i = 0; i < n; i++) W[i] = X[i] - lambda * Y[i];
+    {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+      for (int i = 0; i < n; i++) W[i] = X[i] - lambda * Y[i];
+    }
     fun(ctx, W, Gv, aux);
-    double gnorm = nrm2(n, Gv);
+    double gnorm = nrm2c(ctx, n, Gv);
     if (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
       double lambdaprev = lambda, gnormprev = gnorm;
       double lt = -initslope / (gnorm * gnorm - fnorm * fnorm - 2.0 * lambda * initslope);
       if (lt > 0.5 * lambda) lt = 0.5 * lambda;
       if (lt <= 0.1 * lambda) lambda = 0.1 * lambda; else lambda = lt;
+      {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
       for (int i = 0; i < n; i++) W[i] = X[i] - lambda * Y[i];
+    }
       fun(ctx, W, Gv, aux);
-      gnorm = nrm2(n, Gv);
+      gnorm = nrm2c(ctx, n, Gv);
       while (!(0.5 * gnorm * gnorm <= 0.5 * fnorm * fnorm + lambda * alpha * initslope)) {
         if (lambda <= minlambda) { status = -6; break; }
         double t1 = 0.5 * (gnorm * gnorm - fnorm * fnorm) - lambda * initslope;
@@ -1254,21 +1414,24 @@ static int newton(ora_ctx *ctx, double *X, fun_t fun, jac_t jac, void *aux, int 
         lambdaprev = lambda; gnormprev = gnorm;
         if (lt > 0.5 * lambda) lt = 0.5 * lambda;
         if (lt <= 0.1 * lambda) lambda = 0.1 * lambda; else lambda = lt;
-        for (int i = 0; i < n; i++) W[i] = X[i] - lambda * Y[i];
+        {
+#pragma omp parallel for num_threads(ctx->nthreads) schedule(static)
+      for (int i = 0; i < n; i++) W[i] = X[i] - lambda * Y[i];
+    }
         fun(ctx, W, Gv, aux);
-        gnorm = nrm2(n, Gv);
+        gnorm = nrm2c(ctx, n, Gv);
       }
       if (status) {
         /* SNESSolve_NEWTONLS: a failed line search whose step is below
          * stol*||x|| counts as SNES_CONVERGED_SNORM_RELATIVE, X unchanged */
-        if (ynorm < stol * nrm2(n, X)) status = 0;
+        if (ynorm < stol * nrm2c(ctx, n, X)) status = 0;
         break;
       }
     }
     memcpy(X, W, n * sizeof(double));
     memcpy(F, Gv, n * sizeof(double));
     fnorm = gnorm;
-    double xnorm = nrm2(n, X);
+    double xnorm = nrm2c(ctx, n, X);
     ynorm *= lambda;
     if (!(fnorm == fnorm)) { status = -4; break; }
     if (fnorm < atol) break;
@@ -1284,6 +1447,7 @@ out:
 }
 
 void ora_set_newton_maxit(ora_ctx *ctx, int maxit) { ctx->newton_maxit = maxit > 0 ? maxit : 50; }
+void ora_set_threads(ora_ctx *ctx, int n) { ctx->nthreads = n > 0 ? n : 1; }
 
 int ora_newton_snes(ora_ctx *ctx, double *U, int lin, double rtol, int *stats) {
   return newton(ctx, U, sn_fun, sn_jac, NULL, lin, rtol, stats);

@@ -56,7 +56,13 @@ int ora_snes_presolve(ora_ctx *ctx, const double *Uprev);
 int ora_snes_function(ora_ctx *ctx, const double *U, double *R);
 int ora_snes_jacobian(ora_ctx *ctx, const double *U, double *Jvals);
 
-/* drivers.  lin: 0 dense LU, 1 GMRES(30)+ILU(0).  stats (int[4]):
+/* threads of every parallel loop (OpenMP; default 1).  Results do not depend on
+ * it except through the order of the Krylov solver's dot products. */
+void ora_set_threads(ora_ctx *ctx, int n);
+
+/* drivers.  lin: 0 dense LU, 1 GMRES(30)+ILU(0) in natural order, 2
+ * GMRES(30)+block-Jacobi ILU(0) with one contiguous row block per thread
+ * (PETSc's parallel default).  stats (int[4]):
  * newton iterations, linear iterations, steps, status (0 ok) */
 int ora_run_ts_be(ora_ctx *ctx, double *U, double dt, int nsteps, int lin,
                   double snes_rtol, int *stats);
@@ -72,6 +78,9 @@ int ora_newton_snes(ora_ctx *ctx, double *U, int lin, double snes_rtol, int *sta
 void ora_set_newton_maxit(ora_ctx *ctx, int maxit);
 
 void ora_liquid_mass(ora_ctx *ctx, const double *U, double *out);
+/* source_sink per cell: the TS residual subtracts Q*V, the SNES residual Q
+ * (tdympfao_ts.c:211, tdympfao_snes.c:122); NULL clears it */
+void ora_set_source_sink(ora_ctx *ctx, const double *Q);
 
 #ifdef __cplusplus
 }

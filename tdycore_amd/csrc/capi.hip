@@ -245,6 +245,7 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells;
   a.face_flags = ctx->d_face_flags; a.gtab = ctx->d_gtab;
   a.jdslot = ctx->d_jdslot; a.flux = ctx->d_flux; a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
+  a.src = ctx->has_source ? ctx->d_source : nullptr;
   a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
   a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
   a.perm = ell ? perm : nullptr;
@@ -923,6 +924,19 @@ int tdyhip_ifunction_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const
   int rc = launch_state(ctx, 0, U, Udot, st);
   if (rc) return rc;
   return launch_flux(ctx, 0, shift, U, Udot, R, J, st);
+}
+
+int tdyhip_set_source_sink(tdyhip_ctx *ctx, const double *Q) {
+  if (!ctx) { set_error("null context"); return TDYHIP_ERR_ARG; }
+  if (!Q) { ctx->has_source = false; return 0; }
+  for (int c = 0; c < ctx->nc; c++)
+    if (!std::isfinite(Q[c])) { set_error("source_sink is not finite"); return TDYHIP_ERR_ARG; }
+  int rc;
+  if (!ctx->d_source && (rc = dalloc(ctx, &ctx->d_source, (size_t)ctx->nc))) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // earlier launches may still read the old values
+  HIPCHK(copy_sync(ctx, ctx->d_source, Q, sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
+  ctx->has_source = true;
+  return 0;
 }
 
 int tdyhip_set_dtime(tdyhip_ctx *ctx, double dt) {

@@ -578,6 +578,7 @@ struct GatherArgs {
   const uint8_t *face_flags, *jdslot;
   const uint64_t *gtab;  // 24*nc: see k_gather_table
   const double *flux, *acc, *dM, *d2M, *Udot, *vol, *Jrow;
+  const double *src;   // source_sink per cell, or nullptr (TS form: scaled by the volume)
   const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
   const int32_t *perm; // ELL: position of row c (nullptr: c)
   int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or the ELL row count
@@ -613,7 +614,10 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
       const double fs = ((q.x + q.y) + q.z) + q.w;
       s += (a.face_cells[2 * f] == c) ? fs : -fs;
     }
-    a.R[c] = s + a.acc[c];
+    // fluxes, then accumulation, then the source (tdympfao_ts.c:191-212, tdympfao_snes.c:112-123)
+    double r = s + a.acc[c];
+    if (a.src) r -= (a.form == 0) ? a.src[c] * a.vol[c] : a.src[c];
+    a.R[c] = r;
   }
   if (WITH_J) {
 #pragma unroll

@@ -169,6 +169,8 @@ struct tdyhip_ctx {
   double *d_flux = nullptr;         // 4nf
   double *d_jrow = nullptr;         // 4nf*kMaxCols flux-row derivatives (row-major, 64 B per row)
   double *d_accum_prev = nullptr;   // nc
+  double *d_source = nullptr;       // nc source_sink (tdyhip_set_source_sink)
+  bool has_source = false;
 
   // host copies used by diagnostics
   std::vector<int32_t> h_jrp, h_jcols;

@@ -120,9 +120,13 @@ def _face_geometry(X):
 
 
 def box_mesh(nx: int, ny: int, nz: int, lower=(0.0, 0.0, 0.0), upper=(1.0, 1.0, 1.0),
-             jitter: float = 0.0, seed: int = 0) -> Mesh:
+             jitter: float = 0.0, seed: int = 0, warp: float = 0.0) -> Mesh:
     """Hex box mesh of nx*ny*nz cells.  ``jitter`` moves every interior vertex
-    by U(-jitter, +jitter) times the local spacing per coordinate."""
+    by U(-jitter, +jitter) times the local spacing per coordinate.  ``warp``
+    maps every vertex by the smooth displacement warp * L * s(x) s(y) s(z) in
+    each coordinate, s = sin(2 pi (x - lower) / L) per axis (L the box
+    length), which keeps the box's faces in place and perturbs the cells
+    smoothly and independently of the resolution (faces become non-planar)."""
     lower = np.asarray(lower, dtype=np.float64)
     upper = np.asarray(upper, dtype=np.float64)
     h = (upper - lower) / np.array([nx, ny, nz], dtype=np.float64)
@@ -136,6 +140,11 @@ def box_mesh(nx: int, ny: int, nz: int, lower=(0.0, 0.0, 0.0), upper=(1.0, 1.0, 
     V[I == nx, 0] = upper[0]
     V[J == ny, 1] = upper[1]
     V[K == nz, 2] = upper[2]
+    if warp != 0.0:
+        L = upper - lower
+        t = 2.0 * np.pi * (V - lower[None, :]) / L[None, :]
+        d = warp * np.sin(t[:, 0]) * np.sin(t[:, 1]) * np.sin(t[:, 2])
+        V = V + d[:, None] * L[None, :]
     if jitter > 0.0:
         rng = np.random.default_rng(seed)
         inner = (I > 0) & (I < nx) & (J > 0) & (J < ny) & (K > 0) & (K < nz)

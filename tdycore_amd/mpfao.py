@@ -159,6 +159,19 @@ class MPFAO:
         self._run(_lib.load().tdyhip_ifunction_ijacobian, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(R),
                   _ptr(J), None)
 
+    def SetSourceSink(self, Q=None):
+        """Per-cell source_sink (None: zero), the reference's mpfao->source_sink:
+        IFunction subtracts Q*V (tdympfao_ts.c:211), SNESFunction Q
+        (tdympfao_snes.c:122)."""
+        L = _lib.load()
+        if Q is None:
+            _lib.check(L.tdyhip_set_source_sink(self._h, None))
+        else:
+            q = np.ascontiguousarray(Q, dtype=np.float64)
+            if q.shape != (self.nc,):
+                raise ValueError("source_sink needs one value per cell")
+            _lib.check(L.tdyhip_set_source_sink(self._h, q.ctypes.data))
+
     # ---- SNES callbacks ----
     def SetDtime(self, dt):
         _lib.check(_lib.load().tdyhip_set_dtime(self._h, float(dt)))

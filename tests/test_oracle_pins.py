@@ -267,3 +267,66 @@ def test_hydrostatic_pin_rejects_a_reversed_gradient():
     R = o.ifunction(2e5 + rho * g * m.cell_cen[:, 2], np.zeros(m.nc))
     scale = rho * rho / 9.94e-4 * np.abs(o.gravdis()).max()
     assert np.abs(R).max() > 1e-1 * scale
+
+
+@pytest.mark.parametrize("warp", [0.0, 0.03])
+def test_oracle_manufactured_solution_is_second_order(warp):
+    """The oracle's whole path (G matrices, local systems, residual with the
+    source term, Jacobian, Newton) on the manufactured saturated problem
+    (workloads.mms_problem, full anisotropic tensor): the pressure error falls
+    as h^2 on orthogonal and on smoothly warped (non-planar) hexahedra.  A
+    wrong sub-face area, normal orientation or source sign would break the
+    rate."""
+    from pyoracle import Oracle
+    from tdycore_amd import workloads as w
+    errs = []
+    for n in (4, 8, 16):
+        P = w.mms_problem(n, warp=warp)
+        o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        o.set_source_sink(P["Q"])
+        o.set_dtime(1e6)
+        o.snes_presolve(P["U0"])
+        U, _, s = o.newton_snes(P["U0"], lin="gmres", rtol=1e-13)
+        assert s == 0
+        errs.append(w.mms_error(P, U))
+    rates = [np.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert min(rates) >= 1.85 and rates[-1] >= 1.9, rates
+
+
+def test_oracle_manufactured_solution_fails_with_a_wrong_source_sign():
+    """The same problem with the source's sign flipped does not converge to p."""
+    from pyoracle import Oracle
+    from tdycore_amd import workloads as w
+    P = w.mms_problem(8)
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    o.set_source_sink(-P["Q"])
+    o.set_dtime(1e6)
+    o.snes_presolve(P["U0"])
+    U, _, s = o.newton_snes(P["U0"], lin="gmres", rtol=1e-13)
+    assert w.mms_error(P, U) > 0.1
+
+
+def test_oracle_threads_do_not_change_residual_or_jacobian():
+    """The residual and Jacobian are formed per flux row and summed per cell
+    in a fixed order: 1 and 4 threads give the same bits.  Newton with
+    GMRES + block-Jacobi ILU(0) (one block per thread) reaches the solution
+    of Newton with direct solves."""
+    from tdycore_amd import workloads
+    P = workloads.layered_box_problem(12, 10, 8)
+    nc = P["mesh"].nc
+    rng = np.random.default_rng(4)
+    U = P["U0"] + 2e4 * (rng.random(nc) - 0.5)
+    Ud = 1e-2 * (rng.random(nc) - 0.5)
+    outs = []
+    for nt in (1, 4):
+        o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+        o.set_threads(nt)
+        outs.append((o.ifunction(U, Ud), o.ijacobian(U, Ud, 1.0 / 3600)))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    o = Oracle(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
+    Ulu, st = o.run_snes_steps(P["U0"], 86400.0, 1, lin="lu", rtol=1e-12)
+    assert st[3] == 0
+    o.set_threads(4)
+    Ubj, st = o.run_snes_steps(P["U0"], 86400.0, 1, lin="gmres_bjacobi", rtol=1e-12)
+    assert st[3] == 0
+    assert np.max(np.abs(Ubj - Ulu) / np.abs(Ulu)) <= 1e-9
