@@ -50,3 +50,19 @@ def test_create_rejects_bad_mesh_without_touching_the_gpu():
     rc = L.tdyhip_create(ctypes.byref(ms), ctypes.byref(mt), ctypes.byref(h))
     assert rc == 1
     assert b"empty mesh" in L.tdyhip_last_error()
+
+
+def test_petsc_binding_calls_only_declared_entry_points():
+    """The reference-side PETSc binding (tdycore_amd/petsc/tdympfao_hip.c, not
+    compiled here: PETSc is absent) calls only entry points include/tdyhip.h
+    declares, and defines the five callbacks with the reference's names."""
+    src = open(os.path.join(ROOT, "tdycore_amd", "petsc", "tdympfao_hip.c")).read()
+    code = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    called = set(re.findall(r"\b(tdyhip_[a-z0-9_]+)\s*\(", code))
+    assert called and called <= _declared(), called - _declared()
+    for cb in ("TDyMPFAOIFunction_HIP(TS ts, PetscReal t, Vec U, Vec U_t, Vec R, void *ctx)",
+               "TDyMPFAOIJacobian_HIP(TS ts, PetscReal t, Vec U, Vec U_t, PetscReal shift, Mat A, Mat B, void *ctx)",
+               "TDyMPFAOSNESFunction_HIP(SNES snes, Vec U, Vec R, void *ctx)",
+               "TDyMPFAOSNESJacobian_HIP(SNES snes, Vec U, Mat A, Mat B, void *ctx)",
+               "TDyMPFAOSNESPreSolve_HIP(TDy tdy)"):
+        assert "PetscErrorCode " + cb + " {" in code, cb
