@@ -99,6 +99,16 @@ int tdyhip_jacobian_pattern_device(const tdyhip_ctx *ctx, const int32_t **rowptr
 /* TS residual: replaces TDyMPFAOIFunction (src/fv/mpfao/tdympfao_ts.c:153). R is overwritten. */
 int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R,
                      tdyhip_stream stream);
+/* The TS residual of tdyhip_ifunction, allocating nothing: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels and the same result, bit
+ * for bit, but the per-flux-row fluxes go to a fixed scratch array in device memory that the
+ * library holds for the life of the process, one per GPU, instead of the context's own buffer.
+ * Meshes of at most 2,097,152 faces (8,388,608 flux rows, 64 MiB); larger meshes get
+ * TDYHIP_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from two streams or two contexts) race. */
+int tdyhip_ifunction_noalloc(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R,
+                             tdyhip_stream stream);
 /* TS Jacobian: replaces TDyMPFAOIJacobian (src/fv/mpfao/tdympfao_ts.c:520).  Uses the
  * state of the preceding tdyhip_ifunction, like the reference.  Jvals (nnz, in the
  * pattern order) is overwritten. */

@@ -233,7 +233,7 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
   a.vcols = ctx->d_vcols; a.T = ctx->d_T; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd;
   a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
   a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
-  a.flux = ctx->d_flux; a.Jrow = ctx->d_jrow;
+  a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux; a.Jrow = ctx->d_jrow;
   a.stop = ctx->launch_stop;
   return a;
 }
@@ -244,7 +244,8 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
   a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells;
   a.face_flags = ctx->d_face_flags; a.gtab = ctx->d_gtab;
-  a.jdslot = ctx->d_jdslot; a.flux = ctx->d_flux; a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
+  a.jdslot = ctx->d_jdslot; a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux;
+  a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
   a.src = ctx->has_source ? ctx->d_source : nullptr;
   a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
   a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
@@ -557,7 +558,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   a.ccell = ctx->d_ccell; a.vdcol = ctx->d_vdcol;
   a.T = ctx->d_Tgen; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd; a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr;
   a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr; a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
-  a.flux = ctx->d_flux; a.D = ctx->d_Dgen;
+  a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux; a.D = ctx->d_Dgen;
   a.stop = ctx->launch_stop;
   if (ctx->d_vblk) {
     GenBlkArgs b;
@@ -565,7 +566,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     b.face_cells = ctx->d_face_cells; b.face_flags = ctx->d_face_flags; b.ccell = ctx->d_ccell; b.vdcol = ctx->d_vdcol;
     b.T = ctx->d_Tgen; b.G = ctx->d_G; b.U = U; b.Pbnd = ctx->d_Pbnd; b.rho = ctx->d_rho; b.ukvr = ctx->d_ukvr;
     b.drho = ctx->d_drho; b.dukvr = ctx->d_dukvr; b.rho_bnd = ctx->d_rho_bnd; b.ukvr_bnd = ctx->d_ukvr_bnd;
-    b.flux = ctx->d_flux; b.D = ctx->d_Dgen;
+    b.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux; b.D = ctx->d_Dgen;
     b.stop = ctx->launch_stop;
     const unsigned gv = (unsigned)ctx->nv;
     if (R && J) k_rows_general_blk<true, true><<<gv, 64, 0, st>>>(b);
@@ -644,6 +645,39 @@ static int trans_csr_general(const tdyhip_ctx *ctx, int32_t *rowptr, int32_t *co
 
 }  // namespace tdyhip
 
+namespace tdyhip {
+
+// ---- the allocation-free residual (tdyhip_ifunction_noalloc) ----
+// Its flux-row scratch lives in this file-scope device array, one copy per
+// GPU for the life of the process.  Host code never uses the array's name as
+// a device address: each device's address comes from hipGetSymbolAddress,
+// taken once per device at set-up (tdyhip_create) with that device current,
+// and is passed to the kernels like ctx->d_flux.
+constexpr size_t kNoAllocRows = (size_t)8 << 20;  // flux rows (4 per face): 64 MiB of FP64
+__device__ __align__(256) double scratch[kNoAllocRows];
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of its `scratch`
+
+// The current device's scratch address (looked up and kept the first time).
+static int scratch_address(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) { *out = it->second; return 0; }
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return TDYHIP_ERR_HIP;
+  }
+  g_scratch_ptr[dev] = static_cast<double *>(p);
+  *out = static_cast<double *>(p);
+  return 0;
+}
+
+}  // namespace tdyhip
+
 extern "C" {
 
 const char *tdyhip_last_error(void) { return g_err.c_str(); }
@@ -687,6 +721,10 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   };
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (he != hipSuccess) { set_error(std::string("hipStreamCreate: ") + hipGetErrorString(he)); delete ctx; return TDYHIP_ERR_HIP; }
+  {
+    double *sc = nullptr;  // this device's scratch address, for tdyhip_ifunction_noalloc
+    if ((rc = scratch_address(&sc))) { (void)hipStreamDestroy(ctx->stream); delete ctx; return rc; }
+  }
   ctx->own_stream = true;
   if (!hex) {
     if ((rc = setup_general(ctx, m, mat))) return fail(rc);
@@ -906,6 +944,35 @@ int tdyhip_ifunction(tdyhip_ctx *ctx, double t, const double *U, const double *U
   int rc = launch_state(ctx, 0, U, Udot, st);
   if (rc) return rc;
   return launch_flux(ctx, 0, 0.0, U, Udot, R, nullptr, st);
+}
+
+int tdyhip_ifunction_noalloc(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double *R,
+                             tdyhip_stream s) {
+  (void)t;
+  if (!ctx || !U || !Udot || !R) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  if ((size_t)4 * ctx->nf > kNoAllocRows) {
+    set_error("tdyhip_ifunction_noalloc: the mesh has " + std::to_string((long long)4 * ctx->nf) +
+              " flux rows, more than the " + std::to_string((long long)kNoAllocRows) +
+              " its fixed scratch holds (use tdyhip_ifunction)");
+    return TDYHIP_ERR_ARG;
+  }
+  double *sc = nullptr;
+  int rc = scratch_address(&sc);
+  if (rc) return rc;
+  hipStream_t st; check_stream(ctx, s, &st);
+  const bool timers = ctx->kt.on;  // timer events are created on demand: none here
+  ctx->kt.on = false;
+  // rows the general path's flux kernels do not write (a triangle's fourth
+  // sub-face) must read zero; the array keeps whatever the last call left
+  if (ctx->general) k_fill<<<nblk((int64_t)4 * ctx->nf, 256), 256, 0, st>>>((int64_t)4 * ctx->nf, sc, 0.0);
+  rc = launch_state(ctx, 0, U, Udot, st);
+  if (!rc) {
+    ctx->launch_flux = sc;
+    rc = launch_flux(ctx, 0, 0.0, U, Udot, R, nullptr, st);
+    ctx->launch_flux = nullptr;
+  }
+  ctx->kt.on = timers;
+  return rc;
 }
 
 int tdyhip_ijacobian(tdyhip_ctx *ctx, double t, const double *U, const double *Udot, double shift, double *J,

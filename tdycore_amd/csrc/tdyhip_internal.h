@@ -188,6 +188,7 @@ struct tdyhip_ctx {
   // that makes the launches return at once.  Null for the ABI callbacks.
   const double *launch_dyn = nullptr;
   const int *launch_stop = nullptr;
+  double *launch_flux = nullptr;  // flux-row scratch in place of d_flux (the allocation-free residual)
   // algorithmic-byte bookkeeping
   int64_t trans_nnz = 0;           // transmissibility entries (cells + Dirichlet columns)
   int64_t active_rows = 0;         // flux rows of faces that carry flux

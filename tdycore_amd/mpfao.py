@@ -152,6 +152,12 @@ class MPFAO:
     def IFunction(self, t, U, U_t, R):
         self._run(_lib.load().tdyhip_ifunction, float(t), _ptr(U), _ptr(U_t), _ptr(R), None)
 
+    def IFunctionNoAlloc(self, t, U, U_t, R):
+        """IFunction with its flux-row scratch in the library's fixed per-GPU
+        array instead of the context's buffer: allocates nothing; meshes of at
+        most 2,097,152 faces; calls that overlap on one GPU race."""
+        self._run(_lib.load().tdyhip_ifunction_noalloc, float(t), _ptr(U), _ptr(U_t), _ptr(R), None)
+
     def IJacobian(self, t, U, U_t, shift, J):
         self._run(_lib.load().tdyhip_ijacobian, float(t), _ptr(U), _ptr(U_t), float(shift), _ptr(J), None)
 
