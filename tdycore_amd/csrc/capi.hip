@@ -158,21 +158,25 @@ static void kt_collect(tdyhip_ctx *ctx) {
 static double bytes_state(const tdyhip_ctx *ctx) {
   return (double)ctx->nc * (8 * 5 + 8 * 7) + (double)ctx->nbnd * (4 + 8 + 8 + 8 + 8);
 }
-static double bytes_rows(const tdyhip_ctx *ctx, bool R, bool J) {
-  double b = (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (8 + 4) + (double)ctx->nf * (8 + 1) +
-             (double)ctx->nv * kMaxCols * 4 + (double)ctx->nc * 8 * 3 + (double)ctx->nbnd * 8 * 3;
-  if (R) b += (double)ctx->active_rows * 8;
-  if (J) b += (double)ctx->trans_nnz * 8 + (double)ctx->nc * 8 * 2;  // row derivatives out; drho, dukvr
-  return b;
+// flux rows: T, G, row vertices, face cells and flags, vertex columns, U,
+// rho, kr/mu at cells and boundary faces, P_bnd, and the rows' fluxes out
+static double bytes_rows(const tdyhip_ctx *ctx) {
+  return (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (8 + 4 + 8) + (double)ctx->nf * (8 + 1) +
+         (double)ctx->nv * kMaxCols * 4 + (double)ctx->nc * 8 * 3 + (double)ctx->nbnd * 8 * 3;
 }
-static double bytes_gather(const tdyhip_ctx *ctx, bool R, bool J) {
-  double b = (double)ctx->nc * 24 + (double)ctx->nf * 8;  // cell faces, face cells
-  if (R) b += (double)ctx->nc * (8 + 8) + (double)ctx->nf * 32;
-  if (J)
-    // flux-row derivatives (one 64-byte row each), the gather table, the
-    // accumulation terms and the Jacobian row written once
-    b += (double)ctx->active_rows * kMaxCols * 8 + (double)ctx->nc * 24 * 8 + (double)ctx->nc * (8 * 4 + 1) +
-         (double)ctx->jnnz * 8;
+// residual gather: cell faces, face cells, fluxes, accumulation, R out
+static double bytes_gather(const tdyhip_ctx *ctx) {
+  return (double)ctx->nc * (24 + 8 + 8) + (double)ctx->nf * (8 + 32);
+}
+// single-pass Jacobian (+ residual): the flux rows' inputs once (T, G, row
+// vertices, up/down region indices, face cells and flags, vertex columns and
+// slot map, U, rho, kr/mu and their derivatives, boundary values), cell faces,
+// the accumulation inputs of the form, the diagonal slot and J written once
+static double bytes_cell_jac(const tdyhip_ctx *ctx, int form, bool R) {
+  double b = (double)ctx->trans_nnz * 8 + (double)ctx->active_rows * (8 + 4 + 2) + (double)ctx->nf * (8 + 1) +
+             (double)ctx->nv * (kMaxCols * 4 + 64) + (double)ctx->nc * (8 * 5 + 24 + 1) +
+             (double)ctx->nbnd * 8 * 3 + (double)ctx->nc * 8 * (form == 0 ? 4 : 2) + (double)ctx->jnnz * 8;
+  if (R) b += (double)ctx->nc * (8 + 8);
   return b;
 }
 
@@ -233,7 +237,7 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
   a.vcols = ctx->d_vcols; a.T = ctx->d_T; a.G = ctx->d_G; a.U = U; a.Pbnd = ctx->d_Pbnd;
   a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
   a.rho_bnd = ctx->d_rho_bnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
-  a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux; a.Jrow = ctx->d_jrow;
+  a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux;
   a.stop = ctx->launch_stop;
   return a;
 }
@@ -243,11 +247,11 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   GatherArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
   a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells;
-  a.face_flags = ctx->d_face_flags; a.gtab = ctx->d_gtab;
+  a.face_flags = ctx->d_face_flags;
   a.jdslot = ctx->d_jdslot; a.flux = ctx->launch_flux ? ctx->launch_flux : ctx->d_flux;
   a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
   a.src = ctx->has_source ? ctx->d_source : nullptr;
-  a.Udot = Udot; a.vol = ctx->d_cell_vol; a.Jrow = ctx->d_jrow;
+  a.Udot = Udot; a.vol = ctx->d_cell_vol;
   a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
   a.perm = ell ? perm : nullptr;
   a.R = R; a.J = J;
@@ -269,32 +273,39 @@ static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double
   return 0;
 }
 
-// rows + gather.  want_R / want_J select the outputs.
+// Residual (R) and / or Jacobian (J) at U.  The residual alone: flux rows,
+// then the per-cell gather.  With J: one pass per cell (k_cell_jac).
 // ell = true: J is the solver's block-ordered slot-major ELL array (row c at
 // perm[c], slot stride ell_stride), else CSR values in the pattern order.
-// Every entry of J is written (no prior clear is needed).
+// order (ELL only): the block layout's row list (position -> cell, -1
+// padding, norder entries); the Jacobian's threads follow it so that a
+// workgroup covers one block.  Every entry of J is written (no prior clear).
 static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
                                double *R, double *J, hipStream_t st, const int32_t *perm, int64_t ell_stride);
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
                        double *J, hipStream_t st, bool ell = false, const int32_t *perm = nullptr,
-                       int64_t ell_stride = 0) {
+                       int64_t ell_stride = 0, const int32_t *order = nullptr, int64_t norder = 0) {
   if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st, ell ? perm : nullptr, ell_stride);
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
-  if (R && J) KT(ctx, st, "k_rows_RJ", bytes_rows(ctx, true, true), k_rows<true, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
-  else if (R) KT(ctx, st, "k_rows_R", bytes_rows(ctx, true, false), k_rows<true, false><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
-  else KT(ctx, st, "k_rows_J", bytes_rows(ctx, false, true), k_rows<false, true><<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
   GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell, perm, ell_stride);
-  const unsigned g = nblk(ctx->nc, kGatherBlock);
-  if (R && J)
-    KT(ctx, st, "k_cell_gather_RJ", bytes_gather(ctx, true, true),
-       k_cell_gather<true, true><<<g, kGatherBlock, 0, st>>>(ga));
-  else if (R)
-    KT(ctx, st, "k_cell_gather_R", bytes_gather(ctx, true, false),
-       k_cell_gather<true, false><<<g, kGatherBlock, 0, st>>>(ga));
-  else
-    KT(ctx, st, "k_cell_gather_J", bytes_gather(ctx, false, true),
-       k_cell_gather<false, true><<<g, kGatherBlock, 0, st>>>(ga));
+  if (!J) {
+    KT(ctx, st, "k_rows", bytes_rows(ctx), k_rows<<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
+    KT(ctx, st, "k_cell_gather", bytes_gather(ctx),
+       k_cell_gather<<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga));
+  } else {
+    CellJacArgs cj;
+    cj.order = ell ? order : nullptr;
+    cj.npos = cj.order ? norder : ctx->nc;
+    cj.jslot = ctx->d_jslot; cj.crow = ctx->d_crow; cj.cell_verts = ctx->d_cell_verts;
+    const int64_t nb = (cj.npos + kCellJacCells - 1) / kCellJacCells;
+    cj.xcd_groups = ctx->cj_xcd ? (int)((nb + 7) / 8) : 0;
+    const unsigned g = cj.xcd_groups ? (unsigned)(8 * cj.xcd_groups) : (unsigned)nb;
+    if (R)
+      KT(ctx, st, "k_cell_jac_R", bytes_cell_jac(ctx, form, true), k_cell_jac<true><<<g, kCellJacBlock, 0, st>>>(a, ga, cj));
+    else
+      KT(ctx, st, "k_cell_jac", bytes_cell_jac(ctx, form, false), k_cell_jac<false><<<g, kCellJacBlock, 0, st>>>(a, ga, cj));
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -580,7 +591,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   }
   if (R) {
     GatherArgs ga = gather_args(ctx, form, shift, Udot, R, nullptr, false, nullptr, 0);
-    k_cell_gather<true, false><<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga);
+    k_cell_gather<<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga);
   }
   if (J) {
     GenGatherArgs gg;
@@ -709,6 +720,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   ctx->vg_m = mat->vg_m; ctx->vg_alpha = mat->vg_alpha; ctx->x0 = mat->poly_x0; ctx->x1 = mat->poly_x1;
   ctx->x2 = mat->poly_x2; ctx->dx = mat->poly_dx; ctx->gravity = mat->gravity; ctx->pref = mat->pref;
   ctx->density_type = mat->density_type; ctx->gmatrix = mat->gmatrix_method;
+  if (const char *e = getenv("TDYHIP_CELLJAC_XCD")) ctx->cj_xcd = atoi(e) != 0;
   if (mualem_cubic(ctx->vg_m, ctx->x0, ctx->x1, ctx->x2, ctx->dx, ctx->mualem_c)) {
     set_error("Mualem smoothing polynomial is singular");
     delete ctx;
@@ -757,7 +769,37 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   if ((rc = dupload(ctx, &ctx->d_bnd_cell, topo.bnd_cell.data(), nbnd))) return fail(rc);
   if ((rc = upload_geometry_for_velocity(ctx, m))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_vcols, topo.vcols.data(), topo.vcols.size()))) return fail(rc);
-  if ((rc = dalloc(ctx, &ctx->d_gtab, (size_t)24 * nc))) return fail(rc);
+  if ((rc = dupload(ctx, &ctx->d_jslot, topo.jslot.data(), topo.jslot.size()))) return fail(rc);
+  {
+    // each corner's three flux rows (one per face of the cell at the corner),
+    // the cell's index in the corner vertex's region and its side of each row
+    std::vector<uint32_t> crow((size_t)nc * 8, 0);
+    for (int c = 0; c < nc; c++) {
+      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int lf = 0; lf < 6; lf++) {
+        const int f = m->cell_faces[(size_t)c * 6 + lf];
+        for (int q = 0; q < 4; q++) {
+          const int v = m->face_verts[(size_t)f * 4 + q];
+          int j = 0;
+          while (j < 8 && m->cell_verts[(size_t)c * 8 + j] != v) j++;
+          if (j == 8 || cnt[j] == 3) { set_error("hexahedron with a corner not on exactly three of its faces"); return fail(TDYHIP_ERR_ARG); }
+          uint32_t &w = crow[(size_t)c * 8 + j];
+          w |= (uint32_t)(lf * 4 + q) << (5 * cnt[j]);
+          if (m->face_cells[(size_t)2 * f] == c) w |= 1u << (18 + cnt[j]);
+          cnt[j]++;
+        }
+      }
+      for (int j = 0; j < 8; j++) {
+        const int v = m->cell_verts[(size_t)c * 8 + j];
+        int li = 0;
+        while (li < 8 && topo.vcols[(size_t)v * kMaxCols + li] != c) li++;
+        if (li == 8) { set_error("cell missing from its vertex's region"); return fail(TDYHIP_ERR_ARG); }
+        crow[(size_t)c * 8 + j] |= (uint32_t)li << 15;
+      }
+    }
+    if ((rc = dupload(ctx, &ctx->d_crow, crow.data(), crow.size()))) return fail(rc);
+    if ((rc = dupload(ctx, &ctx->d_cell_verts, m->cell_verts, (size_t)nc * 8))) return fail(rc);
+  }
   if ((rc = dupload(ctx, &ctx->d_jrp, topo.jrp.data(), topo.jrp.size()))) return fail(rc);
   if ((rc = dupload(ctx, &ctx->d_jcols, topo.jcols.data(), topo.jcols.size()))) return fail(rc);
   {
@@ -786,15 +828,13 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   for (double **p : {&ctx->d_Pbnd, &ctx->d_ukvr_bnd, &ctx->d_rho_bnd})
     if ((rc = dalloc(ctx, p, nbnd))) return fail(rc);
   if ((rc = dalloc(ctx, &ctx->d_flux, (size_t)nrows))) return fail(rc);
-  if ((rc = dalloc(ctx, &ctx->d_jrow, (size_t)kMaxCols * nrows))) return fail(rc);
 
   // setup temporaries
   int32_t *d_scf = nullptr; IRDesc *d_ir = nullptr; double *d_Gsub = nullptr, *d_K = nullptr, *d_cc = nullptr,
           *d_fc = nullptr, *d_fa = nullptr, *d_fn = nullptr; int *d_status = nullptr;
-  uint8_t *d_rul = nullptr, *d_jsl = nullptr;
   auto tmp_fail = [&](int code) {
     for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc,
-                    (void *)d_fa, (void *)d_fn, (void *)d_status, (void *)d_rul, (void *)d_jsl})
+                    (void *)d_fa, (void *)d_fn, (void *)d_status})
       if (p) (void)hipFree(p);
     return fail(code);
   };
@@ -815,8 +855,6 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   TMPALLOC(d_fa, (size_t)nf, m->face_area);
   TMPALLOC(d_fn, (size_t)nf * 3, m->face_normal);
   TMPALLOC(d_status, 1, (int *)nullptr);
-  TMPALLOC(d_rul, topo.row_ul.size(), topo.row_ul.data());
-  TMPALLOC(d_jsl, topo.jslot.size(), topo.jslot.data());
 #undef TMPALLOC
   hipStream_t st = ctx->stream;
   k_fill<<<nblk((int64_t)kMaxCols * nrows, 256), 256, 0, st>>>((int64_t)kMaxCols * nrows, ctx->d_T, 0.0);
@@ -834,8 +872,6 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   k_local_systems<<<nblk(nv, kTeamsPerBlock), kTeam * kTeamsPerBlock, 0, st>>>(nv, d_ir, d_Gsub, ctx->d_T, nrows,
                                                                                d_status);
   rec(3);
-  k_gather_table<<<nblk(nc, 256), 256, 0, st>>>(nc, ctx->d_cell_faces, ctx->d_face_cells, ctx->d_face_verts,
-                                                ctx->d_face_flags, d_rul, ctx->d_vcols, d_jsl, ctx->d_gtab);
   rec(4);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
                                            d_fc, d_fa, d_fn, d_K, nullptr, ctx->d_G);
@@ -871,7 +907,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
   int status = 0;
   if (copy_sync(ctx, &status, d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return tmp_fail(TDYHIP_ERR_HIP);
   for (void *p : {(void *)d_scf, (void *)d_ir, (void *)d_Gsub, (void *)d_K, (void *)d_cc, (void *)d_fc, (void *)d_fa,
-                  (void *)d_fn, (void *)d_status, (void *)d_rul, (void *)d_jsl})
+                  (void *)d_fn, (void *)d_status})
     (void)hipFree(p);
   if (status) {
     set_error("singular interaction-region system (zero LU pivot)");

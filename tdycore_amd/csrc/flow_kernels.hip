@@ -7,10 +7,12 @@
 //  k_bnd_state    per boundary face: P_bnd and Kr/mu on the boundary
 //                 (tdympfao_utils.c:24-99, 690-759)
 //  k_rows         per flux row (face*4 + subface): T.P from the vertex's columns,
-//                 upwinded rho^2 kr/mu flux (TDyMPFAOIFunction_Vertices, tdympfao_ts.c:12-150),
-//                 optionally the row's Jacobian entries (TDyMPFAOIJacobian_Vertices, 235-456)
+//                 upwinded rho^2 kr/mu flux (TDyMPFAOIFunction_Vertices, tdympfao_ts.c:12-150)
 //  k_cell_gather  per cell: R = sum over its 6 faces x 4 subfaces (+/- by orientation)
-//                 + accumulation; Jacobian diagonal accumulation term
+//                 + accumulation
+//  k_cell_jac     per cell, one pass: the Jacobian row from its 24 flux rows'
+//                 derivatives (TDyMPFAOIJacobian_Vertices, 235-456) + the
+//                 accumulation diagonal, optionally the residual as well
 //
 // Flux rows are stored face-major (row = face*4 + subface), the reference's
 // Trans_mat row numbering; T is an 8-wide ELL block stored column-major so that
@@ -135,115 +137,104 @@ struct RowArgs {
   const double *rho, *ukvr, *drho, *dukvr;
   const double *rho_bnd, *ukvr_bnd;
   double *flux;
-  double *Jrow;  // nrows * kMaxCols: d(flux row r)/d(p of column k) at Jrow[r*kMaxCols + k]
   const int *stop;
 };
 
-constexpr int kRowBlock = 256;  // k_rows workgroup size (the Jrow staging below assumes it)
+constexpr int kRowBlock = 256;  // k_rows workgroup size
 
-// One flux row: its flux (WITH_R) and its 8 column derivatives into jr
-// (WITH_J; zeros for padding, boundary-pressure and no-flow rows).
-template <bool WITH_R, bool WITH_J>
-__device__ inline void row_eval(const RowArgs &a, int64_t r, double (&jr)[kMaxCols]) {
-#pragma unroll
-  for (int k = 0; k < kMaxCols; k++) jr[k] = 0.0;
-  if (r >= a.nrows) return;
-  const int f = (int)(r >> 2);
-  const uint8_t fl = a.face_flags[f];
-  if (fl & kFaceNoFlow) {
-    if (WITH_R) a.flux[r] = 0.0;
-    return;
-  }
-  const int up = a.face_cells[2 * f], dn = a.face_cells[2 * f + 1];
-  const int v = a.face_verts[r];
-  const int32_t *cols = a.vcols + (int64_t)v * kMaxCols;
-  double Tk[kMaxCols];
-  int ck[kMaxCols];
+// The arithmetic of one flux row (TDyMPFAOIFunction_Vertices,
+// tdympfao_ts.c:12-150; TDyMPFAOIJacobian_Vertices, 235-456), shared by the
+// residual's row kernel and the Jacobian's cell kernel so that both take the
+// same T.P and the same upwind side.  Inputs: the row's 8 transmissibilities
+// T and the vertex's columns ck with their pressures P, rho at the two sides,
+// the gravity term G.
+__device__ inline double row_tp(const double (&T)[kMaxCols], const int (&ck)[kMaxCols],
+                                const double (&P)[kMaxCols]) {
   double TP = 0.0;
 #pragma unroll
-  for (int k = 0; k < kMaxCols; k++) {
-    ck[k] = cols[k];
-    Tk[k] = a.T[(int64_t)k * a.nrows + r];
-  }
+  for (int k = 0; k < kMaxCols; k++) TP = ck[k] >= 0 ? fma(T[k], P[k], TP) : TP;
+  return TP;
+}
+__device__ inline double row_den(double rho_up, double rho_dn) {
+  double den = 0.0;
+  den += rho_up;
+  den += rho_dn;
+  return den * 0.5;
+}
+__device__ inline bool row_upwind_up(double TP, double den, double G) { return fma(den, G, TP) < 0.0; }
+__device__ inline double row_flux(double TP, double den, double G, double ukvr, bool zero) {
+  double fluxm = den * ukvr * (-TP);
+  fluxm += -(den * den) * ukvr * G;
+  return zero ? 0.0 : fluxm;
+}
+// d(flux)/d(p of column k), negated (the up cell's residual gets -Jac, the
+// down cell's +Jac, tdympfao_ts.c:439-444); 0 for padding and boundary-pressure
+// columns, which never enter J, and for a row that carries no flux
+__device__ inline void row_derivs(int nc, const double (&T)[kMaxCols], const int (&ck)[kMaxCols], int up, int dn,
+                                  double TP, double den, double G, double ukvr, bool upwind_up, bool zero,
+                                  double dr_up, double dr_dn, double du, double (&jr)[kMaxCols]) {
+  const double dden_up = up >= 0 ? 0.5 * dr_up : 0.0;
+  const double dden_dn = dn >= 0 ? 0.5 * dr_dn : 0.0;
+  const double dukvr_up = (upwind_up && up >= 0) ? du : 0.0;
+  const double dukvr_dn = (!upwind_up && dn >= 0) ? du : 0.0;
+  // the up (down) cell's column adds the derivatives of rho_bar and kr/mu
+  const double Aup = dden_up * ukvr * TP + den * dukvr_up * TP;
+  const double Dup = 2 * den * dden_up * ukvr * G, Eup = (den * den) * dukvr_up * G;
+  const double Adn = dden_dn * ukvr * TP + den * dukvr_dn * TP;
+  const double Ddn = 2 * den * dden_dn * ukvr * G, Edn = (den * den) * dukvr_dn * G;
 #pragma unroll
   for (int k = 0; k < kMaxCols; k++) {
     const int col = ck[k];
-    if (col >= 0) {
-      const double P = (col < a.nc) ? a.U[col] : a.Pbnd[col - a.nc];
-      TP += Tk[k] * P;
-    }
-  }
-  double den = 0.0;
-  den += up >= 0 ? a.rho[up] : a.rho_bnd[-up - 1];
-  den += dn >= 0 ? a.rho[dn] : a.rho_bnd[-dn - 1];
-  den *= 0.5;
-  const double G = a.G[r];
-  const bool upwind_up = (TP + den * G < 0.0);
-  const int uw = upwind_up ? up : dn;
-  double ukvr;
-  bool zero = false;
-  if (uw >= 0) {
-    ukvr = a.ukvr[uw];
-  } else {
-    ukvr = a.ukvr_bnd[-uw - 1];
-    if ((fl & kFaceSeepage) && a.Pbnd[-uw - 1] <= a.pref) zero = true;
-  }
-  if (WITH_R) {
-    double fluxm = 0.0;
-    if (!zero) {
-      fluxm = den * ukvr * (-TP);
-      fluxm += -(den * den) * ukvr * G;
-    }
-    a.flux[r] = fluxm;
-  }
-  if (WITH_J) {
-    const double dden_up = up >= 0 ? 0.5 * a.drho[up] : 0.0;
-    const double dden_dn = dn >= 0 ? 0.5 * a.drho[dn] : 0.0;
-    double dukvr_up = 0.0, dukvr_dn = 0.0;
-    if (upwind_up) { if (up >= 0) dukvr_up = a.dukvr[up]; }
-    else { if (dn >= 0) dukvr_dn = a.dukvr[dn]; }
-#pragma unroll
-    for (int k = 0; k < kMaxCols; k++) {
-      const int col = ck[k];
-      double Jac = 0.0;
-      if (col < 0 || col >= a.nc) continue;  // padding or boundary-pressure column: never enters J
-      if (zero) Jac = 0.0;
-      else if (up >= 0 && col == up)
-        Jac = dden_up * ukvr * TP + den * dukvr_up * TP + den * ukvr * Tk[k] + 2 * den * dden_up * ukvr * G +
-              (den * den) * dukvr_up * G;
-      else if (dn >= 0 && col == dn)
-        Jac = dden_dn * ukvr * TP + den * dukvr_dn * TP + den * ukvr * Tk[k] + 2 * den * dden_dn * ukvr * G +
-              (den * den) * dukvr_dn * G;
-      else
-        Jac = den * ukvr * Tk[k];
-      // the up cell's residual gets -Jac, the down cell's +Jac (tdympfao_ts.c:439-444)
-      jr[k] = -Jac;
-    }
+    const double C = den * ukvr * T[k];
+    const double Ju = ((Aup + C) + Dup) + Eup, Jd = ((Adn + C) + Ddn) + Edn;
+    const double Jac = col == up ? Ju : (col == dn ? Jd : C);
+    jr[k] = (zero || col < 0 || col >= nc) ? 0.0 : -Jac;
   }
 }
 
-// Thread per flux row.  Jrow is row-major (64 B per row, so the per-cell
-// gather reads a face's four rows as 256 contiguous bytes); the workgroup's
-// rows are staged in LDS and written out as one contiguous, coalesced run.
-template <bool WITH_R, bool WITH_J>
+// the columns of vertex v and their pressures (padding reads U[0], unused)
+__device__ inline void vertex_columns(const RowArgs &a, int v, int (&ck)[kMaxCols], double (&P)[kMaxCols]) {
+  const int4 c0 = *reinterpret_cast<const int4 *>(a.vcols + (int64_t)v * kMaxCols);
+  const int4 c1 = *reinterpret_cast<const int4 *>(a.vcols + (int64_t)v * kMaxCols + 4);
+  ck[0] = c0.x; ck[1] = c0.y; ck[2] = c0.z; ck[3] = c0.w;
+  ck[4] = c1.x; ck[5] = c1.y; ck[6] = c1.z; ck[7] = c1.w;
+#pragma unroll
+  for (int k = 0; k < kMaxCols; k++) {
+    const int col = ck[k];
+    P[k] = *(col < a.nc ? a.U + (col < 0 ? 0 : col) : a.Pbnd + (col - a.nc));
+  }
+}
+
+// Thread per flux row: the residual's flux rows (the Jacobian is assembled by
+// k_cell_jac, one pass per cell).
 __global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
   if (a.stop && *a.stop) return;  // uniform over the workgroup
   const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
-  double jr[kMaxCols];
-  row_eval<WITH_R, WITH_J>(a, r, jr);
-  if (WITH_J) {
-    constexpr int P = kMaxCols + 1;  // odd pitch: conflict-free LDS rows
-    __shared__ double js[kRowBlock * P];
+  if (r >= a.nrows) return;
+  const int f = (int)(r >> 2);
+  const uint8_t fl = a.face_flags[f];
+  double flux = 0.0;
+  if (!(fl & kFaceNoFlow)) {
+    const int2 ud = *reinterpret_cast<const int2 *>(a.face_cells + 2 * f);
+    const int up = ud.x, dn = ud.y;
+    int ck[kMaxCols];
+    double P[kMaxCols], T[kMaxCols];
+    vertex_columns(a, a.face_verts[r], ck, P);
 #pragma unroll
-    for (int k = 0; k < kMaxCols; k++) js[threadIdx.x * P + k] = jr[k];
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kRowBlock * kMaxCols, lim = a.nrows * kMaxCols;
-#pragma unroll
-    for (int i = 0; i < kMaxCols; i++) {
-      const int e = threadIdx.x + kRowBlock * i;
-      if (base + e < lim) a.Jrow[base + e] = js[(e / kMaxCols) * P + e % kMaxCols];
-    }
+    for (int k = 0; k < kMaxCols; k++) T[k] = a.T[(int64_t)k * a.nrows + r];
+    const double G = a.G[r];
+    const double den = row_den(*(up >= 0 ? a.rho + up : a.rho_bnd + (-up - 1)),
+                               *(dn >= 0 ? a.rho + dn : a.rho_bnd + (-dn - 1)));
+    const double TP = row_tp(T, ck, P);
+    const bool upwind_up = row_upwind_up(TP, den, G);
+    const int uw = upwind_up ? up : dn;
+    const double ukvr = *(uw >= 0 ? a.ukvr + uw : a.ukvr_bnd + (-uw - 1));
+    // a seepage face's boundary side carries no flux while its pressure is at
+    // or below the reference pressure
+    const bool zero = uw < 0 && (fl & kFaceSeepage) && a.Pbnd[-uw - 1] <= a.pref;
+    flux = row_flux(TP, den, G, ukvr, zero);
   }
+  a.flux[r] = flux;
 }
 
 // ---------------------------------------------------------------- general (prism) meshes
@@ -535,40 +526,6 @@ __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a
   for (int k = 0; k < nslot; k++) row[k * st] = k < len ? acc[k][t] : 0.0;
 }
 
-// Per-cell gather table, built once at setup.  Row i = 4*lf + q of cell c
-// (its local face lf, sub-face q) has 8 bytes at gtab[i*nc + c]: byte k is
-// the slot of column k in c's Jacobian row, with bit 7 set when c is the
-// face's down cell (the row enters c's residual with a minus sign), or 0xff
-// when column k does not enter J (padding, a boundary pressure) or the face
-// carries no flow.  Lanes of consecutive cells read consecutive words.
-__global__ void k_gather_table(int nc, const int32_t *__restrict__ cell_faces, const int32_t *__restrict__ face_cells,
-                               const int32_t *__restrict__ face_verts, const uint8_t *__restrict__ face_flags,
-                               const uint8_t *__restrict__ row_ul, const int32_t *__restrict__ vcols,
-                               const uint8_t *__restrict__ jslot, uint64_t *__restrict__ gtab) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  for (int lf = 0; lf < 6; lf++) {
-    const int f = cell_faces[c * 6 + lf];
-    const bool noflow = (face_flags[f] & kFaceNoFlow) != 0;
-    const bool is_up = face_cells[2 * f] == c;
-    for (int q = 0; q < 4; q++) {
-      const int64_t r = (int64_t)f * 4 + q;
-      uint64_t w = ~0ull;
-      if (!noflow) {
-        const int v = face_verts[r];
-        const int li = row_ul[2 * r + (is_up ? 0 : 1)];
-        for (int k = 0; k < kMaxCols; k++) {
-          const int col = vcols[(int64_t)v * kMaxCols + k];
-          if (col < 0 || col >= nc) continue;
-          const uint64_t sb = (uint64_t)(jslot[(int64_t)v * 64 + li * 8 + k] | (is_up ? 0u : 0x80u));
-          w = (w & ~(0xffull << (8 * k))) | (sb << (8 * k));
-        }
-      }
-      gtab[(int64_t)(lf * 4 + q) * nc + c] = w;
-    }
-  }
-}
-
 struct GatherArgs {
   int nc;
   int64_t nrows;
@@ -576,8 +533,7 @@ struct GatherArgs {
   double shift, dtime;
   const int32_t *cell_faces, *face_cells;
   const uint8_t *face_flags, *jdslot;
-  const uint64_t *gtab;  // 24*nc: see k_gather_table
-  const double *flux, *acc, *dM, *d2M, *Udot, *vol, *Jrow;
+  const double *flux, *acc, *dM, *d2M, *Udot, *vol;
   const double *src;   // source_sink per cell, or nullptr (TS form: scaled by the volume)
   const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
   const int32_t *perm; // ELL: position of row c (nullptr: c)
@@ -591,62 +547,191 @@ struct GatherArgs {
 constexpr int kGatherBlock = 128;
 constexpr int kJacSlots = 27;  // vertex-adjacency row length of a hexahedral mesh
 
-// Per-cell gather: R[c] sums the fluxes of the cell's faces (+ accumulation);
-// J's row c sums, in a fixed order (face, sub-face, column), the flux-row
-// derivatives of the cell's faces into the row's slots in LDS, adds the
-// accumulation diagonal and writes the whole row once.  Every cell owns its
-// residual entry and its Jacobian row, so the result does not depend on
-// scheduling.
-template <bool WITH_R, bool WITH_J>
+// R[c] from the flux rows: the cell's faces' four rows summed in sub-face
+// order, +/- by orientation, then the accumulation and the source
+// (tdympfao_ts.c:191-212, tdympfao_snes.c:112-123).
+__device__ inline double cell_residual(const GatherArgs &a, int c, double s) {
+  double r = s + a.acc[c];
+  if (a.src) r -= (a.form == 0) ? a.src[c] * a.vol[c] : a.src[c];
+  return r;
+}
+
+// Per-cell residual gather (thread per cell): every cell owns its entry, so
+// the result does not depend on scheduling.
 __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
-  __shared__ double acc[WITH_J ? kJacSlots : 1][kGatherBlock];
   if (a.stop && *a.stop) return;
-  const int t = threadIdx.x;
-  const int c = blockIdx.x * kGatherBlock + t;
+  const int c = blockIdx.x * kGatherBlock + threadIdx.x;
   if (c >= a.nc) return;
-  if (WITH_R) {
-    double s = 0.0;
+  double s = 0.0;
 #pragma unroll
-    for (int lf = 0; lf < 6; lf++) {
-      const int f = a.cell_faces[c * 6 + lf];
-      if (f < 0) continue;  // a prism has 5 faces (a triangle's fourth flux row is zero)
-      const double4 q = *reinterpret_cast<const double4 *>(a.flux + (int64_t)f * 4);
-      const double fs = ((q.x + q.y) + q.z) + q.w;
-      s += (a.face_cells[2 * f] == c) ? fs : -fs;
-    }
-    // fluxes, then accumulation, then the source (tdympfao_ts.c:191-212, tdympfao_snes.c:112-123)
-    double r = s + a.acc[c];
-    if (a.src) r -= (a.form == 0) ? a.src[c] * a.vol[c] : a.src[c];
-    a.R[c] = r;
+  for (int lf = 0; lf < 6; lf++) {
+    const int f = a.cell_faces[c * 6 + lf];
+    if (f < 0) continue;  // a prism has 5 faces (a triangle's fourth flux row is zero)
+    const double4 q = *reinterpret_cast<const double4 *>(a.flux + (int64_t)f * 4);
+    const double fs = ((q.x + q.y) + q.z) + q.w;
+    s += (a.face_cells[2 * f] == c) ? fs : -fs;
   }
-  if (WITH_J) {
+  a.R[c] = cell_residual(a, c, s);
+}
+
+// Single-pass Jacobian (and, WITH_R, residual) of a hexahedral mesh
+// (TDyMPFAOIJacobian_Vertices, tdympfao_ts.c:235-456, evaluated from the
+// cell's side): eight lanes per cell, one per corner.  The three faces of the
+// cell that meet at a corner have one flux row each at that corner's vertex;
+// the three rows share the vertex's 8 columns and pressures (gathered once)
+// and the 8 slots of the cell's Jacobian row those columns occupy, so the lane
+// sums the rows' derivatives in registers (face order) and stages the 8 sums
+// and their slots in LDS.  After a barrier the corner-0 lane adds the eight
+// corners into the row in corner order (a fixed summation order, so the result
+// does not depend on scheduling) plus the accumulation diagonal, and the eight
+// lanes write the row.  Each flux row is evaluated by both cells of its face;
+// the second read of T hits the cache when the two cells run close together,
+// which `order` (position -> cell: the block-Jacobi layout's row list, -1
+// padding) and the XCD-grouped workgroup numbering arrange.
+//   crow[c*8 + j]: the corner's three rows as 5-bit (face slot * 4 + sub-face)
+//                  codes in face order (bits 0-4, 5-9, 10-14), the cell's index
+//                  in the region of the corner's vertex (bits 15-17), and
+//                  whether the cell is each row's up cell (bits 18, 19, 20)
+//   cell_verts[c*8 + j]: the corner's vertex
+//   jslot[v*64 + li*8 + k]: slot of column k of vertex v's region in the row
+//                  of the region's cell li (255: not a cell column)
+// A no-flow face's row is evaluated like the others, at the cell itself on
+// both sides (every read in range), and dropped.
+struct CellJacArgs {
+  const int32_t *order;  // position -> cell, or nullptr (position i is cell i)
+  int64_t npos;
+  int xcd_groups;        // 0: workgroups in launch order; else the launch is 8 * xcd_groups workgroups
+  const uint32_t *crow;
+  const int32_t *cell_verts;
+  const uint8_t *jslot;
+};
+
+constexpr int kCellJacBlock = 256;                // lanes per workgroup
+constexpr int kCellJacCells = kCellJacBlock / 8;  // cells per workgroup
+
+template <bool WITH_R>
+__global__ void __launch_bounds__(kCellJacBlock) k_cell_jac(RowArgs ra, GatherArgs a, CellJacArgs cj) {
+  __shared__ double sd[kMaxCols][kCellJacBlock];     // corner sums: column k of lane t at sd[k][t]
+  __shared__ uint64_t sw[kCellJacBlock];             // their slots (one byte per column)
+  __shared__ double sf[WITH_R ? 24 : 1][kCellJacCells];  // flux of row lf*4+q of cell cl
+  __shared__ double acc[kJacSlots + 1][kCellJacCells];   // the row (slot kJacSlots: non-cell columns)
+  if (a.stop && *a.stop) return;  // uniform over the workgroup
+  const int t = threadIdx.x, cl = t >> 3, corner = t & 7;
+  // with xcd_groups, hardware round-robin places workgroup w on XCD w % 8:
+  // give XCD x the contiguous run of blocks [x * xcd_groups, (x + 1) * xcd_groups)
+  const int64_t blk = cj.xcd_groups ? (int64_t)(blockIdx.x % 8) * cj.xcd_groups + blockIdx.x / 8 : blockIdx.x;
+  const int64_t i = blk * kCellJacCells + cl;
+  const int c = i < cj.npos ? (cj.order ? cj.order[i] : (int)i) : -1;  // -1: no cell
+  if (c >= 0) {
+    for (int sl = corner; sl < kJacSlots; sl += 8) acc[sl][cl] = 0.0;
+    const unsigned code = cj.crow[(int64_t)c * 8 + corner];
+    const int v = cj.cell_verts[(int64_t)c * 8 + corner];
+    const int li = (code >> 15) & 7;
+    int fr[3], lfq[3];
 #pragma unroll
-    for (int sl = 0; sl < kJacSlots; sl++) acc[sl][t] = 0.0;
-    for (int lf = 0; lf < 6; lf++) {
-      const int f = a.cell_faces[c * 6 + lf];
-      for (int q = 0; q < 4; q++) {
-        const uint64_t w = a.gtab[(int64_t)(lf * 4 + q) * a.nc + c];
-        if (w == ~0ull) continue;  // no-flow face
-        const double4 *src = reinterpret_cast<const double4 *>(a.Jrow + ((int64_t)f * 4 + q) * kMaxCols);
-        const double4 j0 = src[0], j1 = src[1];
-        const double d8[kMaxCols] = {j0.x, j0.y, j0.z, j0.w, j1.x, j1.y, j1.z, j1.w};
+    for (int m = 0; m < 3; m++) {
+      lfq[m] = (code >> (5 * m)) & 31;
+      fr[m] = a.cell_faces[c * 6 + (lfq[m] >> 2)];
+    }
+    int ck[kMaxCols];
+    double P[kMaxCols];
+    vertex_columns(ra, v, ck, P);
+    const uint64_t w = *reinterpret_cast<const uint64_t *>(cj.jslot + (int64_t)v * 64 + li * 8);
+    // the three rows' loads in two batches: what the faces fix, then what the
+    // upwind side fixes (a no-flow face's row runs at the cell on both sides,
+    // every read in range, and is dropped)
+    int up[3], dn[3];
+    bool noflow[3], seep[3];
+    double T[3][kMaxCols], G[3], rup[3], rdn[3], drup[3], drdn[3];
 #pragma unroll
-        for (int k = 0; k < kMaxCols; k++) {
-          const unsigned sb = (unsigned)(w >> (8 * k)) & 0xffu;
-          if (sb == 0xffu) continue;
-          const double d = d8[k];
-          acc[sb & 0x7fu][t] += (sb & 0x80u) ? -d : d;
-        }
+    for (int m = 0; m < 3; m++) {
+      const int f = fr[m];
+      const int64_t r = (int64_t)f * 4 + (lfq[m] & 3);
+      const uint8_t fl = a.face_flags[f];
+      const int2 ud = *reinterpret_cast<const int2 *>(a.face_cells + 2 * f);
+      noflow[m] = (fl & kFaceNoFlow) != 0;
+      seep[m] = (fl & kFaceSeepage) != 0;
+      up[m] = noflow[m] ? c : ud.x;
+      dn[m] = noflow[m] ? c : ud.y;
+#pragma unroll
+      for (int k = 0; k < kMaxCols; k++) T[m][k] = ra.T[(int64_t)k * ra.nrows + r];
+      G[m] = ra.G[r];
+      rup[m] = *(up[m] >= 0 ? ra.rho + up[m] : ra.rho_bnd + (-up[m] - 1));
+      rdn[m] = *(dn[m] >= 0 ? ra.rho + dn[m] : ra.rho_bnd + (-dn[m] - 1));
+      drup[m] = ra.drho[up[m] >= 0 ? up[m] : 0];
+      drdn[m] = ra.drho[dn[m] >= 0 ? dn[m] : 0];
+    }
+    double TP[3], den[3], ukvr[3], du[3], pb[3];
+    bool upw[3];
+    int uw[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      TP[m] = row_tp(T[m], ck, P);
+      den[m] = row_den(rup[m], rdn[m]);
+      upw[m] = row_upwind_up(TP[m], den[m], G[m]);
+      uw[m] = upw[m] ? up[m] : dn[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      ukvr[m] = *(uw[m] >= 0 ? ra.ukvr + uw[m] : ra.ukvr_bnd + (-uw[m] - 1));
+      du[m] = ra.dukvr[uw[m] >= 0 ? uw[m] : 0];
+      pb[m] = ra.Pbnd[uw[m] < 0 ? -uw[m] - 1 : 0];  // Pbnd always holds at least one entry
+    }
+    double sum[kMaxCols];
+#pragma unroll
+    for (int k = 0; k < kMaxCols; k++) sum[k] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      const bool zero = (uw[m] < 0) & seep[m] & (pb[m] <= ra.pref);
+      double jr[kMaxCols];
+      row_derivs(ra.nc, T[m], ck, up[m], dn[m], TP[m], den[m], G[m], ukvr[m], upw[m], zero, drup[m], drdn[m], du[m], jr);
+      const bool is_up = (code >> (18 + m)) & 1;
+#pragma unroll
+      for (int k = 0; k < kMaxCols; k++) sum[k] = noflow[m] ? sum[k] : (is_up ? sum[k] + jr[k] : sum[k] - jr[k]);
+      if (WITH_R) sf[lfq[m]][cl] = noflow[m] ? 0.0 : row_flux(TP[m], den[m], G[m], ukvr[m], zero);
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxCols; k++) sd[k][t] = sum[k];
+    sw[t] = w;
+  }
+  __syncthreads();
+  if (c >= 0 && corner == 0) {
+    for (int j = 0; j < 8; j++) {
+      // a region's cells are distinct, so a corner's 8 slots are distinct (apart
+      // from kJacSlots, which is never read): all reads before the writes
+      const uint64_t wj = sw[t + j];
+      unsigned sb[kMaxCols];
+      double cur[kMaxCols];
+#pragma unroll
+      for (int k = 0; k < kMaxCols; k++) {
+        sb[k] = (unsigned)(wj >> (8 * k)) & 0xffu;
+        if (sb[k] == 0xffu) sb[k] = kJacSlots;
+        cur[k] = acc[sb[k]][cl];
       }
+#pragma unroll
+      for (int k = 0; k < kMaxCols; k++) acc[sb[k]][cl] = cur[k] + sd[k][t + j];
     }
     const double shift = a.dyn ? a.dyn[1] : a.shift, dtime = a.dyn ? a.dyn[0] : a.dtime;
     const double dv = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
                                     : a.dM[c] * a.vol[c] * (1.0 / dtime);
-    acc[a.jdslot[c]][t] += dv;
-    const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
-    const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
-    for (int sl = 0; sl < len; sl++) a.J[base + (int64_t)sl * a.jstride] = acc[sl][t];
+    const int ds = a.jdslot[c];
+    acc[ds][cl] = acc[ds][cl] + dv;
+    if (WITH_R) {
+      // fluxes, then accumulation, then the source (tdympfao_ts.c:191-212, tdympfao_snes.c:112-123)
+      double s = 0.0;
+      for (int lf = 0; lf < 6; lf++) {
+        const int f = a.cell_faces[c * 6 + lf];
+        const double fs = ((sf[lf * 4][cl] + sf[lf * 4 + 1][cl]) + sf[lf * 4 + 2][cl]) + sf[lf * 4 + 3][cl];
+        s += (a.face_cells[2 * f] == c) ? fs : -fs;
+      }
+      a.R[c] = cell_residual(a, c, s);
+    }
   }
+  __syncthreads();
+  if (c < 0) return;
+  const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
+  const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
+  for (int sl = corner; sl < len; sl += 8) a.J[base + (int64_t)sl * a.jstride] = acc[sl][cl];
 }
 
 // ELL SpMV over the slot-major layout: slot s of row i at s*n + i.

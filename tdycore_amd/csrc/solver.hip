@@ -885,9 +885,9 @@ static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, h
   int rc;
   if (fm.type == 0) {
     k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, ws.dyn, X, fm.Uprev, ws.Udot, nullptr);
-    rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB);
+    rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB, ws.bj.d_row, ws.NB);
   } else {
-    rc = launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB);
+    rc = launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB, ws.bj.d_row, ws.NB);
   }
   ctx->launch_dyn = nullptr;
   return rc;

@@ -43,7 +43,6 @@ struct HostTopology {
   std::vector<int32_t> vcols;     // nv*kMaxCols: cell ids then nc+b, -1 padding
   std::vector<IRDesc> ir;         // nv
   std::vector<int32_t> sc_faces;  // nc*8*3 subcell face triples (oriented)
-  std::vector<uint8_t> row_ul;    // 4*nf*2: (up, dn) local IR index of the row's cells, 255 = none
   std::vector<uint8_t> jslot;     // nv*64: position of column cell k in Jacobian row of cell i
   std::vector<int32_t> jrp, jcols, jdiag;
   std::vector<int32_t> bnd_cell;  // nbnd
@@ -156,7 +155,10 @@ struct tdyhip_ctx {
   int32_t *d_vcols = nullptr;       // nv*kMaxCols
   double *d_T = nullptr;            // kMaxCols * 4nf (column-major ELL over face-major rows)
   double *d_G = nullptr;            // 4nf gravity term
-  uint64_t *d_gtab = nullptr;       // 24*nc per-cell gather table (built at setup from row_ul, jslot)
+  uint8_t *d_jslot = nullptr;       // nv*64: Jacobian-row slot of (region cell i, column k)
+  uint32_t *d_crow = nullptr;       // nc*8: a corner's three flux rows, the cell's region index, its sides
+  int32_t *d_cell_verts = nullptr;  // nc*8 (hexahedra)
+  int cj_xcd = 1;                   // k_cell_jac: XCD-grouped workgroup numbering (TDYHIP_CELLJAC_XCD=0: off)
   int32_t *d_jrp = nullptr, *d_jcols = nullptr;
   uint8_t *d_jdslot = nullptr;      // nc: slot of the diagonal within its row
   int jell_w = 0;                   // ELL width (longest Jacobian row)
@@ -167,7 +169,6 @@ struct tdyhip_ctx {
   double *d_dM = nullptr, *d_d2M = nullptr, *d_acc = nullptr;
   double *d_Pbnd = nullptr, *d_ukvr_bnd = nullptr, *d_rho_bnd = nullptr;
   double *d_flux = nullptr;         // 4nf
-  double *d_jrow = nullptr;         // 4nf*kMaxCols flux-row derivatives (row-major, 64 B per row)
   double *d_accum_prev = nullptr;   // nc
   double *d_source = nullptr;       // nc source_sink (tdyhip_set_source_sink)
   bool has_source = false;

@@ -99,7 +99,6 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
 
   t.ir.assign(nv, IRDesc{});
   t.vcols.assign((size_t)nv * kMaxCols, -1);
-  t.row_ul.assign((size_t)4 * nf * 2, 255);
   for (int v = 0; v < nv; v++) {
     IRDesc &d = t.ir[v];
     std::memset(&d, 0, sizeof(d));
@@ -147,16 +146,6 @@ std::string build_topology(const tdyhip_mesh &m, HostTopology &t) {
       if (d.fup_i[k] == 255 || (d.role[k] == 0 && d.fdn_i[k] == 255)) return "face without its subcells";
       int f = vface[(size_t)v * kMaxIRFaces + k], q = vsubf[(size_t)v * kMaxIRFaces + k];
       if (d.role[k] != 1) d.row[k] = f * 4 + q;
-      int r = f * 4 + q;
-      int up = fc[2 * f], dn = fc[2 * f + 1];
-      uint8_t ul = 255, dl = 255;
-      for (int i = 0; i < ncell; i++) {
-        int cc = vcell[(size_t)v * kMaxIRCells + i];
-        if (cc == up) ul = (uint8_t)i;
-        if (cc == dn) dl = (uint8_t)i;
-      }
-      t.row_ul[(size_t)r * 2] = ul;
-      t.row_ul[(size_t)r * 2 + 1] = dl;
     }
   }
 
