@@ -236,6 +236,14 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out /* 4*nf */);
  * transmissibility entries, the algorithmic FP64 flops and bytes of the local systems, and the
  * team width used for each bucket (0 for an empty bucket). */
 int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms /* 3 */, int64_t *counts /* 11 */);
+/* The autotuned plan of a general mesh's interaction-region launch (no counterpart in the
+ * reference, whose solves run one region at a time: tdympfao.c:1126-1513).  plan[5] = team width
+ * of each bucket (0: empty), 1 when the buckets are drained largest regions first, work-item
+ * chunk in workgroup steps; ms[13] = the tuning runs (ms, 0 when not run): bucket x width
+ * 16/32/64 (9), then order (smallest / largest first) x chunk (1 / 8) (4).  Plans are cached per
+ * device and bucket shapes, so a later setup of the same shapes runs no tuning.  A hexahedral
+ * mesh within the fixed-size limits has one kernel and reports plan = {16, 0, 0, 0, 1}. */
+int tdyhip_setup_plan(const tdyhip_ctx *ctx, int32_t *plan /* 5 */, double *ms /* 13 */);
 
 #ifdef __cplusplus
 }

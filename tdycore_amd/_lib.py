@@ -23,7 +23,7 @@ EXPORTS = [
     "tdyhip_solver_defaults", "tdyhip_newton_snes", "tdyhip_run_snes_steps", "tdyhip_run_ts_be",
     "tdyhip_run_tdyti", "tdyhip_set_pc_partition", "tdyhip_kernel_timers", "tdyhip_kernel_timer_report",
     "tdyhip_device_bytes", "tdyhip_diagnostics", "tdyhip_get_stream", "tdyhip_cancel",
-    "tdyhip_setup_info", "tdyhip_recover_velocity", "tdyhip_set_source_sink",
+    "tdyhip_setup_info", "tdyhip_setup_plan", "tdyhip_recover_velocity", "tdyhip_set_source_sink",
     "tdyhip_ifunction_noalloc",
 ]
 
@@ -100,7 +100,7 @@ def load():
         "tdyhip_run_ts_be": [P, P, D, C.c_int32, P, P], "tdyhip_run_tdyti": [P, P, D, D, D, D, P, P],
         "tdyhip_set_pc_partition": [P, P], "tdyhip_kernel_timers": [P, C.c_int32],
         "tdyhip_kernel_timer_report": [P, C.c_char_p, C.c_int64], "tdyhip_device_bytes": [P, P],
-        "tdyhip_diagnostics": [P, P, P, P, P, P], "tdyhip_get_stream": [P, P], "tdyhip_cancel": [P, C.c_int32], "tdyhip_setup_info": [P, P, P],
+        "tdyhip_diagnostics": [P, P, P, P, P, P], "tdyhip_get_stream": [P, P], "tdyhip_cancel": [P, C.c_int32], "tdyhip_setup_info": [P, P, P], "tdyhip_setup_plan": [P, P, P],
         "tdyhip_recover_velocity": [P, P, P, P, P], "tdyhip_set_source_sink": [P, P],
         "tdyhip_ifunction_noalloc": [P, D, P, P, P, P],
     }

@@ -316,35 +316,74 @@ using namespace tdyhip;
 
 namespace tdyhip {
 
-template <int LANES>
-static int launch_ir_general(const GenArgs &ga, hipStream_t st) {
-  constexpr int TEAMS = 64 / LANES;
-  const size_t lds = (size_t)TEAMS * ga.maxU * ga.pitch * sizeof(double);
-  if (lds > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ir_general<LANES>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+// One persistent launch of the interaction-region solves (k_ir_persistent)
+// over the buckets in `order` (bucket ids), each at its team width; the
+// counter is zeroed first.  Every schedule, chunk and width computes the
+// same values (one lane per equation, the same operations in the same order).
+struct IRPlan {
+  int lanes[3] = {0, 0, 0};
+  int order[3] = {0, 1, 2};
+  int chunk = 1;
+};
+
+static int ir_lds_bytes(const GenArgs (&ga)[3], const int (&lanes)[3]) {
+  size_t lds = 8;
+  for (int b = 0; b < 3; b++)
+    if (ga[b].n > 0 && lanes[b]) lds = std::max(lds, (size_t)(64 / lanes[b]) * ga[b].maxU * ga[b].pitch * sizeof(double));
+  return (int)lds;
+}
+
+static int launch_ir_persistent(const GenArgs (&ga)[3], const IRPlan &plan, int *d_counter, int grid,
+                                hipStream_t st) {
+  IRSchedule sc;
+  for (int b = 0; b < 3; b++) { sc.ga[b] = ga[b]; sc.lanes[b] = plan.lanes[b]; }
+  sc.nseg = 0;
+  int items = 0;
+  for (int k = 0; k < 3; k++) {
+    const int b = plan.order[k];
+    if (b < 0 || ga[b].n == 0 || !plan.lanes[b]) continue;
+    const int teams = 64 / plan.lanes[b];
+    const int nblk = (ga[b].n + teams - 1) / teams;
+    items += (nblk + plan.chunk - 1) / plan.chunk;
+    sc.seg_bucket[sc.nseg] = b;
+    sc.seg_end[sc.nseg] = items;
+    sc.nseg++;
+  }
+  if (sc.nseg == 0) return 0;
+  sc.chunk = plan.chunk;
+  sc.counter = d_counter;
+  const int lds = ir_lds_bytes(ga, plan.lanes);
+  if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ir_persistent),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) {
     set_error("interaction regions too large for the LDS of one workgroup");
     return TDYHIP_ERR_ARG;
   }
-  k_ir_general<LANES><<<(unsigned)((ga.n + TEAMS - 1) / TEAMS), 64, lds, st>>>(ga);
+  if (hipMemsetAsync(d_counter, 0, sizeof(int), st) != hipSuccess) return TDYHIP_ERR_HIP;
+  k_ir_persistent<<<(unsigned)std::max(1, std::min(grid, items)), 64, lds, st>>>(sc);
   return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }
 
-static int launch_ir_lanes(int lanes, const GenArgs &ga, hipStream_t st) {
-  return lanes == 16 ? launch_ir_general<16>(ga, st) : lanes == 32 ? launch_ir_general<32>(ga, st)
-                                                                   : launch_ir_general<64>(ga, st);
+// workgroups of k_ir_persistent resident at once on the device (its grid)
+static int ir_grid(const GenArgs (&ga)[3], const IRPlan &plan) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_ir_persistent), 64,
+                                                  (size_t)ir_lds_bytes(ga, plan.lanes)) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  return ncu * per_cu;
 }
 
-// Autotuned team width per bucket: a bucket of at least kTuneMin regions is
-// run once at every width that holds its largest region (16, 32, 64 lanes;
-// a wider team runs fewer regions per wavefront) and the fastest width is
-// kept for this device, bucket and largest-region shape for the rest of the
-// process.  Every width computes the same values (one lane per equation,
-// the same operations in the same order), so the tuning runs also leave T
-// complete.
+// Autotuning of the launch, kept per device and bucket shapes for the rest of
+// the process: first the team width of each bucket of at least kTuneMin
+// regions (that bucket alone at every width that holds its largest region,
+// 16, 32, 64 lanes; a wider team runs fewer regions per wavefront), then the
+// bucket order (largest regions first or last) and the chunk of workgroup
+// steps per work item (1 or 8) over all buckets together.  The tuning runs
+// leave T complete.
 constexpr int kTuneMin = 4096;
 static std::mutex g_tune_mu;
-static std::map<std::array<int, 4>, int> g_tune;
+static std::map<std::array<int, 10>, IRPlan> g_tune;
 
 // Transmissibility setup of a general (prism) mesh: G matrices, the
 // size-bucketed local systems and the gravity term, with HIP-event times of
@@ -403,65 +442,106 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
   (void)hipEventSynchronize(ev[1]);
   float gm_ms = 0.f;
   (void)hipEventElapsedTime(&gm_ms, ev[0], ev[1]);
-  GenArgs ga{nullptr, d_vdesc, d_csub, d_csck, d_fa, d_fb, d_Gsub, ctx->d_Tgen, d_status, 0, 1, 1};
-  int dev = 0;
-  (void)hipGetDevice(&dev);
+  GenArgs ga[3];
+  std::array<int, 10> key{};
+  (void)hipGetDevice(&key[0]);
   for (int b = 0; b < 3; b++) {
-    ga.vlist = d_vl[b];
-    ga.n = (int)g->bucket[b].size();
-    g->lanes[b] = 0;
-    if (ga.n == 0) continue;
+    ga[b] = GenArgs{d_vl[b], d_vdesc, d_csub, d_csck, d_fa, d_fb, d_Gsub, ctx->d_Tgen, d_status,
+                    (int)g->bucket[b].size(), 1, 1};
     int mu = 1, mb = 1;
     for (int v : g->bucket[b]) {
       mu = std::max(mu, g->vdesc[(size_t)v * 8 + 2]);
       mb = std::max(mb, g->vdesc[(size_t)v * 8 + 0] + g->vdesc[(size_t)v * 8 + 3]);
     }
-    ga.maxU = mu;
-    ga.pitch = (mu + mb) | 1;
-    const std::array<int, 4> key{dev, b, mu, mb};
-    int lanes = 16 << b;
-    bool cached = false;
-    {
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      auto it = g_tune.find(key);
-      if (it != g_tune.end()) { lanes = it->second; cached = true; }
+    ga[b].maxU = mu;
+    ga[b].pitch = (mu + mb) | 1;
+    key[1 + 3 * b] = ga[b].n >= kTuneMin;  // buckets below the tuning size take the default width
+    key[2 + 3 * b] = mu;
+    key[3 + 3 * b] = mb;
+  }
+  int *d_counter = nullptr;
+  if ((rc = up(&d_counter, (const int *)nullptr, 1))) return done(rc);
+  IRPlan plan;
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) { plan = it->second; cached = true; }
+  }
+  for (int b = 0; b < 3; b++) {
+    g->lanes[b] = 0;
+    if (!cached) plan.lanes[b] = ga[b].n ? 16 << b : 0;
+  }
+  // TDYHIP_IR_PLAN="w0,w1,w2,largest_first,chunk" replaces the tuned plan
+  // (widths of empty buckets are ignored; a width must hold its bucket)
+  if (const char *e = getenv("TDYHIP_IR_PLAN")) {
+    int w[3], desc, chunk;
+    if (sscanf(e, "%d,%d,%d,%d,%d", &w[0], &w[1], &w[2], &desc, &chunk) != 5 || chunk < 1) {
+      set_error("TDYHIP_IR_PLAN: expected w0,w1,w2,largest_first,chunk");
+      return done(TDYHIP_ERR_ARG);
     }
-    if (!cached && ga.n >= kTuneMin) {
+    for (int b = 0; b < 3; b++) {
+      if (!ga[b].n) continue;
+      if (w[b] != 16 && w[b] != 32 && w[b] != 64) { set_error("TDYHIP_IR_PLAN: widths are 16, 32 or 64"); return done(TDYHIP_ERR_ARG); }
+      if (w[b] < (16 << b)) { set_error("TDYHIP_IR_PLAN: a width smaller than its bucket's regions"); return done(TDYHIP_ERR_ARG); }
+      plan.lanes[b] = w[b];
+    }
+    for (int k = 0; k < 3; k++) plan.order[k] = desc ? 2 - k : k;
+    plan.chunk = chunk;
+    cached = true;  // no tuning, nothing cached
+  }
+  auto timed = [&](const IRPlan &p, float &ms) -> int {
+    hipEvent_t t0, t1;
+    if (hipEventCreate(&t0) != hipSuccess) return TDYHIP_ERR_HIP;
+    if (hipEventCreate(&t1) != hipSuccess) { (void)hipEventDestroy(t0); return TDYHIP_ERR_HIP; }
+    (void)hipEventRecord(t0, st);
+    int r = launch_ir_persistent(ga, p, d_counter, ir_grid(ga, p), st);
+    (void)hipEventRecord(t1, st);
+    (void)hipEventSynchronize(t1);
+    ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return r;
+  };
+  if (!cached) {
+    for (int b = 0; b < 3; b++) {
+      if (ga[b].n < kTuneMin) continue;
       float best = 1e30f;
       for (int w = 16 << b; w <= 64; w *= 2) {
-        hipEvent_t t0, t1;
-        (void)hipEventCreate(&t0);
-        (void)hipEventCreate(&t1);
-        (void)hipEventRecord(t0, st);
-        if ((rc = launch_ir_lanes(w, ga, st))) return done(rc);
-        (void)hipEventRecord(t1, st);
-        (void)hipEventSynchronize(t1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, t0, t1);
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
+        IRPlan one;
+        one.lanes[b] = w;
+        one.order[0] = b; one.order[1] = -1; one.order[2] = -1;
+        float ms;
+        if ((rc = timed(one, ms))) return done(rc);
         g->tune_ms[b][w == 16 ? 0 : w == 32 ? 1 : 2] = ms;
-        if (ms < best) { best = ms; lanes = w; }
+        if (ms < best) { best = ms; plan.lanes[b] = w; }
       }
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tune[key] = lanes;
     }
-    g->lanes[b] = lanes;
+    int nbig = 0;
+    for (int b = 0; b < 3; b++) nbig += ga[b].n >= kTuneMin;
+    if (nbig > 0) {
+      float best = 1e30f;
+      IRPlan base = plan;
+      for (int desc = 0; desc < 2; desc++)
+        for (int chunk : {1, 8}) {
+          IRPlan p = base;
+          for (int k = 0; k < 3; k++) p.order[k] = desc ? 2 - k : k;
+          p.chunk = chunk;
+          float ms;
+          if ((rc = timed(p, ms))) return done(rc);
+          g->sched_ms[desc * 2 + (chunk == 8)] = ms;
+          if (ms < best) { best = ms; plan = p; }
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = plan;
   }
+  for (int b = 0; b < 3; b++) g->lanes[b] = plan.lanes[b];
+  g->ir_desc = plan.order[0] == 2;
+  g->ir_chunk = plan.chunk;
   (void)hipEventRecord(ev[1], st);
-  for (int b = 0; b < 3; b++) {
-    if (!g->lanes[b]) continue;
-    ga.vlist = d_vl[b];
-    ga.n = (int)g->bucket[b].size();
-    int mu = 1, mb = 1;
-    for (int v : g->bucket[b]) {
-      mu = std::max(mu, g->vdesc[(size_t)v * 8 + 2]);
-      mb = std::max(mb, g->vdesc[(size_t)v * 8 + 0] + g->vdesc[(size_t)v * 8 + 3]);
-    }
-    ga.maxU = mu;
-    ga.pitch = (mu + mb) | 1;
-    if ((rc = launch_ir_lanes(g->lanes[b], ga, st))) return done(rc);
-  }
+  if ((rc = launch_ir_persistent(ga, plan, d_counter, ir_grid(ga, plan), st))) return done(rc);
   (void)hipEventRecord(ev[2], st);
   k_gravdis<<<nblk(nf, 256), 256, 0, st>>>(nf, ctx->gmatrix, ctx->gravity, ctx->d_face_cells, ctx->d_face_flags, d_cc,
                                            d_fc, d_fa2, d_fn, d_K, d_fnv, ctx->d_G);
@@ -1179,6 +1259,23 @@ int tdyhip_setup_info(const tdyhip_ctx *ctx, double *ms, int64_t *counts) {
     counts[6] = (int64_t)ctx->ls_flops;
     counts[7] = (int64_t)ctx->ls_bytes;
     counts[8] = kTeam;
+  }
+  return 0;
+}
+
+int tdyhip_setup_plan(const tdyhip_ctx *ctx, int32_t *plan, double *ms) {
+  if (!ctx || !plan || !ms) { set_error("null argument"); return TDYHIP_ERR_ARG; }
+  for (int i = 0; i < 13; i++) ms[i] = 0.0;
+  if (ctx->general) {
+    const HostGeneral &g = *ctx->hgen;
+    for (int b = 0; b < 3; b++) plan[b] = g.lanes[b];
+    plan[3] = g.ir_desc ? 1 : 0;
+    plan[4] = g.ir_chunk;
+    for (int b = 0; b < 3; b++)
+      for (int w = 0; w < 3; w++) ms[b * 3 + w] = g.tune_ms[b][w];
+    for (int i = 0; i < 4; i++) ms[9 + i] = g.sched_ms[i];
+  } else {
+    plan[0] = kTeam; plan[1] = plan[2] = plan[3] = 0; plan[4] = 1;
   }
   return 0;
 }

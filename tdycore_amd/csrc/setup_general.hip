@@ -5,10 +5,11 @@
 // faces, so its local system is 3d x 3d (tdympfao.c:1126-1513 sized by the
 // region): 18 x 18 at d = 6, up to 36 x 36 and more on a random Delaunay
 // triangulation, against 12 x 12 for an interior hexahedral vertex.  The host
-// sorts the vertices into buckets by max(unknowns, columns) <= 16, 32, 64, and
-// each bucket is one launch of k_ir_general<LANES> with a team of LANES lanes
-// per region and 64 / LANES regions per one-wavefront workgroup, so a small
-// region does not pay for the largest one.
+// sorts the vertices into buckets by max(unknowns, columns) <= 16, 32, 64; a
+// bucket's regions take a team of LANES lanes each, 64 / LANES regions per
+// one-wavefront workgroup step, so a small region does not pay for the
+// largest one, and one persistent launch (k_ir_persistent) drains every
+// bucket.
 //
 // Per region the team assembles [A | B] in LDS (lane = equation), eliminates
 // with partial pivoting applied to B as it goes (the same operations, in the
@@ -68,16 +69,17 @@ __device__ inline double crow_at(const int (&col)[3], const double (&val)[3], in
   return col[0] == j ? val[0] : col[1] == j ? val[1] : col[2] == j ? val[2] : 0.0;
 }
 
+// One workgroup step (64 lanes, one wavefront): regions blk * TEAMS + team of
+// the bucket, one team of LANES lanes each.  [A | B] of a region sits in LDS,
+// rows of a.pitch doubles (odd: column reads spread over the banks), sized by
+// the bucket's largest region.
 template <int LANES>
-__global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
+__device__ inline void ir_block(const GenArgs &a, int blk, double *lds) {
 #pragma clang fp contract(off)
-  // [A | B] of a region in LDS, rows of a.pitch doubles (odd: column reads
-  // spread over the banks), sized by the bucket's largest region
-  extern __shared__ double lds[];
   constexpr int TEAMS = 64 / LANES;
   const int P = a.pitch;
   const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-  const int vi = blockIdx.x * TEAMS + team;
+  const int vi = blk * TEAMS + team;
   const bool act = vi < a.n;  // idle teams still reach every barrier
   double *A = lds + (size_t)team * a.maxU * P, *B = A + a.maxU;
   int ncell = 0, nface = 0, nU = 0, ndir = 0, oc = 0, of = 0, ot = 0;
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
     }
   }
   __syncthreads();
-  if (!act || bad) return;
+  if (act && !bad) {
   // T = C_up X - F_up (+ C_up,dir) for internal and Dirichlet faces
   for (int k = 0; k < nface; k++) {
     const int fak = a.fa[of + k], fbk = a.fb[of + k];
@@ -172,6 +174,47 @@ __global__ void __launch_bounds__(64) k_ir_general(GenArgs a) {
         if (cu[m] < nU) s += vu[m] * B[cu[m] * P + lane];
       const double t = lane < ncell ? s + ((lane == ui) ? -Fs : 0.0) : s + crow_at(cu, vu, nU + lane - ncell);
       a.T[(int64_t)ot + (int64_t)trow * nB + lane] = t;
+    }
+  }
+  }
+  __syncthreads();  // the LDS is reused by the workgroup's next step
+}
+
+// One persistent launch over every bucket.  Work items are chunks of `chunk`
+// workgroup steps of one bucket, laid out segment by segment in the
+// schedule's bucket order; each one-wavefront workgroup takes the next item
+// from a device counter (zeroed before the launch) until none is left, which
+// every workgroup reaches.  The LDS holds the largest bucket's tiles.
+struct IRSchedule {
+  GenArgs ga[3];      // per bucket: region list, count, largest region, pitch
+  int lanes[3];       // team width of each bucket (16, 32, 64)
+  int nseg;           // buckets in schedule order
+  int seg_bucket[3];
+  int seg_end[3];     // items up to the end of each segment
+  int chunk;          // workgroup steps per item
+  int *counter;
+};
+
+__global__ void __launch_bounds__(64) k_ir_persistent(IRSchedule sc) {
+  extern __shared__ double lds[];
+  const int total = sc.seg_end[sc.nseg - 1];
+  for (;;) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(sc.counter, 1);
+    item = __shfl(item, 0);
+    if (item >= total) return;
+    int s = 0;
+    while (item >= sc.seg_end[s]) s++;
+    const int b = sc.seg_bucket[s];
+    const GenArgs &a = sc.ga[b];
+    const int first = (item - (s ? sc.seg_end[s - 1] : 0)) * sc.chunk;
+    const int L = sc.lanes[b];
+    const int nblk = (a.n + 64 / L - 1) / (64 / L);
+    const int last = min(first + sc.chunk, nblk);
+    for (int blk = first; blk < last; blk++) {
+      if (L == 16) ir_block<16>(a, blk, lds);
+      else if (L == 32) ir_block<32>(a, blk, lds);
+      else ir_block<64>(a, blk, lds);
     }
   }
 }

@@ -85,6 +85,9 @@ struct HostGeneral {
   std::vector<int32_t> jrp, jcols, jdiag;  // vertex-adjacency Jacobian pattern
   int lanes[3] = {0, 0, 0};       // team width chosen for each bucket (0: empty bucket)
   float tune_ms[3][3] = {};       // tuning runs: bucket x width 16/32/64 (0: not run)
+  float sched_ms[4] = {};         // tuning runs of the whole launch: order (small, large first) x chunk (1, 8)
+  bool ir_desc = false;           // buckets drained largest regions first
+  int ir_chunk = 1;               // workgroup steps per work item
 };
 std::string build_topology_general(const tdyhip_mesh &m, HostGeneral &g);
 // True when every cell has 8 vertices and every face 4 (the hexahedral path).

@@ -148,6 +148,17 @@ class MPFAO:
                 "trans_nnz": int(cnt[5]), "local_systems_flops": int(cnt[6]), "local_systems_bytes": int(cnt[7]),
                 "lanes": [int(x) for x in cnt[8:11]]}
 
+    def setup_plan(self):
+        """The autotuned interaction-region launch: {'lanes' (team width per
+        bucket), 'largest_first', 'chunk', 'width_ms' (3x3 tuning runs,
+        bucket x width 16/32/64, 0 = not run), 'schedule_ms' (order
+        smallest/largest first x chunk 1/8)}."""
+        plan = np.zeros(5, np.int32)
+        ms = np.zeros(13)
+        _lib.check(_lib.load().tdyhip_setup_plan(self._h, plan.ctypes.data, ms.ctypes.data))
+        return {"lanes": [int(x) for x in plan[:3]], "largest_first": bool(plan[3]), "chunk": int(plan[4]),
+                "width_ms": ms[:9].reshape(3, 3).tolist(), "schedule_ms": ms[9:].tolist()}
+
     # ---- TS callbacks ----
     def IFunction(self, t, U, U_t, R):
         self._run(_lib.load().tdyhip_ifunction, float(t), _ptr(U), _ptr(U_t), _ptr(R), None)

@@ -62,9 +62,41 @@ def test_prism_setup_matches_oracle(name):
         assert info["max_cells"] == 12 and info["max_faces"] == 18  # took the general path
     for b, w in zip(info["buckets"], info["lanes"]):
         assert (w == 0) == (b == 0) and w in (0, 16, 32, 64)
+    plan = g.setup_plan()
+    assert plan["lanes"] == info["lanes"] and plan["chunk"] >= 1
     if name == "delaunay_tuned":
-        assert max(info["buckets"]) >= 4096
+        # the first setup of these bucket shapes in the process tunes: every
+        # width of the large bucket, then order x chunk over all buckets
+        big = int(np.argmax(info["buckets"]))
+        assert info["buckets"][big] >= 4096
+        assert all(plan["width_ms"][big][w] > 0 for w in range(big, 3))
+        assert all(x > 0 for x in plan["schedule_ms"])
     g.close()
+
+
+@pytest.mark.parametrize("name", ["wheel", "delaunay_tuned"])
+def test_prism_setup_plans_agree_bitwise(name, monkeypatch):
+    """One persistent launch drains every bucket; whatever the plan (team
+    width per bucket, bucket order, work-item chunk), T is the same bit for
+    bit, and the forced plan is the one reported."""
+    from tdycore_amd.mpfao import MPFAO
+    print(f"[case] prism setup plans {name}", flush=True)
+    P = _problem(name)
+    args = (P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"])
+    g = MPFAO(*args, **P["kw"])
+    v0 = g.trans_csr()[2]
+    buckets = g.setup_info()["buckets"]
+    g.close()
+    for spec in ("64,64,64,0,1", "16,32,64,1,8", "32,64,64,1,3", "16,64,64,0,2"):
+        monkeypatch.setenv("TDYHIP_IR_PLAN", spec)
+        g = MPFAO(*args, **P["kw"])
+        v = g.trans_csr()[2]
+        plan = g.setup_plan()
+        g.close()
+        w = [int(x) for x in spec.split(",")]
+        assert plan["lanes"] == [w[b] if buckets[b] else 0 for b in range(3)]
+        assert plan["largest_first"] == bool(w[3]) and plan["chunk"] == w[4]
+        assert np.array_equal(v, v0), spec
 
 
 @pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet", "hex_star"])
