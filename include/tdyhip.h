@@ -204,6 +204,31 @@ int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, 
  * criteria): it is for servers that bound the time a request may hold the device. */
 int tdyhip_cancel(tdyhip_ctx *ctx, int32_t on);
 
+/* Fused serving batch.  No counterpart in the reference (PETSc solves one problem per SNES):
+ * it is for servers running an ensemble of requests at once.  A batch holds up to 32 requests,
+ * each a context of a hexahedral mesh within the fixed-size limits with its block-Jacobi
+ * partition set, its device state U (in/out) and the parameters of
+ * TDyTimeIntegratorRunToTime (src/tdyti.c:100-196, as tdyhip_run_tdyti).  Each round advances
+ * every request by one Newton iteration (or starts its next time step): the state, residual
+ * and Jacobian evaluations of all requests run as one segmented launch each, the Krylov
+ * solves side by side on the requests' own streams.  A request's results are the same bit
+ * for bit alone, in any mix, and through tdyhip_run_tdyti.  Requests may be added between
+ * rounds; a request that reaches its final time or fails leaves the batch in that round and
+ * is reported (id, rc: 0 or the error code tdyhip_run_tdyti would return, its stats; the
+ * last error message is that of the last failed request).  The caller keeps ownership of the
+ * contexts and states and must not use them while they are in the batch; cancellation
+ * (tdyhip_cancel) ends a request at its next round with TDYHIP_ERR_CANCELLED. */
+typedef struct tdyhip_batch tdyhip_batch;
+int tdyhip_batch_create(tdyhip_batch **out);
+int tdyhip_batch_destroy(tdyhip_batch *b);
+int tdyhip_batch_add(tdyhip_batch *b, tdyhip_ctx *ctx, double *U, const tdyhip_solver_opts *o, double dt_init,
+                     double growth, double dt_max, double final_time, int32_t *id);
+int tdyhip_batch_remove(tdyhip_batch *b, int32_t id);
+int tdyhip_batch_size(const tdyhip_batch *b, int32_t *n);
+/* cap: length of ids, rcs and stats (at least the batch size); ndone: entries written */
+int tdyhip_batch_round(tdyhip_batch *b, int32_t cap, int32_t *ids, int32_t *rcs, tdyhip_solver_stats *stats,
+                       int32_t *ndone);
+
 /* Device memory held by the context (setup arrays plus the solver workspace), in
  * bytes: what an HBM-footprint admission policy accounts per problem.  No
  * counterpart in the reference (PETSc's PetscMallocGetCurrentUsage plays that

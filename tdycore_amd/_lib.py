@@ -24,7 +24,8 @@ EXPORTS = [
     "tdyhip_run_tdyti", "tdyhip_set_pc_partition", "tdyhip_kernel_timers", "tdyhip_kernel_timer_report",
     "tdyhip_device_bytes", "tdyhip_diagnostics", "tdyhip_get_stream", "tdyhip_cancel",
     "tdyhip_setup_info", "tdyhip_setup_plan", "tdyhip_recover_velocity", "tdyhip_set_source_sink",
-    "tdyhip_ifunction_noalloc",
+    "tdyhip_ifunction_noalloc", "tdyhip_batch_create", "tdyhip_batch_destroy", "tdyhip_batch_add",
+    "tdyhip_batch_remove", "tdyhip_batch_size", "tdyhip_batch_round",
 ]
 
 
@@ -103,6 +104,9 @@ def load():
         "tdyhip_diagnostics": [P, P, P, P, P, P], "tdyhip_get_stream": [P, P], "tdyhip_cancel": [P, C.c_int32], "tdyhip_setup_info": [P, P, P], "tdyhip_setup_plan": [P, P, P],
         "tdyhip_recover_velocity": [P, P, P, P, P], "tdyhip_set_source_sink": [P, P],
         "tdyhip_ifunction_noalloc": [P, D, P, P, P, P],
+        "tdyhip_batch_create": [P], "tdyhip_batch_destroy": [P],
+        "tdyhip_batch_add": [P, P, P, P, D, D, D, D, P], "tdyhip_batch_remove": [P, C.c_int32],
+        "tdyhip_batch_size": [P, P], "tdyhip_batch_round": [P, C.c_int32, P, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

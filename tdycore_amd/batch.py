@@ -1,0 +1,73 @@
+"""Fused serving batch over the C ABI (tdyhip_batch_*).
+
+Several requests - MPFAO contexts of hexahedral meshes with their block-Jacobi
+partitions set, device states and time-stepping parameters - advance in
+lockstep, one Newton iteration per ``round()``: the state, residual and
+Jacobian evaluations of all of them run as one segmented launch each, their
+Krylov solves side by side on the contexts' own streams.  Each request's
+result is the same bit for bit alone, in any mix, and through
+``MPFAO.run_tdyti`` (TDyTimeIntegratorRunToTime, src/tdyti.c:100-196).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+
+MAX_REQUESTS = 32
+
+
+class FusedBatch:
+    def __init__(self):
+        h = C.c_void_p()
+        _lib.check(_lib.load().tdyhip_batch_create(C.byref(h)))
+        self._h = h
+        self._members = {}  # id -> (context, state, options): kept alive while in the batch
+
+    def add(self, g, U, opts, dt_init, growth, dt_max, final_time) -> int:
+        """Add a request: U (float64 device tensor) is advanced in place."""
+        rid = C.c_int32()
+        _lib.check(_lib.load().tdyhip_batch_add(self._h, g._h, C.c_void_p(U.data_ptr()), C.byref(opts),
+                                                float(dt_init), float(growth), float(dt_max), float(final_time),
+                                                C.byref(rid)))
+        self._members[rid.value] = (g, U, opts)
+        return rid.value
+
+    def remove(self, rid: int):
+        _lib.check(_lib.load().tdyhip_batch_remove(self._h, int(rid)))
+        self._members.pop(rid, None)
+
+    def __len__(self):
+        n = C.c_int32()
+        _lib.check(_lib.load().tdyhip_batch_size(self._h, C.byref(n)))
+        return n.value
+
+    def round(self):
+        """One round; returns [(id, rc, stats dict)] of the requests that left
+        the batch (rc 0: reached the final time; else the error code)."""
+        cap = max(1, len(self._members))
+        ids = (C.c_int32 * cap)()
+        rcs = (C.c_int32 * cap)()
+        sts = (_lib.SolverStats * cap)()
+        nd = C.c_int32()
+        _lib.check(_lib.load().tdyhip_batch_round(self._h, cap, ids, rcs, sts, C.byref(nd)))
+        out = []
+        for k in range(nd.value):
+            st = sts[k]
+            out.append((ids[k], rcs[k], {"newton_its": st.newton_its, "linear_its": st.linear_its,
+                                         "steps": st.steps, "cuts": st.cuts, "status": st.status,
+                                         "krylov_fallbacks": st.krylov_fallbacks, "last_fnorm": st.last_fnorm}))
+            self._members.pop(ids[k], None)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.check(_lib.load().tdyhip_batch_destroy(self._h))
+            self._h = C.c_void_p()
+        self._members.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

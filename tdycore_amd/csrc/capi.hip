@@ -4,8 +4,11 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <cmath>
+#include <functional>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "flow_kernels.hip"
 #include "setup_kernels.hip"
@@ -259,18 +262,38 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   return a;
 }
 
+static CellStateArgs cell_state_args(tdyhip_ctx *ctx, int form, const double *U, const double *Udot) {
+  CellStateArgs a;
+  a.nc = ctx->nc; a.sp = state_params(ctx, form, 0.0);
+  a.U = U; a.Udot = Udot; a.phi = ctx->d_phi; a.Sr = ctx->d_Sr; a.vol = ctx->d_cell_vol; a.accum_prev = ctx->d_accum_prev;
+  a.rho = ctx->d_rho; a.ukvr = ctx->d_ukvr; a.drho = ctx->d_drho; a.dukvr = ctx->d_dukvr;
+  a.dM = ctx->d_dM; a.d2M = ctx->d_d2M; a.acc = ctx->d_acc;
+  return a;
+}
+static BndStateArgs bnd_state_args(tdyhip_ctx *ctx, int form, const double *U) {
+  BndStateArgs a;
+  a.nbnd = ctx->nbnd; a.sp = state_params(ctx, form, 0.0);
+  a.U = U; a.bnd_cell = ctx->d_bnd_cell; a.Sr = ctx->d_Sr; a.pbnd_fn = ctx->d_pbnd_fn;
+  a.Pbnd = ctx->d_Pbnd; a.ukvr_bnd = ctx->d_ukvr_bnd;
+  return a;
+}
+
 // state (+boundary) at U
 static int launch_state(tdyhip_ctx *ctx, int form, const double *U, const double *Udot, hipStream_t st) {
-  StateParams sp = state_params(ctx, form, 0.0);
   KT(ctx, st, "k_cell_state", bytes_state(ctx),
-     k_cell_state<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, sp, U, Udot, ctx->d_phi, ctx->d_Sr, ctx->d_cell_vol,
-                                                      ctx->d_accum_prev, ctx->d_rho, ctx->d_ukvr, ctx->d_drho,
-                                                      ctx->d_dukvr, ctx->d_dM, ctx->d_d2M, ctx->d_acc));
-  if (ctx->nbnd > 0)
-    k_bnd_state<<<nblk(ctx->nbnd, 256), 256, 0, st>>>(ctx->nbnd, sp, U, ctx->d_bnd_cell, ctx->d_Sr,
-                                                      ctx->d_pbnd_fn, ctx->d_Pbnd, ctx->d_ukvr_bnd);
+     k_cell_state<<<nblk(ctx->nc, 256), 256, 0, st>>>(cell_state_args(ctx, form, U, Udot)));
+  if (ctx->nbnd > 0) k_bnd_state<<<nblk(ctx->nbnd, 256), 256, 0, st>>>(bnd_state_args(ctx, form, U));
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+static CellJacArgs cell_jac_args(tdyhip_ctx *ctx, const int32_t *order, int64_t norder) {
+  CellJacArgs cj;
+  cj.order = order;
+  cj.npos = order ? norder : ctx->nc;
+  cj.jslot = ctx->d_jslot; cj.crow = ctx->d_crow; cj.cell_verts = ctx->d_cell_verts;
+  cj.xcd_groups = 0;
+  return cj;
 }
 
 // Residual (R) and / or Jacobian (J) at U.  The residual alone: flux rows,
@@ -294,10 +317,7 @@ static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U,
     KT(ctx, st, "k_cell_gather", bytes_gather(ctx),
        k_cell_gather<<<nblk(ctx->nc, kGatherBlock), kGatherBlock, 0, st>>>(ga));
   } else {
-    CellJacArgs cj;
-    cj.order = ell ? order : nullptr;
-    cj.npos = cj.order ? norder : ctx->nc;
-    cj.jslot = ctx->d_jslot; cj.crow = ctx->d_crow; cj.cell_verts = ctx->d_cell_verts;
+    CellJacArgs cj = cell_jac_args(ctx, ell ? order : nullptr, norder);
     const int64_t nb = (cj.npos + kCellJacCells - 1) / kCellJacCells;
     cj.xcd_groups = ctx->cj_xcd ? (int)((nb + 7) / 8) : 0;
     const unsigned g = cj.xcd_groups ? (unsigned)(8 * cj.xcd_groups) : (unsigned)nb;
@@ -1008,6 +1028,7 @@ int tdyhip_destroy(tdyhip_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   tdyhip::ws_destroy(ctx->ws);
   ctx->ws = nullptr;
+  if (ctx->ws_event) (void)hipEventDestroy(ctx->ws_event);
   for (void *p : ctx->allocations) (void)hipFree(p);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx->hgen;
@@ -1290,6 +1311,7 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out) {
 }  // extern "C"
 
 #include "solver.hip"
+#include "batch.hip"
 
 namespace tdyhip {
 static void ws_destroy(SolverWS *ws) {

@@ -34,18 +34,17 @@ struct StateParams {
   const int *stop;
 };
 
-__global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ U,
-                             const double *__restrict__ Udot, const double *__restrict__ phi,
-                             const double *__restrict__ Sr, const double *__restrict__ vol,
-                             const double *__restrict__ accum_prev, double *__restrict__ rho_o,
-                             double *__restrict__ ukvr_o, double *__restrict__ drho_o,
-                             double *__restrict__ dukvr_o, double *__restrict__ dM_o,
-                             double *__restrict__ d2M_o, double *__restrict__ acc_o) {
-  if (sp.stop && *sp.stop) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  const double p = U[c];
-  const double sr = Sr[c];
+struct CellStateArgs {
+  int nc;
+  StateParams sp;
+  const double *U, *Udot, *phi, *Sr, *vol, *accum_prev;
+  double *rho, *ukvr, *drho, *dukvr, *dM, *d2M, *acc;
+};
+
+__device__ inline void cell_state(const CellStateArgs &a, int c) {
+  const StateParams &sp = a.sp;
+  const double p = a.U[c];
+  const double sr = a.Sr[c];
   double S, dS, d2S;
   vg_saturation(sp.vg_m, sp.vg_alpha, sr, sp.pref - p, S, dS, d2S);
   const double Se = (S - sr) / (1.0 - sr);
@@ -56,20 +55,26 @@ __global__ void k_cell_state(int nc, StateParams sp, const double *__restrict__ 
   double rho, drho, d2rho;
   water_density(sp.density_type, p, rho, drho, d2rho);
   const double vis = kViscosity, dvis = 0.0;
-  rho_o[c] = rho;
-  ukvr_o[c] = Kr / vis;
-  drho_o[c] = drho;
-  dukvr_o[c] = dKr * dS / vis - Kr / (vis * vis) * dvis;
-  const double ph = phi[c];
+  a.rho[c] = rho;
+  a.ukvr[c] = Kr / vis;
+  a.drho[c] = drho;
+  a.dukvr[c] = dKr * dS / vis - Kr / (vis * vis) * dvis;
+  const double ph = a.phi[c];
   const double dM = drho * ph * S + rho * ph * dS;
   const double d2M = 2.0 * dS * drho * ph + S * d2rho * ph + d2S * rho * ph;
-  dM_o[c] = dM;
-  d2M_o[c] = d2M;
+  a.dM[c] = dM;
+  a.d2M[c] = d2M;
   if (sp.form == 0) {
-    acc_o[c] = dM * Udot[c] * vol[c];
+    a.acc[c] = dM * a.Udot[c] * a.vol[c];
   } else {
-    acc_o[c] = rho * ph * S * vol[c] / (sp.dyn ? sp.dyn[0] : sp.dtime) - accum_prev[c];
+    a.acc[c] = rho * ph * S * a.vol[c] / (sp.dyn ? sp.dyn[0] : sp.dtime) - a.accum_prev[c];
   }
+}
+
+__global__ void k_cell_state(CellStateArgs a) {
+  if (a.sp.stop && *a.sp.stop) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < a.nc) cell_state(a, c);
 }
 
 // rho phi S V / dt at Uprev (TDyMPFAOSNESPreSolve, tdympfao_snes.c:32-64)
@@ -102,24 +107,34 @@ __global__ void k_snes_presolve(int nc, StateParams sp, const double *__restrict
   accum_prev[c] = rho * phi[c] * S * vol[c] / sp.dtime;
 }
 
-__global__ void k_bnd_state(int nbnd, StateParams sp, const double *__restrict__ U,
-                            const int32_t *__restrict__ bnd_cell, const double *__restrict__ Sr,
-                            const double *__restrict__ pbnd_fn, double *__restrict__ Pbnd,
-                            double *__restrict__ ukvr_bnd) {
-  if (sp.stop && *sp.stop) return;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbnd) return;
-  const int c = bnd_cell[b];
-  const double P = pbnd_fn ? pbnd_fn[b] : U[c];
-  Pbnd[b] = P;
-  const double sr = Sr[c];
+struct BndStateArgs {
+  int nbnd;
+  StateParams sp;
+  const double *U;
+  const int32_t *bnd_cell;
+  const double *Sr, *pbnd_fn;
+  double *Pbnd, *ukvr_bnd;
+};
+
+__device__ inline void bnd_state(const BndStateArgs &a, int b) {
+  const StateParams &sp = a.sp;
+  const int c = a.bnd_cell[b];
+  const double P = a.pbnd_fn ? a.pbnd_fn[b] : a.U[c];
+  a.Pbnd[b] = P;
+  const double sr = a.Sr[c];
   double S, dS, d2S;
   vg_saturation(sp.vg_m, sp.vg_alpha, sr, sp.pref - P, S, dS, d2S);
   const double Se = (S - sr) / (1.0 - sr);
   double Kr, dKr;
   const double cc[4] = {sp.c0, sp.c1, sp.c2, sp.c3};
   mualem_kr(sp.vg_m, sp.x0, sp.x1, sp.dx, cc, Se, Kr, dKr);
-  ukvr_bnd[b] = Kr / kViscosity;
+  a.ukvr_bnd[b] = Kr / kViscosity;
+}
+
+__global__ void k_bnd_state(BndStateArgs a) {
+  if (a.sp.stop && *a.sp.stop) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < a.nbnd) bnd_state(a, b);
 }
 
 struct RowArgs {
@@ -207,10 +222,7 @@ __device__ inline void vertex_columns(const RowArgs &a, int v, int (&ck)[kMaxCol
 
 // Thread per flux row: the residual's flux rows (the Jacobian is assembled by
 // k_cell_jac, one pass per cell).
-__global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
-  if (a.stop && *a.stop) return;  // uniform over the workgroup
-  const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
-  if (r >= a.nrows) return;
+__device__ inline void flux_row(const RowArgs &a, int64_t r) {
   const int f = (int)(r >> 2);
   const uint8_t fl = a.face_flags[f];
   double flux = 0.0;
@@ -235,6 +247,12 @@ __global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
     flux = row_flux(TP, den, G, ukvr, zero);
   }
   a.flux[r] = flux;
+}
+
+__global__ void __launch_bounds__(kRowBlock) k_rows(RowArgs a) {
+  if (a.stop && *a.stop) return;  // uniform over the workgroup
+  const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+  if (r < a.nrows) flux_row(a, r);
 }
 
 // ---------------------------------------------------------------- general (prism) meshes
@@ -558,10 +576,7 @@ __device__ inline double cell_residual(const GatherArgs &a, int c, double s) {
 
 // Per-cell residual gather (thread per cell): every cell owns its entry, so
 // the result does not depend on scheduling.
-__global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
-  if (a.stop && *a.stop) return;
-  const int c = blockIdx.x * kGatherBlock + threadIdx.x;
-  if (c >= a.nc) return;
+__device__ inline void cell_gather(const GatherArgs &a, int c) {
   double s = 0.0;
 #pragma unroll
   for (int lf = 0; lf < 6; lf++) {
@@ -572,6 +587,12 @@ __global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
     s += (a.face_cells[2 * f] == c) ? fs : -fs;
   }
   a.R[c] = cell_residual(a, c, s);
+}
+
+__global__ void __launch_bounds__(kGatherBlock) k_cell_gather(GatherArgs a) {
+  if (a.stop && *a.stop) return;
+  const int c = blockIdx.x * kGatherBlock + threadIdx.x;
+  if (c < a.nc) cell_gather(a, c);
 }
 
 // Single-pass Jacobian (and, WITH_R, residual) of a hexahedral mesh
@@ -609,17 +630,14 @@ struct CellJacArgs {
 constexpr int kCellJacBlock = 256;                // lanes per workgroup
 constexpr int kCellJacCells = kCellJacBlock / 8;  // cells per workgroup
 
+// block blk of the cells (in `order`): kCellJacCells cells, 8 lanes each
 template <bool WITH_R>
-__global__ void __launch_bounds__(kCellJacBlock) k_cell_jac(RowArgs ra, GatherArgs a, CellJacArgs cj) {
+__device__ inline void cell_jac_block(const RowArgs &ra, const GatherArgs &a, const CellJacArgs &cj, int64_t blk) {
   __shared__ double sd[kMaxCols][kCellJacBlock];     // corner sums: column k of lane t at sd[k][t]
   __shared__ uint64_t sw[kCellJacBlock];             // their slots (one byte per column)
   __shared__ double sf[WITH_R ? 24 : 1][kCellJacCells];  // flux of row lf*4+q of cell cl
   __shared__ double acc[kJacSlots + 1][kCellJacCells];   // the row (slot kJacSlots: non-cell columns)
-  if (a.stop && *a.stop) return;  // uniform over the workgroup
   const int t = threadIdx.x, cl = t >> 3, corner = t & 7;
-  // with xcd_groups, hardware round-robin places workgroup w on XCD w % 8:
-  // give XCD x the contiguous run of blocks [x * xcd_groups, (x + 1) * xcd_groups)
-  const int64_t blk = cj.xcd_groups ? (int64_t)(blockIdx.x % 8) * cj.xcd_groups + blockIdx.x / 8 : blockIdx.x;
   const int64_t i = blk * kCellJacCells + cl;
   const int c = i < cj.npos ? (cj.order ? cj.order[i] : (int)i) : -1;  // -1: no cell
   if (c >= 0) {
@@ -732,6 +750,71 @@ __global__ void __launch_bounds__(kCellJacBlock) k_cell_jac(RowArgs ra, GatherAr
   const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
   const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
   for (int sl = corner; sl < len; sl += 8) a.J[base + (int64_t)sl * a.jstride] = acc[sl][cl];
+}
+
+template <bool WITH_R>
+__global__ void __launch_bounds__(kCellJacBlock) k_cell_jac(RowArgs ra, GatherArgs a, CellJacArgs cj) {
+  if (a.stop && *a.stop) return;  // uniform over the workgroup
+  // with xcd_groups, hardware round-robin places workgroup w on XCD w % 8:
+  // give XCD x the contiguous run of blocks [x * xcd_groups, (x + 1) * xcd_groups)
+  const int64_t blk = cj.xcd_groups ? (int64_t)(blockIdx.x % 8) * cj.xcd_groups + blockIdx.x / 8 : blockIdx.x;
+  cell_jac_block<WITH_R>(ra, a, cj, blk);
+}
+
+// ---------------------------------------------------------------- fused multi-request launches
+// One launch runs the same evaluation for several independent problems (the
+// requests of a serving batch), each a segment of the grid: a workgroup
+// belongs to the segment whose block range holds it and computes exactly
+// what the single-problem kernel computes for its local block, so each
+// request's result is the same bit for bit whatever else shares the launch.
+// A segment whose stop flag is set does nothing.
+constexpr int kMaxSeg = 32;
+struct SegBlocks {
+  int nseg;
+  int64_t start[kMaxSeg + 1];  // first global block of each segment; start[nseg] = the grid
+};
+__device__ inline int seg_find(const SegBlocks &sb, int64_t b) {
+  int s = 0;
+  while (s + 1 < sb.nseg && b >= sb.start[s + 1]) s++;
+  return s;
+}
+struct SegProblem {  // one request's arguments of every fused launch
+  CellStateArgs cs;
+  BndStateArgs bs;
+  RowArgs ra;
+  GatherArgs ga;
+  CellJacArgs cj;
+  const int *stop;
+};
+
+__global__ void __launch_bounds__(256) k_seg_state(const SegProblem *__restrict__ P, SegBlocks sb) {
+  const int s = seg_find(sb, blockIdx.x);
+  if (P[s].stop && *P[s].stop) return;
+  const int c = (int)((blockIdx.x - sb.start[s]) * 256 + threadIdx.x);
+  if (c < P[s].cs.nc) cell_state(P[s].cs, c);
+}
+__global__ void __launch_bounds__(256) k_seg_bnd_state(const SegProblem *__restrict__ P, SegBlocks sb) {
+  const int s = seg_find(sb, blockIdx.x);
+  if (P[s].stop && *P[s].stop) return;
+  const int b = (int)((blockIdx.x - sb.start[s]) * 256 + threadIdx.x);
+  if (b < P[s].bs.nbnd) bnd_state(P[s].bs, b);
+}
+__global__ void __launch_bounds__(kRowBlock) k_seg_rows(const SegProblem *__restrict__ P, SegBlocks sb) {
+  const int s = seg_find(sb, blockIdx.x);
+  if (P[s].stop && *P[s].stop) return;
+  const int64_t r = (blockIdx.x - sb.start[s]) * kRowBlock + threadIdx.x;
+  if (r < P[s].ra.nrows) flux_row(P[s].ra, r);
+}
+__global__ void __launch_bounds__(kGatherBlock) k_seg_cell_gather(const SegProblem *__restrict__ P, SegBlocks sb) {
+  const int s = seg_find(sb, blockIdx.x);
+  if (P[s].stop && *P[s].stop) return;
+  const int c = (int)((blockIdx.x - sb.start[s]) * kGatherBlock + threadIdx.x);
+  if (c < P[s].ga.nc) cell_gather(P[s].ga, c);
+}
+__global__ void __launch_bounds__(kCellJacBlock) k_seg_cell_jac(const SegProblem *__restrict__ P, SegBlocks sb) {
+  const int s = seg_find(sb, blockIdx.x);
+  if (P[s].stop && *P[s].stop) return;
+  cell_jac_block<false>(P[s].ra, P[s].ga, P[s].cj, blockIdx.x - sb.start[s]);
 }
 
 // ELL SpMV over the slot-major layout: slot s of row i at s*n + i.

@@ -986,7 +986,9 @@ static void dot_to(tdyhip_ctx *ctx, const double *x, const double *y, int slot, 
 
 constexpr double kMaxStep = 1e8, kBTAlpha = 1e-4;  // PETSc 3.17 SNESLINESEARCHBT defaults
 
-static int newton_tail(tdyhip_ctx *ctx, const Form &fm, double *X, hipStream_t st) {
+// The tail in three parts: up to the full step W = X + Y, the residual at W
+// (ws.G, skipped when the tail is), and from its norm on.
+static void newton_tail_pre(tdyhip_ctx *ctx, double *X, hipStream_t st) {
   SolverWS &ws = *ctx->ws;
   NewtonDev *nd = ws.nd;
   const int64_t n = ctx->nc;
@@ -999,14 +1001,27 @@ static int newton_tail(tdyhip_ctx *ctx, const Form &fm, double *X, hipStream_t s
   dot_to(ctx, ws.F, ws.JY, 1, &nd->skip, st);
   k_nt_slope<<<1, 1, 0, st>>>(nd, ws.nh + 1);
   KT(ctx, st, "k_waxpy", (double)n * 24, k_waxpy<<<g, 256, 0, st>>>(n, 1.0, X, ws.Y, ws.W, &nd->skip));
-  int rc = eval_F(ctx, fm, ws.W, ws.G, st, &nd->skip);
-  if (rc) return rc;
+}
+
+static void newton_tail_post(tdyhip_ctx *ctx, double *X, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  NewtonDev *nd = ws.nd;
+  const int64_t n = ctx->nc;
+  const unsigned g = nblk(n, 256);
   dot_to(ctx, ws.G, ws.G, 2, &nd->skip, st);
   k_nt_accept<<<1, 1, 0, st>>>(nd, ws.nh + 2, kBTAlpha);
   k_copy_gated<<<g, 256, 0, st>>>(n, ws.W, X, &nd->not_accepted);
   k_copy_gated<<<g, 256, 0, st>>>(n, ws.G, ws.F, &nd->not_accepted);
   dot_to(ctx, X, X, 3, nullptr, st);
   k_nt_final<<<1, 1, 0, st>>>(nd, ws.nh + 3, ws.gdev, ws.status);
+}
+
+static int newton_tail(tdyhip_ctx *ctx, const Form &fm, double *X, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  newton_tail_pre(ctx, X, st);
+  int rc = eval_F(ctx, fm, ws.W, ws.G, st, &ws.nd->skip);
+  if (rc) return rc;
+  newton_tail_post(ctx, X, st);
   return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }
 

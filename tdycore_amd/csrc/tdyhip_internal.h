@@ -136,6 +136,7 @@ struct tdyhip_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::atomic<int> cancel{0};  // set by tdyhip_cancel from any thread
+  hipEvent_t ws_event = nullptr;  // marks the stream's progress for a serving batch's fused launches
 
   double vg_m, vg_alpha, x0, x1, x2, dx, gravity, pref;
   int density_type, gmatrix;
