@@ -151,3 +151,39 @@ def test_context_solves_again_after_a_cancel():
     g2.close()
     assert st1["status"] == 0 and st2["status"] == 0
     assert np.array_equal(U1, U2)
+
+
+def test_fused_server_equals_unfused():
+    """The fused engine (one segmented launch per evaluation over every
+    request on the device) serves the same pressures bit for bit as the
+    one-context-per-slot server, with several requests in the batch at once."""
+    print("[case] fused server vs slots", flush=True)
+    srv = serving.EnsembleServer(slots=4, build_workers=4)
+    ref = {f.result(timeout=120)["rid"]: f.result() for f in [srv.submit(r) for r in REQS]}
+    srv.close()
+    srv = serving.EnsembleServer(slots=8, build_workers=4, fused=True)
+    futs = [srv.submit(r) for r in REQS]
+    got = {f.result(timeout=120)["rid"]: f.result() for f in futs}
+    srv.close()
+    assert srv.peak_inflight >= 2
+    for r in REQS:
+        assert got[r.rid]["stats"]["status"] == 0
+        assert np.array_equal(got[r.rid]["U"], ref[r.rid]["U"]), r
+        for k in ("newton_its", "linear_its", "steps"):
+            assert got[r.rid]["stats"][k] == ref[r.rid]["stats"][k], (k, r)
+
+
+def test_fused_server_deadline_cancels_in_the_batch():
+    """A request past the deadline is cancelled inside the batch (its future
+    fails with DeadlineExceeded, carrying its latency); a short one beside it
+    is served."""
+    print("[case] fused server deadline", flush=True)
+    long_r = serving.Request(rid=100, nx=64, ny=64, nz=32, seed=9100, sigma=1.4, steps=400)
+    short_r = REQS[0]
+    srv = serving.EnsembleServer(slots=4, build_workers=2, fused=True, deadline_s=1.0)
+    f_long, f_short = srv.submit(long_r), srv.submit(short_r)
+    assert f_short.result(timeout=120)["stats"]["status"] == 0
+    with pytest.raises(serving.DeadlineExceeded) as ei:
+        f_long.result(timeout=120)
+    srv.close()
+    assert ei.value.rid == 100 and ei.value.latency_s >= 1.0
