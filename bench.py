@@ -40,7 +40,15 @@ carries:
   cfg0_latency  the 16^3 jittered cube: residual, Jacobian and per-step
                 latency (cache-resident, so latency rather than bandwidth);
   cpu_baseline  the oracle (C restatement of the reference algorithm) timed on
-                this host's cores.
+                this host's cores;
+  cfg2_prisms   the 4M-prism configuration (200,000 random Delaunay triangles
+                x 20 layers): its one-launch region setup and Newton cells/s
+                over three 1 h steps (bench.py --prisms runs it in full);
+  serving_sample  a bounded sample of the serving ensemble (the first
+                --serve-sample requests, Poisson arrivals): solves/s and
+                p50/p99 latency of the server with the evaluation fuser
+                (bench.py --serve runs the full ensemble, --fused 0 without
+                fusion).
 """
 import argparse
 import json
@@ -78,9 +86,16 @@ def parse():
                     help="prisms: setups run (the first tunes the region launch; the rest are timed)")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving ensemble instead (Poisson arrivals of box requests)")
+    ap.add_argument("--fused", type=int, default=1,
+                    help="serving: fused batch engine (1: one segmented launch per evaluation over all requests "
+                         "on the device) or one context per slot (0)")
+    ap.add_argument("--no-prisms", action="store_true", help="default run: skip the prism-mesh (cfg2) figures")
+    ap.add_argument("--no-serve", action="store_true", help="default run: skip the bounded serving sample")
+    ap.add_argument("--serve-sample", type=int, default=24,
+                    help="default run: requests of the bounded serving sample (fused server)")
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: Poisson arrival rate (requests/s)")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests on the device")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent requests on the device")
     ap.add_argument("--build-workers", type=int, default=12, help="serving: host threads building problems")
     ap.add_argument("--pc-coarse", type=int, default=1, help="coarse correction on block Jacobi (1) or not (0)")
     ap.add_argument("--serve-fp32", type=int, default=0, help="serving: FP32 Krylov (1) or FP64 (0)")
@@ -418,19 +433,20 @@ def prisms(a):
                                           f"line search) of step 1 on a 20,000-triangle x {a.prism_layers}-layer prism "
                                           f"mesh ({ncs} cells): {el:.2f} s",
                                 "setup_regions_per_s": S["mesh"].nv / el_setup}
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def serve(a):
+def serve(a, n_requests=None, fused=None):
     """Serving ensemble: requests arrive as a Poisson stream and are admitted by
     HBM footprint into a fixed number of device slots."""
     import numpy as np
     import torch
     from tdycore_amd import serving, workloads
     torch.cuda.init()
-    specs = workloads.ensemble_request_specs(a.requests)
+    fused = a.fused if fused is None else fused
+    specs = workloads.ensemble_request_specs(n_requests or a.requests)
     srv = serving.EnsembleServer(slots=a.slots, build_workers=a.build_workers, pc_coarse=bool(a.pc_coarse),
-                                 krylov_fp32=bool(a.serve_fp32), deadline_s=a.deadline or None)
+                                 krylov_fp32=bool(a.serve_fp32), deadline_s=a.deadline or None, fused=bool(fused))
     # warm the library and the device once (not timed)
     srv.submit(serving.Request(rid=-1, nx=16, ny=16, nz=8, seed=1, sigma=1.0, steps=1)).result()
     rng = np.random.default_rng(20261207)
@@ -502,7 +518,11 @@ def serve(a):
         "data": "synthetic (seeded jittered boxes, rotated log-normal permeability)",
         "config": {"workload": f"{len(res)} requests, Poisson arrivals at {a.rate:g}/s, nx,ny~U{{16..96}}, "
                                f"nz~U{{8..48}}, sigma~U(0.5,1.5), 5 backward-Euler steps of 1 h, FP64 residual, "
-                               f"{'FP32' if a.serve_fp32 else 'FP64'} Krylov, {a.slots} device slots",
+                               f"{'FP32' if a.serve_fp32 else 'FP64'} Krylov, {a.slots} device slots, "
+                               + ("evaluation fuser (the SNES residual and Jacobian evaluations pending at once "
+                                  "across slots run as one segmented launch per kernel)" if fused
+                                  else "one context per slot, no fusion"),
+                   "fused": bool(fused),
                    "total_cells": cells, "newton_its": newton, "failed": bad, "failures": failed[:20],
                    "timed_out": timed_out, "deadline_s": a.deadline or None,
                    "not_served": len(dropped), "served": len(res), "max_seconds": a.max_seconds,
@@ -511,8 +531,9 @@ def serve(a):
                       "p99_latency_s": float(np.percentile(lat_all, 99))},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"], "serving": summ,
         "peak_inflight": srv.peak_inflight, "peak_admitted_bytes": srv.peak_used, "hbm_budget": srv.hbm_budget,
+        "fuser": getattr(srv, "fuser_stats", None),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -523,9 +544,11 @@ def main():
               "the path is measured on one GPU's full mesh", file=sys.stderr)
         sys.exit(2)
     if a.serve:
-        return serve(a)
+        print(json.dumps(serve(a)), flush=True)
+        return
     if a.prisms:
-        return prisms(a)
+        print(json.dumps(prisms(a)), flush=True)
+        return
     import numpy as np
     import torch
     from tdycore_amd import workloads
@@ -635,6 +658,29 @@ def main():
         cpu = cpu_baseline(P, a.dt)
         log(f"cpu baseline {cpu['value']:.3e} Newton cells/s")
 
+    # the 4M-prism configuration: two setups (the first tunes the region
+    # launch) and three backward-Euler steps of 1 h
+    cfg2 = None
+    if not a.no_prisms:
+        import argparse as _ap
+        pa = _ap.Namespace(**vars(a))
+        pa.prism_setups, pa.prism_steps, pa.no_cpu_baseline = 2, 3, True
+        pl = prisms(pa)
+        cfg2 = {k: pl[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline", "phases_ms",
+                                   "setup", "newton")}
+        log(f"cfg2 prisms: {cfg2['value']:.3e} Newton cells/s")
+    # a bounded serving sample: the first requests of the ensemble, fused
+    # batch engine and one context per slot on the same arrivals
+    srv_sample = None
+    if not a.no_serve:
+        sf = serve(a, n_requests=a.serve_sample, fused=1)
+        srv_sample = {k: sf[k] for k in ("metric", "value", "unit", "p50_latency_s", "p99_latency_s", "responses",
+                                          "serving", "peak_inflight", "fuser")}
+        srv_sample["config"] = {k: sf["config"][k] for k in ("workload", "served", "failed", "timed_out",
+                                                              "not_served", "total_cells", "newton_its")}
+        log(f"serving sample: {sf['value']:.2f} solves/s, p50 {sf['p50_latency_s']:.2f} s, "
+            f"p99 {sf['p99_latency_s']:.2f} s, fuser {sf['fuser']}")
+
     line = {
         "metric": "MPFA-O Newton cells/s at 1M hex (FP64)", "value": value, "unit": "Newton cells/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps,
@@ -649,6 +695,7 @@ def main():
                    "setup_s": round(setup_s, 3), "headline_kernel_timers": False, "hip_graphs": True},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "timer_pass": timer_pass, "setup": setup,
         "fp32_krylov": fp32, "one_level_fp64": one_level, "cfg0_latency": small,
+        "cfg2_prisms": cfg2, "serving_sample": srv_sample,
     }
     print(json.dumps(line), flush=True)
 

@@ -218,6 +218,24 @@ int tdyhip_cancel(tdyhip_ctx *ctx, int32_t on);
  * last error message is that of the last failed request).  The caller keeps ownership of the
  * contexts and states and must not use them while they are in the batch; cancellation
  * (tdyhip_cancel) ends a request at its next round with TDYHIP_ERR_CANCELLED. */
+/* Evaluation fuser.  No counterpart in the reference: it is for servers solving many requests at
+ * once, each with its own thread and solver (tdyhip_run_tdyti and the other drivers).  The
+ * SNES-form residual and Jacobian evaluations of every context attached to a fuser are launched
+ * by the fuser's thread, which runs all evaluations pending when it wakes as one segmented launch
+ * per kernel; each context keeps its own pace and its results are the same bit for bit as
+ * without the fuser.  Attached contexts run their Newton iterations without whole-iteration
+ * graphs.  Only hexahedral meshes within the fixed-size limits can be attached; NULL detaches.
+ * Destroy a fuser only after every context attached to it is detached or destroyed.
+ * stats[3] = fused launch groups, evaluations, largest group. */
+typedef struct tdyhip_fuser tdyhip_fuser;
+int tdyhip_fuser_create(tdyhip_fuser **out);
+int tdyhip_fuser_destroy(tdyhip_fuser *fz);
+int tdyhip_set_fuser(tdyhip_ctx *ctx, tdyhip_fuser *fz);
+int tdyhip_fuser_stats(const tdyhip_fuser *fz, int64_t *stats /* 3 */);
+/* The first pending evaluation waits up to `us` microseconds (default 0) for others, or until
+ * every attached context has one pending, before the group is launched. */
+int tdyhip_fuser_set_window(tdyhip_fuser *fz, int32_t us);
+
 typedef struct tdyhip_batch tdyhip_batch;
 int tdyhip_batch_create(tdyhip_batch **out);
 int tdyhip_batch_destroy(tdyhip_batch *b);

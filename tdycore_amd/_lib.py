@@ -25,7 +25,8 @@ EXPORTS = [
     "tdyhip_device_bytes", "tdyhip_diagnostics", "tdyhip_get_stream", "tdyhip_cancel",
     "tdyhip_setup_info", "tdyhip_setup_plan", "tdyhip_recover_velocity", "tdyhip_set_source_sink",
     "tdyhip_ifunction_noalloc", "tdyhip_batch_create", "tdyhip_batch_destroy", "tdyhip_batch_add",
-    "tdyhip_batch_remove", "tdyhip_batch_size", "tdyhip_batch_round",
+    "tdyhip_batch_remove", "tdyhip_batch_size", "tdyhip_batch_round", "tdyhip_fuser_create",
+    "tdyhip_fuser_destroy", "tdyhip_set_fuser", "tdyhip_fuser_stats", "tdyhip_fuser_set_window",
 ]
 
 
@@ -107,6 +108,8 @@ def load():
         "tdyhip_batch_create": [P], "tdyhip_batch_destroy": [P],
         "tdyhip_batch_add": [P, P, P, P, D, D, D, D, P], "tdyhip_batch_remove": [P, C.c_int32],
         "tdyhip_batch_size": [P, P], "tdyhip_batch_round": [P, C.c_int32, P, P, P, P],
+        "tdyhip_fuser_create": [P], "tdyhip_fuser_destroy": [P], "tdyhip_set_fuser": [P, P],
+        "tdyhip_fuser_stats": [P, P], "tdyhip_fuser_set_window": [P, C.c_int32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

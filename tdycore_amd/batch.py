@@ -1,4 +1,11 @@
-"""Fused serving batch over the C ABI (tdyhip_batch_*).
+"""Fused evaluation of many requests over the C ABI.
+
+``Fuser`` (tdyhip_fuser_*): contexts attached to it keep their own solver
+threads and pace; their SNES-form residual and Jacobian evaluations are
+launched by the fuser's thread, all evaluations pending at once as one
+segmented launch per kernel.  This is what the ensemble server uses.
+
+``FusedBatch`` (tdyhip_batch_*), lockstep:
 
 Several requests - MPFAO contexts of hexahedral meshes with their block-Jacobi
 partitions set, device states and time-stepping parameters - advance in
@@ -15,6 +22,41 @@ import ctypes as C
 from . import _lib
 
 MAX_REQUESTS = 32
+
+
+class Fuser:
+    def __init__(self, window_us: int = 0):
+        """window_us: how long the first pending evaluation waits for others
+        (or until every attached context has one pending)."""
+        h = C.c_void_p()
+        _lib.check(_lib.load().tdyhip_fuser_create(C.byref(h)))
+        self._h = h
+        if window_us:
+            _lib.check(_lib.load().tdyhip_fuser_set_window(self._h, int(window_us)))
+
+    def attach(self, g):
+        _lib.check(_lib.load().tdyhip_set_fuser(g._h, self._h))
+
+    @staticmethod
+    def detach(g):
+        _lib.check(_lib.load().tdyhip_set_fuser(g._h, None))
+
+    def stats(self):
+        """{'launches': fused launch groups, 'evaluations', 'largest_group'}."""
+        a = (C.c_int64 * 3)()
+        _lib.check(_lib.load().tdyhip_fuser_stats(self._h, a))
+        return {"launches": int(a[0]), "evaluations": int(a[1]), "largest_group": int(a[2])}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.check(_lib.load().tdyhip_fuser_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FusedBatch:

@@ -304,6 +304,7 @@ int tdyhip_batch_add(tdyhip_batch *b, tdyhip_ctx *ctx, double *U, const tdyhip_s
   if (!b || !ctx || !U || !o || !id) { set_error("null argument"); return TDYHIP_ERR_ARG; }
   if (!(dt_init > 0) || !(growth > 0) || !(dt_max > 0) || !(final_time > 0)) { set_error("bad time stepping"); return TDYHIP_ERR_ARG; }
   if (ctx->general) { set_error("a batch takes hexahedral meshes within the fixed-size limits"); return TDYHIP_ERR_ARG; }
+  if (ctx->fuser) { set_error("the context is attached to an evaluation fuser"); return TDYHIP_ERR_ARG; }
   if ((int)b->members.size() >= kMaxSeg) { set_error("batch is full (32 requests)"); return TDYHIP_ERR_ARG; }
   for (BatchMember *m : b->members)
     if (m->ctx == ctx) { set_error("context already in the batch"); return TDYHIP_ERR_ARG; }

@@ -4,7 +4,9 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -1025,6 +1027,7 @@ int tdyhip_create(const tdyhip_mesh *m, const tdyhip_materials *mat, tdyhip_ctx 
 
 int tdyhip_destroy(tdyhip_ctx *ctx) {
   if (!ctx) return 0;
+  if (ctx->fuser) (void)tdyhip_set_fuser(ctx, nullptr);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   tdyhip::ws_destroy(ctx->ws);
   ctx->ws = nullptr;
@@ -1312,6 +1315,7 @@ int tdyhip_gravdis(const tdyhip_ctx *ctx, double *out) {
 
 #include "solver.hip"
 #include "batch.hip"
+#include "fuser.hip"
 
 namespace tdyhip {
 static void ws_destroy(SolverWS *ws) {

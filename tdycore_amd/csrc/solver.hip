@@ -858,10 +858,15 @@ __global__ void k_udot(int64_t n, const double *__restrict__ dyn, const double *
   if (i < n) Ud[i] = (U[i] - Up[i]) * dyn[1];
 }
 
+static int fuser_eval(tdyhip_ctx *ctx, int kind, const double *X, double *out, const int *stop);
+
 // The solver's residual and Jacobian launches read dtime and shift from
-// ws.dyn (so a replayed graph follows dt) and honour the given stop flag.
+// ws.dyn (so a replayed graph follows dt) and honour the given stop flag.  A
+// context attached to an evaluation fuser hands its SNES-form evaluations to
+// the fuser's fused launches instead.
 static int eval_F(tdyhip_ctx *ctx, const Form &fm, const double *X, double *F, hipStream_t st,
                   const int *stop = nullptr) {
+  if (ctx->fuser && fm.type == 1) return fuser_eval(ctx, 0, X, F, stop);
   SolverWS &ws = *ctx->ws;
   ctx->launch_dyn = ws.dyn;
   ctx->launch_stop = stop;
@@ -880,6 +885,7 @@ static int eval_F(tdyhip_ctx *ctx, const Form &fm, const double *X, double *F, h
 }
 
 static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, hipStream_t st) {
+  if (ctx->fuser && fm.type == 1) return fuser_eval(ctx, 1, X, J, nullptr);
   SolverWS &ws = *ctx->ws;
   ctx->launch_dyn = ws.dyn;
   int rc;
@@ -1091,7 +1097,9 @@ static int newton_iteration(tdyhip_ctx *ctx, const Form &fm, double *X, const td
   const int fp32 = o.krylov_fp32 ? 1 : 0;
   tdyhip_solver_opts ok = o;
   if (fp32) ok.ksp_max_it = std::min(o.ksp_max_it, kFp32MaxIt);
-  const bool graphs = o.hip_graphs && !ctx->kt.on;
+  // (a context on an evaluation fuser has its evaluations launched by the
+  // fuser's thread, outside any graph of its own)
+  const bool graphs = o.hip_graphs && !ctx->kt.on && !ctx->fuser;
   int rc;
   if (graphs) {
     // Jacobian, preparation, the cycles the last iteration needed and the

@@ -130,6 +130,8 @@ struct KernelTimers {
 
 }  // namespace tdyhip
 
+struct tdyhip_fuser;
+
 struct tdyhip_ctx {
   int nc = 0, nf = 0, nv = 0, nbnd = 0;
   int64_t jnnz = 0;
@@ -137,6 +139,7 @@ struct tdyhip_ctx {
   bool own_stream = false;
   std::atomic<int> cancel{0};  // set by tdyhip_cancel from any thread
   hipEvent_t ws_event = nullptr;  // marks the stream's progress for a serving batch's fused launches
+  tdyhip_fuser *fuser = nullptr;  // evaluation fuser the SNES-form evaluations go through, if any
 
   double vg_m, vg_alpha, x0, x1, x2, dx, gravity, pref;
   int density_type, gmatrix;

@@ -24,14 +24,15 @@ Results are bit-for-bit those of the same request run alone: every kernel of
 the path (residual and Jacobian gathers, Krylov reductions) sums in a fixed
 order, whatever else runs on the device.
 
-With ``fused=True`` the solve stage is one engine thread driving a fused
-batch (``batch.FusedBatch``, the library's tdyhip_batch_*): setup threads
-build each admitted request's context on its own stream and hand it to the
-engine, which adds it between rounds; each round advances every request on
-the device by one Newton iteration, with the state, residual and Jacobian
-evaluations of all of them in one segmented launch each and their Krylov
-solves side by side.  A request leaves the batch at its final time.  Results
-are still bit for bit those of the request alone.
+With ``fused=True`` every context is attached to one evaluation fuser
+(``batch.Fuser``, the library's tdyhip_fuser_*): each slot thread still runs
+its own request's solver at its own pace, and the SNES residual and Jacobian
+evaluations of all slots that are pending at once are launched together, one
+segmented launch per kernel.  Results are still bit for bit those of the
+request alone.  (The library's lockstep batch, tdyhip_batch_*, advances all
+requests one Newton iteration per round; on requests of very different
+Krylov costs every round waits for the slowest, so the server does not use
+it.)
 
 A request may hold a slot for at most ``deadline_s`` seconds of solving: past
 that a watchdog cancels its context (the solver stops at its next Newton
@@ -147,11 +148,12 @@ class EnsembleServer:
     def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, build_workers: int = 8,
                  krylov_fp32: bool = True, snes_rtol: float = 1e-10, brick=(8, 8, 4),
                  solve_fn: Optional[Callable] = None, pc_coarse: bool = True,
-                 deadline_s: Optional[float] = None, fused: bool = False, setup_workers: int = 2):
+                 deadline_s: Optional[float] = None, fused: bool = False, fuse_window_us: int = 200):
         self.fused = fused
-        if fused:
-            from .batch import MAX_REQUESTS
-            slots = min(slots, MAX_REQUESTS)
+        self.fuser = None
+        if fused and solve_fn is None:
+            from .batch import Fuser
+            self.fuser = Fuser(fuse_window_us)
         self.slots = slots
         self.deadline_s = deadline_s
         self.pc_coarse = pc_coarse
@@ -173,18 +175,9 @@ class EnsembleServer:
         self.peak_inflight = 0
         self._closed = False
         self._draining = False
-        self._running = {}  # slot (fused: batch id) -> job on the device
-        if fused:
-            # setup threads hand contexts to the engine; admission caps the
-            # requests on the device at `slots`
-            self._join: "queue.Queue" = queue.Queue()
-            self._workers = [threading.Thread(target=self._setup, daemon=True, name=f"setup{i}")
-                             for i in range(setup_workers)]
-            self._engine_thread = threading.Thread(target=self._engine, daemon=True, name="engine")
-            self._engine_thread.start()
-        else:
-            self._workers = [threading.Thread(target=self._slot, args=(i,), daemon=True, name=f"slot{i}")
-                             for i in range(slots)]
+        self._running = {}  # slot -> job on the device
+        self._workers = [threading.Thread(target=self._slot, args=(i,), daemon=True, name=f"slot{i}")
+                         for i in range(slots)]
         for w in self._workers:
             w.start()
         self._watch_stop = threading.Event()
@@ -234,12 +227,13 @@ class EnsembleServer:
             self._ready.put(None)
         for w in self._workers:
             w.join()
-        if self.fused:
-            self._join.put(None)
-            self._engine_thread.join()
         self._watch_stop.set()
         if self._watch is not None:
             self._watch.join()
+        if self.fuser is not None:
+            self.fuser_stats = self.fuser.stats()
+            self.fuser.close()
+            self.fuser = None
 
     def _watchdog(self):
         while not self._watch_stop.wait(0.05):
@@ -271,7 +265,7 @@ class EnsembleServer:
 
     def _admit(self, job: _Job):
         with self._cv:
-            while self._used + job.need > self.hbm_budget or (self.fused and self._inflight >= self.slots):
+            while self._used + job.need > self.hbm_budget:
                 self._cv.wait()
             self._used += job.need
             self._inflight += 1
@@ -318,118 +312,6 @@ class EnsembleServer:
                 job.problem = None
                 self._release(job)
 
-    # ---- fused mode ----
-    def _options(self, g):
-        bs = self.brick[0] * self.brick[1] * self.brick[2]
-        return g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs,
-                                pc_coarse=1 if self.pc_coarse else 0)
-
-    def _setup(self):
-        """Admit the next built request and set it up on its own stream."""
-        import torch
-        from .mpfao import MPFAO
-        while True:
-            job = self._ready.get()
-            if job is None:
-                return
-            if self._draining:
-                job.future.cancel()
-                continue
-            self._admit(job)
-            g = None
-            try:
-                P = job.problem
-                g = MPFAO(P["mesh"], P["face_bc"], P["K"], P["porosity"], P["Sr"], **P["kw"])
-                job.handle.attach(g.cancel)
-                g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
-                with torch.cuda.stream(g.stream):
-                    U = torch.tensor(P["U0"], device="cuda", dtype=torch.float64)
-                    torch.cuda.current_stream().synchronize()
-                self._join.put((job, g, U, self._options(g)))
-            except BaseException as e:  # noqa: BLE001
-                if g is not None:
-                    job.handle.attach(None)
-                    g.close()
-                self._fail(job, e)
-
-    def _fail(self, job: _Job, e: BaseException):
-        job.t_done = time.perf_counter()
-        if job.handle.cancelled and not isinstance(e, DeadlineExceeded):
-            e = DeadlineExceeded(f"request {job.req.rid} cancelled after "
-                                 f"{job.t_done - (job.t_admit or job.t_done):.1f} s of solving ({e})")
-        e.rid, e.t_arrive, e.t_done = job.req.rid, job.t_arrive, job.t_done
-        e.latency_s = job.t_done - job.t_arrive
-        job.future.set_exception(e)
-        job.problem = None
-        self._release(job)
-
-    def _engine(self):
-        """Run the fused batch: add arrivals between rounds, one round at a
-        time while any request is on the device."""
-        from . import _lib
-        from .batch import FusedBatch
-        b = FusedBatch()
-        active = {}  # batch id -> (job, context, state)
-        closing = False
-        try:
-            while True:
-                items = []
-                if not active and not closing:
-                    items.append(self._join.get())
-                while True:
-                    try:
-                        items.append(self._join.get_nowait())
-                    except queue.Empty:
-                        break
-                for it in items:
-                    if it is None:
-                        closing = True
-                        continue
-                    job, g, U, o = it
-                    r = job.req
-                    try:
-                        rid = b.add(g, U, o, r.dt, 1.25, r.dt, r.steps * r.dt)
-                    except BaseException as e:  # noqa: BLE001
-                        job.handle.attach(None)
-                        g.close()
-                        self._fail(job, e)
-                        continue
-                    active[rid] = (job, g, U)
-                    with self._cv:
-                        self._running[rid] = job
-                if not active:
-                    if closing and self._join.empty():
-                        return
-                    continue
-                for rid, rc, st in b.round():
-                    job, g, U = active.pop(rid)
-                    with self._cv:
-                        self._running.pop(rid, None)
-                    job.handle.attach(None)
-                    try:
-                        if rc == 0:
-                            out = U.cpu().numpy()
-                            st["device_bytes"] = g.device_bytes()
-                            job.t_done = time.perf_counter()
-                            job.future.set_result({
-                                "rid": job.req.rid, "nc": job.problem["mesh"].nc, "U": out, "stats": st,
-                                "latency_s": job.t_done - job.t_arrive, "build_s": job.t_built - job.t_arrive,
-                                "queue_s": job.t_admit - job.t_built, "solve_s": job.t_done - job.t_admit,
-                                "t_arrive": job.t_arrive, "t_done": job.t_done, "slot": rid})
-                            job.problem = None
-                            self._release(job)
-                        else:
-                            self._fail(job, _lib.TdyHipError(f"request {job.req.rid} failed (error {rc})", rc))
-                    finally:
-                        del U
-                        g.close()
-        finally:
-            for job, g, U in active.values():
-                job.handle.attach(None)
-                g.close()
-                self._fail(job, RuntimeError("serving engine stopped"))
-            b.close()
-
     def _gpu_solve(self, P, req: Request, handle: Optional[SolveHandle] = None):
         import torch
         from .mpfao import MPFAO
@@ -437,6 +319,8 @@ class EnsembleServer:
         if handle is not None:
             handle.attach(g.cancel)
         try:
+            if self.fuser is not None:
+                self.fuser.attach(g)
             g.set_pc_partition(brick_partition(P["mesh"], *self.brick))
             bs = self.brick[0] * self.brick[1] * self.brick[2]
             o = g.solver_options(snes_rtol=self.snes_rtol, krylov_fp32=1 if self.krylov_fp32 else 0, pc_block=bs,
@@ -455,6 +339,8 @@ class EnsembleServer:
         finally:
             if handle is not None:
                 handle.attach(None)
+            if self.fuser is not None:
+                self.fuser.detach(g)
             g.close()
 
 

@@ -124,3 +124,48 @@ def test_batch_cancel_leaves_others_untouched(reqs):
     assert res[i1][0] == _lib.ERR_CANCELLED
     assert res[i0][0] == 0
     assert np.array_equal(U0.cpu().numpy(), ref[0])
+
+
+def test_fuser_threads_equal_alone(reqs):
+    """Requests solving in their own threads (run_tdyti) with an evaluation
+    fuser attached give the same pressures and statistics bit for bit as
+    without it, and evaluations that were pending together shared a launch."""
+    import threading
+    import torch
+    from tdycore_amd.batch import Fuser
+    print("[case] evaluation fuser, 4 threads", flush=True)
+    extra = _requests()[1]  # a second context of request 1's problem: four solves in flight
+    rs = list(reqs) + [extra]
+    ref = [_alone(r) for r in rs]
+    fz = Fuser(window_us=2000)
+    for r in rs:
+        fz.attach(r["g"])
+    out = [None] * len(rs)
+    go = threading.Barrier(len(rs))
+
+    def run(i):
+        r = rs[i]
+        with torch.cuda.stream(r["g"].stream):
+            U = torch.tensor(r["P"]["U0"], device="cuda", dtype=torch.float64)
+            torch.cuda.current_stream().synchronize()
+            go.wait()
+            st = r["g"].run_tdyti(U, HOUR, 1.25, HOUR, r["final"], r["o"])
+            out[i] = (U.cpu().numpy(), st)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(rs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for r in rs:
+        fz.detach(r["g"])
+    stats = fz.stats()
+    fz.close()
+    extra["g"].close()
+    print(f"  fuser: {stats}", flush=True)
+    for i in range(len(rs)):
+        assert out[i] is not None
+        assert np.array_equal(out[i][0], ref[i][0]), f"request {i} through the fuser"
+        assert all(out[i][1][k] == ref[i][1][k] for k in KEYS), (out[i][1], ref[i][1])
+    assert stats["evaluations"] >= sum(o[1]["newton_its"] for o in out)
+    assert stats["largest_group"] >= 2
