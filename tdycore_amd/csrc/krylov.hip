@@ -490,28 +490,44 @@ struct Prolong {
   int bs_log2 = 0;
 };
 
+// Two adjacent rows per lane, read as one 16-byte (FP64) or 8-byte (FP32)
+// access when the rows' addresses allow it (ldv and the row even); the last
+// odd row of a vector is read alone.
+template <class TV> struct Pair2;
+template <> struct Pair2<double> { using T = double2; };
+template <> struct Pair2<float> { using T = float2; };
+template <class TV>
+__device__ inline void load2(const TV *__restrict__ p, int64_t r, int64_t n, bool vec, TV &a, TV &b) {
+  if (vec && r + 1 < n) {
+    const typename Pair2<TV>::T q = *reinterpret_cast<const typename Pair2<TV>::T *>(p + r);
+    a = q.x;
+    b = q.y;
+  } else {
+    a = r < n ? p[r] : (TV)0;
+    b = r + 1 < n ? p[r + 1] : (TV)0;
+  }
+}
+
 // partial[j*nchunk + chunk] = <w, V_j> over the chunk, j = 0..nv-1.  Each
-// thread accumulates up to NS dots in registers; one reduce-scatter per
-// wavefront and one barrier per group of NS vectors finish the chunk.  With
-// pro.z1 set, w first receives the coarse prolongation (written back).
+// thread holds two adjacent rows and accumulates up to NS dots in registers;
+// one reduce-scatter per wavefront and one barrier per group of NS vectors
+// finish the chunk.  With pro.z1 set, w first receives the coarse
+// prolongation (written back).
 template <class TV, int NS>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                     TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
                                                     Prolong pro, const int *__restrict__ stop = nullptr) {
+  static_assert(Red<TV>::E == 2, "two rows per thread");
   __shared__ TV red[kRedBlock / 64][NS];
   if (stop && *stop) return;
   const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
-  constexpr int E = Red<TV>::E;
-  const int64_t i0 = (int64_t)blockIdx.x * Red<TV>::chunk + t;
-  TV wv[E];
-#pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int64_t i = i0 + (int64_t)e * kRedBlock;
-    wv[e] = i < n ? w[i] : (TV)0;
-    if (pro.z1 && i < n) {
-      wv[e] += (TV)pro.z1[pro.pos[i] >> pro.bs_log2];
-      w[i] = wv[e];
-    }
+  const int64_t r0 = (int64_t)blockIdx.x * Red<TV>::chunk + 2 * t;
+  const bool vec = (ldv & 1) == 0;
+  TV w0, w1;
+  load2(w, r0, n, true, w0, w1);
+  if (pro.z1) {
+    if (r0 < n) { w0 += (TV)pro.z1[pro.pos[r0] >> pro.bs_log2]; w[r0] = w0; }
+    if (r0 + 1 < n) { w1 += (TV)pro.z1[pro.pos[r0 + 1] >> pro.bs_log2]; w[r0 + 1] = w1; }
   }
   for (int j0 = 0; j0 < nv; j0 += NS) {
     TV acc[32];
@@ -519,12 +535,10 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
     for (int j = 0; j < 32; j++) {
       acc[j] = 0;
       if (j < NS && j0 + j < nv) {
-        const TV *vj = V + (int64_t)(j0 + j) * ldv;
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-          const int64_t i = i0 + (int64_t)e * kRedBlock;
-          if (i < n) acc[j] += wv[e] * vj[i];
-        }
+        TV v0, v1;
+        load2(V + (int64_t)(j0 + j) * ldv, r0, n, vec, v0, v1);
+        acc[j] += w0 * v0;
+        acc[j] += w1 * v1;
       }
     }
     const TV r = wave_reduce_scatter<NS>(acc);
@@ -553,7 +567,8 @@ __global__ void k_reduce_partials(int nv, int nchunk, const TV *__restrict__ par
   if (threadIdx.x == 0) out[j] = (double)s;
 }
 
-// w -= sum_j h[j] V_j ; partial[chunk] = ||w||^2 over the chunk
+// w -= sum_j h[j] V_j ; partial[chunk] = ||w||^2 over the chunk (two
+// adjacent rows per thread, as k_mdot)
 template <class TV>
 __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                           const double *__restrict__ h, TV *__restrict__ w,
@@ -561,17 +576,25 @@ __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, con
                                                           const int *__restrict__ stop = nullptr) {
   __shared__ TV sh[kRedBlock / 64];
   if (stop && *stop) return;
-  const int64_t i0 = (int64_t)blockIdx.x * Red<TV>::chunk + threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * Red<TV>::chunk + 2 * threadIdx.x;
+  const bool vec = (ldv & 1) == 0;
+  TV x0, x1;
+  load2(w, r0, n, true, x0, x1);
+  for (int j = 0; j < nv; j++) {
+    TV v0, v1;
+    load2(V + (int64_t)j * ldv, r0, n, vec, v0, v1);
+    const TV hj = (TV)h[j];
+    x0 -= hj * v0;
+    x1 -= hj * v1;
+  }
   TV s = 0;
-#pragma unroll
-  for (int e = 0; e < Red<TV>::E; e++) {
-    const int64_t i = i0 + (int64_t)e * kRedBlock;
-    if (i < n) {
-      TV x = w[i];
-      for (int j = 0; j < nv; j++) x -= (TV)h[j] * V[j * ldv + i];
-      w[i] = x;
-      s += x * x;
-    }
+  if (r0 + 1 < n) {
+    *reinterpret_cast<typename Pair2<TV>::T *>(w + r0) = typename Pair2<TV>::T{x0, x1};
+    s += x0 * x0;
+    s += x1 * x1;
+  } else if (r0 < n) {
+    w[r0] = x0;
+    s += x0 * x0;
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
