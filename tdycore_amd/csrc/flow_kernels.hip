@@ -812,9 +812,14 @@ __global__ void __launch_bounds__(kGatherBlock) k_seg_cell_gather(const SegProbl
   if (c < P[s].ga.nc) cell_gather(P[s].ga, c);
 }
 __global__ void __launch_bounds__(kCellJacBlock) k_seg_cell_jac(const SegProblem *__restrict__ P, SegBlocks sb) {
-  const int s = seg_find(sb, blockIdx.x);
+  // the segment is uniform over the workgroup: its arguments are copied out
+  // once (scalar loads) rather than read through the table at every use
+  const int s = __builtin_amdgcn_readfirstlane(seg_find(sb, blockIdx.x));
   if (P[s].stop && *P[s].stop) return;
-  cell_jac_block<false>(P[s].ra, P[s].ga, P[s].cj, blockIdx.x - sb.start[s]);
+  const RowArgs ra = P[s].ra;
+  const GatherArgs ga = P[s].ga;
+  const CellJacArgs cj = P[s].cj;
+  cell_jac_block<false>(ra, ga, cj, blockIdx.x - sb.start[s]);
 }
 
 // ELL SpMV over the slot-major layout: slot s of row i at s*n + i.
