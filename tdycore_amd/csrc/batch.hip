@@ -78,7 +78,7 @@ static int fused_eval(tdyhip_batch *b, int table, std::vector<BatchMember *> &ms
     p.cs = cell_state_args(ctx, 1, X, nullptr);
     p.bs = bnd_state_args(ctx, 1, X);
     p.ra = row_args(ctx, X);
-    p.ga = jac ? gather_args(ctx, 1, 0.0, nullptr, nullptr, ws.J, true, ws.d_perm, ws.NB)
+    p.ga = jac ? gather_args(ctx, 1, 0.0, nullptr, nullptr, ws.J, true, ws.d_perm, ws.bj.bs)
                : gather_args(ctx, 1, 0.0, nullptr, rof(ms[i]), nullptr, false, nullptr, 0);
     p.cj = cell_jac_args(ctx, ws.bj.d_row, ws.NB);
     p.stop = stopof(ms[i]);

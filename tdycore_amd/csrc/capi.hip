@@ -248,7 +248,7 @@ static RowArgs row_args(tdyhip_ctx *ctx, const double *U) {
 }
 
 static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const double *Udot, double *R, double *J,
-                              bool ell, const int32_t *perm, int64_t ell_stride) {
+                              bool ell, const int32_t *perm, int ell_bs) {
   GatherArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.form = form; a.shift = shift; a.dtime = ctx->dtime;
   a.cell_faces = ctx->d_cell_faces; a.face_cells = ctx->d_face_cells;
@@ -257,7 +257,7 @@ static GatherArgs gather_args(tdyhip_ctx *ctx, int form, double shift, const dou
   a.acc = ctx->d_acc; a.dM = ctx->d_dM; a.d2M = ctx->d_d2M;
   a.src = ctx->has_source ? ctx->d_source : nullptr;
   a.Udot = Udot; a.vol = ctx->d_cell_vol;
-  a.jrp = ell ? nullptr : ctx->d_jrp; a.jstride = ell ? ell_stride : 1; a.ell_w = ctx->jell_w;
+  a.jrp = ell ? nullptr : ctx->d_jrp; a.jbs = ell ? ell_bs : 1; a.ell_w = ctx->jell_w;
   a.perm = ell ? perm : nullptr;
   a.R = R; a.J = J;
   a.dyn = ctx->launch_dyn; a.stop = ctx->launch_stop;
@@ -300,20 +300,20 @@ static CellJacArgs cell_jac_args(tdyhip_ctx *ctx, const int32_t *order, int64_t 
 
 // Residual (R) and / or Jacobian (J) at U.  The residual alone: flux rows,
 // then the per-cell gather.  With J: one pass per cell (k_cell_jac).
-// ell = true: J is the solver's block-ordered slot-major ELL array (row c at
-// perm[c], slot stride ell_stride), else CSR values in the pattern order.
+// ell = true: J is the solver's block-contiguous ELL array (row c at position
+// perm[c] of blocks of ell_bs rows), else CSR values in the pattern order.
 // order (ELL only): the block layout's row list (position -> cell, -1
 // padding, norder entries); the Jacobian's threads follow it so that a
 // workgroup covers one block.  Every entry of J is written (no prior clear).
 static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
-                               double *R, double *J, hipStream_t st, const int32_t *perm, int64_t ell_stride);
+                               double *R, double *J, hipStream_t st, const int32_t *perm, int ell_bs);
 static int launch_flux(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot, double *R,
                        double *J, hipStream_t st, bool ell = false, const int32_t *perm = nullptr,
-                       int64_t ell_stride = 0, const int32_t *order = nullptr, int64_t norder = 0) {
-  if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st, ell ? perm : nullptr, ell_stride);
+                       int ell_bs = 0, const int32_t *order = nullptr, int64_t norder = 0) {
+  if (ctx->general) return launch_flux_general(ctx, form, shift, U, Udot, R, J, st, ell ? perm : nullptr, ell_bs);
   const int64_t nrows = (int64_t)4 * ctx->nf;
   RowArgs a = row_args(ctx, U);
-  GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell, perm, ell_stride);
+  GatherArgs ga = gather_args(ctx, form, shift, Udot, R, J, ell, perm, ell_bs);
   if (!J) {
     KT(ctx, st, "k_rows", bytes_rows(ctx), k_rows<<<nblk(nrows, kRowBlock), kRowBlock, 0, st>>>(a));
     KT(ctx, st, "k_cell_gather", bytes_gather(ctx),
@@ -663,7 +663,7 @@ static int setup_general(tdyhip_ctx *ctx, const tdyhip_mesh *m, const tdyhip_mat
 // vertex blocks, the residual by k_cell_gather, the Jacobian as CSR values in
 // the pattern order.
 static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const double *U, const double *Udot,
-                               double *R, double *J, hipStream_t st, const int32_t *perm, int64_t ell_stride) {
+                               double *R, double *J, hipStream_t st, const int32_t *perm, int ell_bs) {
   GenRowArgs a;
   a.nc = ctx->nc; a.nrows = (int64_t)4 * ctx->nf; a.pref = ctx->pref;
   a.face_cells = ctx->d_face_cells; a.face_flags = ctx->d_face_flags;
@@ -698,7 +698,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
   if (J) {
     GenGatherArgs gg;
     gg.nc = ctx->nc; gg.form = form; gg.shift = shift; gg.dtime = ctx->dtime;
-    gg.ell_w = ctx->jell_w; gg.perm = perm; gg.stride = perm ? ell_stride : 1;
+    gg.ell_w = ctx->jell_w; gg.perm = perm; gg.jbs = perm ? ell_bs : 1;
     gg.cell_faces = ctx->d_cell_faces; gg.face_cells = ctx->d_face_cells; gg.rtoff = ctx->d_rtoff;
     gg.rinfo = ctx->d_rinfo; gg.roc = ctx->d_roc; gg.ccell = ctx->d_ccell; gg.jrp = ctx->d_jrp;
     gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags; gg.gslot = ctx->d_gslot;

@@ -25,7 +25,7 @@ constexpr int kNG = 128;    // level-2 unknowns (dense inverse in LDS)
 struct CoarseLayout {
   int nb = 0, ng = 0, gsz = 0;    // blocks, groups, max blocks per group
   int32_t *nbr = nullptr;         // nb*kCQ neighbour block ids, ascending (-1 pad)
-  uint8_t *cq = nullptr;          // W*NB: position in nbr[block(p)] of slot k's column block (255 none)
+  uint8_t *cq = nullptr;          // W*NB, block-contiguous as the ELL: position in nbr[block(p)] of slot k's column block (255 none)
   int32_t *grp = nullptr;         // nb: group of each block
   int32_t *gmem = nullptr;        // ng*gsz: blocks of each group (-1 pad)
   double *A1 = nullptr;           // nb*kCQ: Galerkin rows (column = neighbour position)
@@ -42,7 +42,7 @@ template <int BS>
 __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, int W, const double *__restrict__ J) {
   __shared__ double sh[BS / 64];
   const int b = blockIdx.x, t = threadIdx.x;
-  const int64_t p = (int64_t)b * BS + t;
+  const int64_t e0 = ((int64_t)b * W) * BS + t;  // block-contiguous ELL: slot k at e0 + k*BS
   uint8_t q_of[32];
   double v_of[32];
 #pragma unroll
@@ -50,8 +50,8 @@ __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, 
     q_of[k] = 255;
     v_of[k] = 0.0;
     if (k < W) {
-      q_of[k] = C.cq[k * NB + p];
-      v_of[k] = q_of[k] != 255 ? J[k * NB + p] : 0.0;
+      q_of[k] = C.cq[e0 + k * BS];
+      v_of[k] = q_of[k] != 255 ? J[e0 + k * BS] : 0.0;
     }
   }
   for (int q = 0; q < kCQ; q++) {

@@ -349,7 +349,7 @@ constexpr int kGenJBlock = 64;
 constexpr int kGenJSlots = 64;
 struct GenGatherArgs {
   int nc, form, ell_w;
-  int64_t stride;        // distance between a row's slots: 1 (CSR) or the ELL row count
+  int jbs;               // ELL: block size of the block-contiguous layout (see jac_row_addr)
   const int32_t *perm;   // ELL: position of row c (nullptr: CSR at jrp[c])
   double shift, dtime;
   const int32_t *cell_faces, *face_cells, *rtoff, *rinfo, *roc, *ccell, *jrp, *jcols, *jdiag;
@@ -538,8 +538,13 @@ __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a
   const double dv = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
                                   : a.dM[c] * a.vol[c] * (1.0 / dtime);
   acc[a.jdiag[c]][t] += dv;
-  const int64_t st = a.stride;
-  double *row = a.J + (a.perm ? (int64_t)a.perm[c] : (int64_t)r0);
+  int64_t st = 1;
+  double *row = a.J + r0;
+  if (a.perm) {
+    const int64_t p = a.perm[c], bs = a.jbs;
+    row = a.J + (p / bs * a.ell_w) * bs + p % bs;
+    st = bs;
+  }
   const int nslot = a.perm ? a.ell_w : len;  // ELL rows are padded to ell_w slots with zeros
   for (int k = 0; k < nslot; k++) row[k * st] = k < len ? acc[k][t] : 0.0;
 }
@@ -555,7 +560,8 @@ struct GatherArgs {
   const double *src;   // source_sink per cell, or nullptr (TS form: scaled by the volume)
   const int32_t *jrp;  // CSR row starts, or nullptr for the slot-major ELL layout
   const int32_t *perm; // ELL: position of row c (nullptr: c)
-  int64_t jstride;     // distance between consecutive slots of a row: 1 (CSR) or the ELL row count
+  int jbs;             // ELL: block size bs of the block-contiguous layout (slot k of position p at
+                       // (p/bs*ell_w + k)*bs + p%bs)
   int ell_w;           // ELL width (slots written per row in the ELL layout)
   double *R, *J;
   const double *dyn;   // device dtime / shift (see StateParams), or nullptr
@@ -564,6 +570,15 @@ struct GatherArgs {
 
 constexpr int kGatherBlock = 128;
 constexpr int kJacSlots = 27;  // vertex-adjacency row length of a hexahedral mesh
+
+// Where cell c's Jacobian row starts and the distance between its slots: CSR
+// values at jrp[c] (stride 1), or the solver's block-contiguous ELL.
+__device__ inline void jac_row_addr(const GatherArgs &a, int c, int64_t &base, int64_t &stride) {
+  if (a.jrp) { base = a.jrp[c]; stride = 1; return; }
+  const int64_t p = a.perm ? a.perm[c] : c, bs = a.jbs;
+  base = (p / bs * a.ell_w) * bs + p % bs;
+  stride = bs;
+}
 
 // R[c] from the flux rows: the cell's faces' four rows summed in sub-face
 // order, +/- by orientation, then the accumulation and the source
@@ -747,9 +762,10 @@ __device__ inline void cell_jac_block(const RowArgs &ra, const GatherArgs &a, co
   }
   __syncthreads();
   if (c < 0) return;
-  const int64_t base = a.jrp ? a.jrp[c] : (a.perm ? a.perm[c] : c);
+  int64_t base = c, stride = 1;
+  jac_row_addr(a, c, base, stride);
   const int len = a.jrp ? a.jrp[c + 1] - a.jrp[c] : a.ell_w;
-  for (int sl = corner; sl < len; sl += 8) a.J[base + (int64_t)sl * a.jstride] = acc[sl][cl];
+  for (int sl = corner; sl < len; sl += 8) a.J[base + (int64_t)sl * stride] = acc[sl][cl];
 }
 
 template <bool WITH_R>

@@ -55,7 +55,7 @@ static int fuser_launch(tdyhip_fuser *fz, std::vector<tdyhip_fuser::Req *> &rs) 
     p.cs = cell_state_args(ctx, 1, X, nullptr);
     p.bs = bnd_state_args(ctx, 1, X);
     p.ra = row_args(ctx, X);
-    p.ga = jac ? gather_args(ctx, 1, 0.0, nullptr, nullptr, rs[i]->out, true, ws.d_perm, ws.NB)
+    p.ga = jac ? gather_args(ctx, 1, 0.0, nullptr, nullptr, rs[i]->out, true, ws.d_perm, ws.bj.bs)
                : gather_args(ctx, 1, 0.0, nullptr, rs[i]->out, nullptr, false, nullptr, 0);
     p.cj = cell_jac_args(ctx, ws.bj.d_row, ws.NB);
     p.stop = rs[i]->stop;

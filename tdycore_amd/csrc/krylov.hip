@@ -30,7 +30,8 @@ struct BJLayout {
   int nblocks = 0;
   int bs = 0;                  // stride / launch width (64, 256 or 512)
   int32_t *d_row = nullptr;    // nblocks*bs global row (-1 pad)
-  int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian less the block's first position (-1 pad)
+  int32_t *d_jpos = nullptr;   // nblocks*kW*bs position in the ELL Jacobian less the block's first entry (-1 pad)
+  int ell_w = 0;               // ELL width W: block b's entries start at b*W*bs
   uint16_t *d_lcol = nullptr;  // nblocks*kW*bs local column (0xffff pad)
   uint8_t *d_diag = nullptr;   // nblocks*bs slot of the diagonal
   uint8_t *d_nslot = nullptr;  // nblocks*bs number of slots
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   const int64_t sbase = (int64_t)b * kW * BS;
   for (int e = tid; e < kW * BS; e += NT) {
     const int32_t pos = L.d_jpos[ibase + e];  // relative to the block's first position
-    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos + (int64_t)b * BS] : (TV)0;
+    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos + (int64_t)b * L.ell_w * BS] : (TV)0;
     lc[e / BS][e % BS] = L.d_lcol[ibase + e];
   }
   for (int r = tid; r < BS; r += NT) {
@@ -260,9 +261,10 @@ __global__ void __launch_bounds__(BS) k_bj_apply(BJLayout L, const TV *__restric
 }
 
 // ---------------------------------------------------------------- block-ordered SpMV
-// The solver's Jacobian is stored slot-major over the block layout: slot k of
-// block position p at a[k*NB + p], its global column at ecol[k*NB + p]; the
-// global row of p is L.d_row[p] (-1 for padding positions).
+// The solver's Jacobian is stored block-contiguous over the block layout:
+// slot k of position p = b*bs + t at a[(b*W + k)*bs + t], its global column at
+// the same place of ecol (or of the block pattern's table, see ecol_addr);
+// the global row of p is L.d_row[p] (-1 for padding positions).
 
 // Column of slot k at block position p: an int32 global column, or an int16
 // offset from the row (used when every offset of the mesh fits, which halves
@@ -278,7 +280,7 @@ __device__ inline int blk_col(const TI *__restrict__ ecol, int64_t q, int row) {
 // Slot k's column index sits at ecol[eb + k*es]: eb = p, es = NB in the
 // slot-major layout; eb = (pattern*W)*bs + t, es = bs with shared patterns.
 template <class TV, class TI, bool UNROLL = (sizeof(TV) == 4)>
-__device__ inline TV blk_row(int64_t NB, int W, int64_t p, int row, const TI *__restrict__ ecol, int64_t eb,
+__device__ inline TV blk_row(int64_t ab, int64_t as, int W, int row, const TI *__restrict__ ecol, int64_t eb,
                              int64_t es, const TV *__restrict__ a, const TV *__restrict__ x) {
   TV s = 0;
   if (UNROLL && W == kW) {
@@ -287,27 +289,36 @@ __device__ inline TV blk_row(int64_t NB, int W, int64_t p, int row, const TI *__
 #pragma unroll
     for (int k = 0; k < kW; k++) {
       c[k] = blk_col(ecol, eb + k * es, row);
-      v[k] = a[k * NB + p];
+      v[k] = a[ab + k * as];
     }
 #pragma unroll
     for (int k = 0; k < kW; k++) s += v[k] * x[c[k]];
   } else {
-    for (int k = 0; k < W; k++) s += a[k * NB + p] * x[blk_col(ecol, eb + k * es, row)];
+    for (int k = 0; k < W; k++) s += a[ab + k * as] * x[blk_col(ecol, eb + k * es, row)];
   }
   return s;
 }
 
+// The solver's ELL arrays (Jacobian values, column indices, coarse slot
+// map) are block-contiguous: slot k of position p = b*bs + t (block b, row t
+// of the block) at (b*W + k)*bs + t, so a block's W slots are one contiguous
+// run of W*bs entries (one DRAM stream per workgroup instead of W streams
+// NB apart).  Index tables shared by the blocks of one pattern sit at
+// (pattern*W + k)*bs + t.
+__device__ inline int64_t ell_base(int64_t p, int bs, int W) {
+  const int64_t b = p / bs;
+  return (b * W) * bs + (p - b * bs);
+}
+
 // Base and stride of block position p's column indices (see blk_row).
-__device__ inline void ecol_addr(const int32_t *__restrict__ pat, int bs, int W, int64_t NB, int64_t p, int64_t &eb,
-                                 int64_t &es) {
+__device__ inline void ecol_addr(const int32_t *__restrict__ pat, int bs, int W, int64_t p, int64_t &eb, int64_t &es) {
   if (pat) {
     const int64_t b = p / bs;
     eb = ((int64_t)pat[b] * W) * bs + (p - b * bs);
-    es = bs;
   } else {
-    eb = p;
-    es = NB;
+    eb = ell_base(p, bs, W);
   }
+  es = bs;
 }
 
 // y[row(p)] = (A x)[row(p)]
@@ -323,8 +334,8 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32
   const int row = rowof[p];
   if (row < 0) return;
   int64_t eb, es;
-  ecol_addr(pat, bs, W, NB, p, eb, es);
-  y[row] = blk_row<TV, TI>(NB, bw ? bw[p / bs] : W, p, row, ecol, eb, es, a, x);
+  ecol_addr(pat, bs, W, p, eb, es);
+  y[row] = blk_row<TV, TI>(ell_base(p, bs, W), bs, bw ? bw[p / bs] : W, row, ecol, eb, es, a, x);
 }
 
 // Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
@@ -413,9 +424,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
   const int row = L.d_row[p];
   // the sweeps below hold ~100 registers anyway: the unrolled row costs no occupancy here
   int64_t eb, es;
-  ecol_addr(L.d_pat, BS, W, NB, p, eb, es);
+  ecol_addr(L.d_pat, BS, W, p, eb, es);
   const int wb = L.d_bw ? L.d_bw[b] : W;  // slots past the block's longest row are padding
-  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(NB, wb, p, row, ecol, eb, es, a, u) : (TV)0;
+  x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(((int64_t)b * W) * BS + t, BS, wb, row, ecol, eb, es, a, u) : (TV)0;
   if (row >= 0) vj[row] = sc * u[row];
   if (t == 0) x[BS] = (TV)0;
   if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform

@@ -123,8 +123,9 @@ static std::string build_bj(const std::vector<int32_t> &jrp, const std::vector<i
         const int c = jcols[k];
         if (blk[c] != b) continue;
         if (ns >= kW) return "more than 27 in-block entries in a row";
-        // position of (r, slot k-jrp[r]) in the block-ordered, slot-major Jacobian
-        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (int32_t)((k - jrp[r]) * NB + i);
+        // position of (r, slot k-jrp[r]) in the block-contiguous Jacobian, from
+        // the block's first entry (block b's W slots start at b*W*S)
+        out.jpos[(size_t)b * kW * S + (size_t)ns * S + i] = (int32_t)((k - jrp[r]) * S + i);
         out.lcol[(size_t)b * kW * S + (size_t)ns * S + i] = (uint16_t)local[c];
         if (c == r) out.diag[(size_t)b * S + i] = (uint8_t)ns;
         ns++;
@@ -424,6 +425,19 @@ static int ws_alloc(SolverWS &ws, T **p, size_t n) {
   return 0;
 }
 
+// an ELL table built slot-major over the block layout (slot k of position p
+// at k*NB + p) in the device's block-contiguous order ((b*W + k)*bs + t)
+template <class T>
+static std::vector<T> block_contiguous(const std::vector<T> &in, int W, int bs, int nblocks) {
+  const int64_t NB = (int64_t)nblocks * bs;
+  std::vector<T> out(in.size());
+  for (int64_t b = 0; b < nblocks; b++)
+    for (int k = 0; k < W; k++)
+      std::copy(in.begin() + (size_t)(k * NB + b * bs), in.begin() + (size_t)(k * NB + (b + 1) * bs),
+                out.begin() + (size_t)((b * W + k) * bs));
+  return out;
+}
+
 static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   if (!ctx->ws) ctx->ws = new SolverWS();
   SolverWS &ws = *ctx->ws;
@@ -482,9 +496,14 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
     ws.w_mean = hb.nblocks ? sum / hb.nblocks : 0.0;
   }
   if (!hb.pat.empty() && (rc = up(&ws.bj.d_pat, hb.pat))) return rc;
+  ws.bj.ell_w = ctx->jell_w;
+  // the host builds the ELL tables slot-major over the layout (k*NB + p); the
+  // device keeps them block-contiguous ((b*W + k)*bs + t), as the Jacobian
+  const int Wd = ctx->jell_w;
   if (!hb.ecol16.empty()) {
-    if ((rc = up(&ws.d_ecol16, hb.ecol16))) return rc;
-  } else if ((rc = up(&ws.d_ecol, hb.ecol))) {
+    if ((rc = up(&ws.d_ecol16, hb.pat.empty() ? block_contiguous(hb.ecol16, Wd, hb.bs, hb.nblocks) : hb.ecol16)))
+      return rc;
+  } else if ((rc = up(&ws.d_ecol, block_contiguous(hb.ecol, Wd, hb.bs, hb.nblocks)))) {
     return rc;
   }
   ws.part_version = ctx->pc_part_version;
@@ -518,7 +537,8 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
     if (ctx->jell_w <= 32 && build_coarse(hb, ctx->h_jrp, ctx->h_jcols, ctx->nc, ctx->jell_w, hc).empty()) {
       CoarseLayout &C = ws.cl;
       C.nb = hc.nb; C.ng = hc.ng; C.gsz = hc.gsz;
-      if ((rc = up(&C.nbr, hc.nbr)) || (rc = up(&C.cq, hc.cq)) || (rc = up(&C.grp, hc.grp)) ||
+      if ((rc = up(&C.nbr, hc.nbr)) || (rc = up(&C.cq, block_contiguous(hc.cq, ctx->jell_w, hb.bs, hb.nblocks))) ||
+          (rc = up(&C.grp, hc.grp)) ||
           (rc = up(&C.gmem, hc.gmem)))
         return rc;
       if ((rc = ws_alloc(ws, &C.A1, (size_t)hc.nb * kCQ)) ||
@@ -891,9 +911,9 @@ static int eval_J(tdyhip_ctx *ctx, const Form &fm, const double *X, double *J, h
   int rc;
   if (fm.type == 0) {
     k_udot<<<nblk(ctx->nc, 256), 256, 0, st>>>(ctx->nc, ws.dyn, X, fm.Uprev, ws.Udot, nullptr);
-    rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.NB, ws.bj.d_row, ws.NB);
+    rc = launch_flux(ctx, 0, 0.0, X, ws.Udot, nullptr, J, st, true, ws.d_perm, ws.bj.bs, ws.bj.d_row, ws.NB);
   } else {
-    rc = launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.NB, ws.bj.d_row, ws.NB);
+    rc = launch_flux(ctx, 1, 0.0, X, nullptr, nullptr, J, st, true, ws.d_perm, ws.bj.bs, ws.bj.d_row, ws.NB);
   }
   ctx->launch_dyn = nullptr;
   return rc;
