@@ -206,41 +206,55 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
 // with r2[h] the sum of rc1 over the blocks of group h.  Every workgroup
 // forms all of r2 itself (ng*gsz reads, in a fixed order), so one kernel
 // finishes the coarse solve and the prolongation only adds z1[block].
+// Every load is issued at the start (the group's G1 rows, its A2 row and
+// the member lists, then rc1 through them), so the kernel waits on memory
+// twice rather than once per loop trip; the sums keep their fixed order.
 constexpr int kL1Threads = 4 * kNG;
+constexpr int kL1Quarter = kGMax / 4;  // members per lane of an r2 sum
 __global__ void __launch_bounds__(kL1Threads) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
   __shared__ double rg[kGMax];
   __shared__ double r2s[kNG];
   __shared__ double sh[kL1Threads / 64];
   if (stop && *stop) return;
   const int g = blockIdx.x, t = threadIdx.x, G = C.gsz, ng = C.ng;
+  // this thread's row of the group's G1 (column-major: entry (t, j) at j*G + t)
+  double gv[kGMax];
+  const double *g1 = C.G1 + (int64_t)g * G * G + t;
+#pragma unroll
+  for (int j = 0; j < kGMax; j++) gv[j] = (t < G && j < G) ? g1[j * G] : 0.0;
+  const double a2 = t < ng ? C.A2[(int64_t)g * ng + t] : 0.0;
   // r2[h]: lanes 4h..4h+3 each sum a quarter of group h's members, then
   // ((q0 + q1) + (q2 + q3)) on all four lanes
+  const int h = t >> 2, qt = t & 3, QG = (G + 3) / 4;
+  const int m0 = qt * QG, nm = h < ng ? max(0, min(G, m0 + QG) - m0) : 0;
+  int bm[kL1Quarter];
+#pragma unroll
+  for (int k = 0; k < kL1Quarter; k++) bm[k] = k < nm ? C.gmem[h * G + m0 + k] : -1;
+  const int bt = t < G ? C.gmem[g * G + t] : -1;
+  double rv[kL1Quarter];
+#pragma unroll
+  for (int k = 0; k < kL1Quarter; k++) rv[k] = bm[k] >= 0 ? C.rc1[bm[k]] : 0.0;
+  const double rt = bt >= 0 ? C.rc1[bt] : 0.0;
   {
-    const int h = t >> 2, qt = t & 3, QG = (G + 3) / 4;
     double s = 0.0;
-    if (h < ng) {
-      const int m1 = min(G, (qt + 1) * QG);
-      for (int m = qt * QG; m < m1; m++) {
-        const int bm = C.gmem[h * G + m];
-        if (bm >= 0) s += C.rc1[bm];
-      }
-    }
+#pragma unroll
+    for (int k = 0; k < kL1Quarter; k++)
+      if (bm[k] >= 0) s += rv[k];
     const double o1 = __shfl_xor(s, 1, 64);
     const double p = (t & 1) ? o1 + s : s + o1;
     const double o2 = __shfl_xor(p, 2, 64);
     if (h < ng && qt == 0) r2s[h] = p + o2;
   }
-  const int bt = t < G ? C.gmem[g * G + t] : -1;
-  if (t < kGMax) rg[t] = bt >= 0 ? C.rc1[bt] : 0.0;
+  if (t < kGMax) rg[t] = rt;
   __syncthreads();
   // z2 = row g of A2^-1 times r2
-  const double z2 = block_sum_n<kL1Threads>(t < ng ? C.A2[(int64_t)g * ng + t] * r2s[t] : 0.0, sh);
-  // G1 column-major per group; padded members have zero rows, columns and rg
+  const double z2 = block_sum_n<kL1Threads>(t < ng ? a2 * r2s[t] : 0.0, sh);
+  // padded members have zero rows, columns and rg
   if (t < G) {
-    const double *g1 = C.G1 + (int64_t)g * G * G + t;
     double z = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < G; j++) z += g1[j * G] * rg[j];
+#pragma unroll
+    for (int j = 0; j < kGMax; j++)
+      if (j < G) z += gv[j] * rg[j];
     if (bt >= 0) C.z1[bt] = z + z2;
   }
 }
