@@ -134,26 +134,56 @@ __device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col
 }
 
 // One workgroup (kGMax threads) per group: invert the group's diagonal block
-// of A1 into G1 and form the group's row of A2.
+// of A1 into G1 and form the group's row of A2.  The members' neighbour
+// lists (block, group, A1 value) are staged in LDS first, all their loads in
+// flight together.
 __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
   constexpr int LD = kGMax + 1;  // odd row pitch: column reads do not hit one LDS bank
+  constexpr int NQ = kGMax * kCQ;
+  constexpr int PER = NQ / kGMax;  // neighbour entries per thread
   __shared__ double M[kGMax * LD];
   __shared__ double col[kGMax];
   __shared__ int piv[kGMax];
   __shared__ int flag;
   __shared__ int mem[kGMax];
+  __shared__ int gq[NQ];     // group of each member's neighbour q (-1 none)
+  __shared__ int nq[NQ];     // the neighbour block
+  __shared__ double aq[NQ];  // its A1 entry
   const int g = blockIdx.x, t = threadIdx.x, G = C.gsz;
   if (t < G) mem[t] = C.gmem[g * G + t];
   __syncthreads();
   int n = 0;
   while (n < G && mem[n] >= 0) n++;
+  {
+    int nb[PER];
+    double av[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int e = t + r * kGMax, i = e / kCQ, q = e % kCQ;
+      nb[r] = i < n ? C.nbr[mem[i] * kCQ + q] : -1;
+      av[r] = i < n ? C.A1[mem[i] * kCQ + q] : 0.0;
+    }
+    int gv[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) gv[r] = nb[r] >= 0 ? C.grp[nb[r]] : -1;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int e = t + r * kGMax;
+      nq[e] = nb[r];
+      gq[e] = gv[r];
+      aq[e] = nb[r] >= 0 ? av[r] : 0.0;
+    }
+  }
+  for (int e = t; e < n * LD; e += blockDim.x) M[e] = 0.0;
+  __syncthreads();
   // row i of the diagonal block: A1 of member i at the columns of member j
-  for (int e = t; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e % n, bi = mem[i], bj = mem[j];
-    double a = 0.0;
-    for (int q = 0; q < kCQ; q++)
-      if (C.nbr[bi * kCQ + q] == bj) a = C.A1[bi * kCQ + q];
-    M[i * LD + j] = a;
+  // (a neighbour list holds each block once)
+  for (int e = t; e < n * kCQ; e += blockDim.x) {
+    if (gq[e] != g) continue;
+    const int i = e / kCQ, bj = nq[e];
+    int j = 0;
+    while (j < n && mem[j] != bj) j++;
+    if (j < n) M[i * LD + j] = aq[e];
   }
   __syncthreads();
   const bool ok = gj_invert(M, n, LD, piv, col, &flag);
@@ -166,17 +196,8 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     const int j = e / G, i = e % G;
     C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * LD + j] : 0.0;
   }
-  // A2[g][h] = sum over members and their neighbour entries in group h,
-  // from an LDS list of (group, value) in a fixed order
-  __shared__ int gq[kGMax * kCQ];
-  __shared__ double aq[kGMax * kCQ];
-  for (int e = t; e < n * kCQ; e += blockDim.x) {
-    const int i = e / kCQ, q = e % kCQ;
-    const int nb = C.nbr[mem[i] * kCQ + q];
-    gq[e] = nb >= 0 ? C.grp[nb] : -1;
-    aq[e] = nb >= 0 ? C.A1[mem[i] * kCQ + q] : 0.0;
-  }
-  __syncthreads();
+  // A2[g][h] = sum over members and their neighbour entries in group h, in
+  // the fixed order of the list
   for (int h = t; h < C.ng; h += blockDim.x) {
     double s = 0.0;
     for (int e = 0; e < n * kCQ; e++) s += (gq[e] == h) ? aq[e] : 0.0;
@@ -184,21 +205,118 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
   }
 }
 
-// Single workgroup: A2 <- A2^-1 (ng <= kNG, held in LDS).
+// Single workgroup of 1024 threads: A2 <- A2^-1 (ng <= kNG) by the
+// Gauss-Jordan steps of gj_invert (same pivots, same operations per entry,
+// so the same result), with the matrix in registers: thread t holds the 4x4
+// tile of rows 4*(t/32).. and columns 4*(t%32)..; each step publishes the
+// pivot row, the row it replaces and the pivot column through LDS.
 __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
-  constexpr int LD = kNG + 1;  // odd row pitch: column reads do not hit one LDS bank
-  __shared__ double M[kNG * LD];
-  __shared__ double col[kNG];
-  __shared__ int piv[kNG];
-  __shared__ int flag;
-  const int n = C.ng;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[(e / n) * LD + e % n] = C.A2[e];
-  __syncthreads();
-  if (!gj_invert(M, n, LD, piv, col, &flag)) {
-    if (threadIdx.x == 0) atomicOr(C.status, 1);
-    return;
+  constexpr int NT = 1024, TB = 4, NTC = kNG / TB;  // tiles per row of tiles
+  static_assert(NTC * NTC == NT, "one 4x4 tile per thread");
+  __shared__ double rowp[kNG], rowk[kNG], colk[kNG];
+  __shared__ double cv[NTC];
+  __shared__ int ci[NTC];
+  __shared__ int piv[kNG], src[kNG], dst[kNG];
+  __shared__ int flag, pk;
+  __shared__ double dkk;
+  const int n = C.ng, t = threadIdx.x;
+  const int rb = t / NTC, cb = t % NTC, r0 = rb * TB, c0 = cb * TB;
+  double m[TB][TB];
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++)
+      m[a][b] = (r0 + a < n && c0 + b < n) ? C.A2[(int64_t)(r0 + a) * n + c0 + b] : 0.0;
+  if (t == 0) flag = 0;
+  for (int k = 0; k < n; k++) {
+    const int kb = k / TB, ka = k % TB;
+    // pivot of column k: the first row i >= k of largest |M[i][k]|
+    if (cb == kb) {
+      double mx = -1.0;
+      int p = n;
+#pragma unroll
+      for (int a = 0; a < TB; a++) {
+        const int i = r0 + a;
+        const double v = fabs(m[a][ka]);
+        if (i >= k && i < n && v > mx) { mx = v; p = i; }
+      }
+      cv[rb] = mx;
+      ci[rb] = p;
+      if (rb == kb) dkk = m[ka][ka];
+    }
+    __syncthreads();
+    if (t < 64) {
+      double mx = t < NTC ? cv[t] : -1.0;
+      int p = t < NTC ? ci[t] : n;
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) {
+        const double om = __shfl_xor(mx, off, 64);
+        const int op = __shfl_xor(p, off, 64);
+        if (om > mx || (om == mx && op < p)) { mx = om; p = op; }
+      }
+      if (t == 0) {
+        piv[k] = p;
+        pk = p;
+        if (!(mx > 0.0) || dkk != dkk) flag = 1;
+      }
+    }
+    __syncthreads();
+    if (flag) {
+      if (t == 0) atomicOr(C.status, 1);
+      return;  // uniform over the workgroup
+    }
+    const int p = pk, pb = p / TB, pa = p % TB;
+    // publish row p (the new row k) and row k (the new row p)
+    if (rb == pb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) rowp[c0 + b] = m[pa][b];
+    if (rb == kb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) rowk[c0 + b] = m[ka][b];
+    __syncthreads();
+    const double inv = 1.0 / rowp[k];
+    double sr[TB];  // row k after the swap and scaling: inv at column k
+#pragma unroll
+    for (int b = 0; b < TB; b++) sr[b] = (c0 + b == k) ? inv : rowp[c0 + b] * inv;
+    if (p != k && rb == pb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[pa][b] = rowk[c0 + b];
+    if (rb == kb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[ka][b] = sr[b];
+    // column k after the swap (0 at row k)
+    if (cb == kb)
+#pragma unroll
+      for (int a = 0; a < TB; a++) colk[r0 + a] = (r0 + a == k) ? 0.0 : m[a][ka];
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < TB; a++) {
+      const int i = r0 + a;
+      if (i == k) continue;
+      const double c = colk[i];
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[a][b] = ((c0 + b == k) ? 0.0 : m[a][b]) - c * sr[b];
+    }
   }
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) C.A2[e] = M[(e / n) * LD + e % n];
+  // undo the row interchanges as column interchanges, last first: the
+  // result's column j is column src[j] of M
+  __syncthreads();
+  if (t == 0) {
+    for (int j = 0; j < n; j++) src[j] = j;
+    for (int k = n - 1; k >= 0; k--) {
+      const int q = src[k];
+      src[k] = src[piv[k]];
+      src[piv[k]] = q;
+    }
+  }
+  __syncthreads();
+  if (t < n) dst[src[t]] = t;
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++)
+      if (r0 + a < n && c0 + b < n) C.A2[(int64_t)(r0 + a) * n + dst[c0 + b]] = m[a][b];
 }
 
 // Level 1 and 2 of an application, one workgroup per group g:
