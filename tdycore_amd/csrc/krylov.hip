@@ -105,10 +105,26 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   const int64_t base = (int64_t)pb * BS;
   const int64_t ibase = (int64_t)pb * kW * BS;
   const int64_t sbase = (int64_t)b * kW * BS;
-  for (int e = tid; e < kW * BS; e += NT) {
-    const int32_t pos = L.d_jpos[ibase + e];  // relative to the block's first position
-    a[e / BS][e % BS] = (pos >= 0) ? (TV)J[pos + (int64_t)b * L.ell_w * BS] : (TV)0;
-    lc[e / BS][e % BS] = L.d_lcol[ibase + e];
+  {
+    // all of the block's value loads in flight together
+    constexpr int NE = (kW * BS + NT - 1) / NT;
+    const double *Jb = J + (int64_t)b * L.ell_w * BS;
+    int32_t pos[NE];
+    uint16_t lcv[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+      const int e = tid + i * NT;
+      pos[i] = e < kW * BS ? L.d_jpos[ibase + e] : -1;  // relative to the block's first position
+      lcv[i] = e < kW * BS ? L.d_lcol[ibase + e] : (uint16_t)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+      const int e = tid + i * NT;
+      if (e < kW * BS) {
+        a[e / BS][e % BS] = (pos[i] >= 0) ? (TV)Jb[pos[i]] : (TV)0;
+        lc[e / BS][e % BS] = lcv[i];
+      }
+    }
   }
   for (int r = tid; r < BS; r += NT) {
     ns[r] = L.d_nslot[base + r];
