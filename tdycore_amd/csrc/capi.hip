@@ -704,7 +704,7 @@ static int launch_flux_general(tdyhip_ctx *ctx, int form, double shift, const do
     gg.jcols = ctx->d_jcols; gg.jdiag = ctx->d_jdiag; gg.face_flags = ctx->d_face_flags; gg.gslot = ctx->d_gslot;
     gg.D = ctx->d_Dgen; gg.dM = ctx->d_dM; gg.d2M = ctx->d_d2M; gg.Udot = Udot; gg.vol = ctx->d_cell_vol; gg.J = J;
     gg.dyn = ctx->launch_dyn; gg.stop = ctx->launch_stop;
-    k_gather_general_J<<<nblk(ctx->nc, kGenJBlock), kGenJBlock, 0, st>>>(gg);
+    k_gather_general_J<<<nblk(ctx->nc, kGenJBlock / kGenJLanes), kGenJBlock, 0, st>>>(gg);  // kGenJLanes lanes per cell
   }
   return hipGetLastError() == hipSuccess ? 0 : TDYHIP_ERR_HIP;
 }

@@ -342,10 +342,14 @@ __global__ void __launch_bounds__(256) k_rows_general(GenRowArgs a) {
 
 // Jacobian row of each cell (CSR in the vertex-adjacency pattern, or the
 // solver's ELL): the derivatives of its faces' flux rows in a fixed (face,
-// sub-face, column) order, then the accumulation diagonal.  One thread owns
-// one row; the row is summed in LDS (up to kGenJSlots entries) and every
-// slot written once.
-constexpr int kGenJBlock = 64;
+// sub-face) row order, then the accumulation diagonal.  kGenJLanes lanes own
+// one row, summed in LDS (up to kGenJSlots entries), every slot written once.
+// Lane l first reads row l's metadata (face lf = l/4, sub-face l%4); then a
+// lane per column loads the rows' derivatives eight rows at a time and adds
+// them row by row.  A flux row adds to each slot at most once (its columns
+// are distinct cells), so every slot is summed in row order.
+constexpr int kGenJBlock = 256;
+constexpr int kGenJLanes = 32;  // lanes per cell: columns of a flux row (a row of more loops)
 constexpr int kGenJSlots = 64;
 struct GenGatherArgs {
   int nc, form, ell_w;
@@ -510,34 +514,61 @@ __global__ void __launch_bounds__(256) k_gen_slot_table(int64_t nrows, const int
 }
 
 __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a) {
-  __shared__ double acc[kGenJSlots][kGenJBlock];
+  constexpr int CPB = kGenJBlock / kGenJLanes;  // cells per workgroup
+  constexpr int NR = 24;                        // flux rows of a cell: 6 faces x 4 sub-faces
+  constexpr int RB = 8;                         // rows whose loads are in flight together
+  static_assert(kGenJLanes >= NR && 64 % kGenJLanes == 0, "a cell's lanes hold its rows and share a wavefront");
+  __shared__ double acc[CPB][kGenJSlots];
   if (a.stop && *a.stop) return;
-  const int t = threadIdx.x;
-  const int c = blockIdx.x * kGenJBlock + t;
-  if (c >= a.nc) return;
+  const int g = threadIdx.x / kGenJLanes, l = threadIdx.x % kGenJLanes;
+  const int base = (threadIdx.x & 63) - l;  // the cell's first lane within the wavefront
+  const int c = blockIdx.x * CPB + g;
+  if (c >= a.nc) return;  // uniform over the cell's lanes (the wavefront's other cell is independent)
   const int r0 = a.jrp[c], len = a.jrp[c + 1] - r0;
-  for (int k = 0; k < len; k++) acc[k][t] = 0.0;
-  for (int lf = 0; lf < 6; lf++) {
-    const int f = a.cell_faces[c * 6 + lf];
-    if (f < 0 || (a.face_flags[f] & kFaceNoFlow)) continue;
-    const bool is_up = a.face_cells[2 * f] == c;
-    const int side = is_up ? 0 : 1;
-    for (int q = 0; q < 4; q++) {
-      const int64_t r = (int64_t)f * 4 + q;
-      const int to = a.rtoff[r];
-      if (to < 0) continue;
-      const int ncell = a.rinfo[r] & 0xff;
-      const uint8_t *sl = a.gslot + 2 * (int64_t)to + side;
-      for (int j = 0; j < ncell; j++) {
-        const double d = a.D[to + j];
-        acc[sl[2 * j]][t] += is_up ? d : -d;
+  for (int k = l; k < len; k += kGenJLanes) acc[g][k] = 0.0;
+  // row l's metadata: its derivatives at D[to ..], its slots, the sign by the cell's side
+  int to = -1, nce = 0, side = 0;
+  bool up = false;
+  if (l < NR) {
+    const int f = a.cell_faces[c * 6 + (l >> 2)];
+    if (f >= 0 && !(a.face_flags[f] & kFaceNoFlow)) {
+      const int64_t r = (int64_t)f * 4 + (l & 3);
+      to = a.rtoff[r];
+      nce = to >= 0 ? (a.rinfo[r] & 0xff) : 0;
+      up = a.face_cells[2 * f] == c;
+      side = up ? 0 : 1;
+    }
+  }
+  for (int i0 = 0; i0 < NR; i0 += RB) {
+    double dv[RB];
+    int sv[RB];
+#pragma unroll
+    for (int k = 0; k < RB; k++) {
+      const int ti = __shfl(to, base + i0 + k, 64), ni = __shfl(nce, base + i0 + k, 64);
+      const int si = __shfl(side, base + i0 + k, 64);
+      const bool on = ti >= 0 && l < ni;
+      dv[k] = on ? a.D[ti + l] : 0.0;
+      sv[k] = on ? a.gslot[2 * ((int64_t)ti + l) + si] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < RB; k++) {
+      const int ti = __shfl(to, base + i0 + k, 64), ni = __shfl(nce, base + i0 + k, 64);
+      const int si = __shfl(side, base + i0 + k, 64);
+      const bool ui = __shfl((int)up, base + i0 + k, 64) != 0;
+      if (sv[k] >= 0) acc[g][sv[k]] += ui ? dv[k] : -dv[k];
+      // columns past the lanes (regions of more than kGenJLanes cells): the rest of the same row
+      for (int j = l + kGenJLanes; ti >= 0 && j < ni; j += kGenJLanes) {
+        const double d = a.D[ti + j];
+        acc[g][a.gslot[2 * ((int64_t)ti + j) + si]] += ui ? d : -d;
       }
     }
   }
-  const double shift = a.dyn ? a.dyn[1] : a.shift, dtime = a.dyn ? a.dyn[0] : a.dtime;
-  const double dv = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
-                                  : a.dM[c] * a.vol[c] * (1.0 / dtime);
-  acc[a.jdiag[c]][t] += dv;
+  if (l == 0) {
+    const double shift = a.dyn ? a.dyn[1] : a.shift, dtime = a.dyn ? a.dyn[0] : a.dtime;
+    const double dvc = (a.form == 0) ? (shift * a.dM[c] + a.d2M[c] * a.Udot[c]) * a.vol[c]
+                                     : a.dM[c] * a.vol[c] * (1.0 / dtime);
+    acc[g][a.jdiag[c]] += dvc;
+  }
   int64_t st = 1;
   double *row = a.J + r0;
   if (a.perm) {
@@ -546,7 +577,7 @@ __global__ void __launch_bounds__(kGenJBlock) k_gather_general_J(GenGatherArgs a
     st = bs;
   }
   const int nslot = a.perm ? a.ell_w : len;  // ELL rows are padded to ell_w slots with zeros
-  for (int k = 0; k < nslot; k++) row[k * st] = k < len ? acc[k][t] : 0.0;
+  for (int k = l; k < nslot; k += kGenJLanes) row[k * st] = k < len ? acc[g][k] : 0.0;
 }
 
 struct GatherArgs {

@@ -29,13 +29,16 @@ def _problem(name):
         pr["face_bc"] = m.face_bc(default=mg.BC_DIRICHLET)
         pr["kw"]["p_bnd_fn"] = mg.boundary_values(m, lambda x: mg.hydrostatic_pressure(x[:, 2], 2.0))
         return pr
+    if name == "wheel17":  # 34 cells around the middle axis vertex: flux rows wider than the gather's 32 lanes
+        P, T = workloads.wheel_triangulation(n_spokes=17)
+        return workloads.prism_problem(nlayers=2, seed=16, points=P, triangles=T, lattice=8)
     if name == "hex_star":  # hexahedra beyond the fixed-size kernels' limits take the general path
         return workloads.star_hex_problem(n_quads=6, layers=3)
     raise KeyError(name)
 
 
 @pytest.mark.parametrize("name", ["delaunay", "delaunay_tuned", "delaunay_tpf", "wheel", "wheel_all_dirichlet",
-                                  "hex_star"])
+                                  "wheel17", "hex_star"])
 def test_prism_setup_matches_oracle(name):
     import torch
     from pyoracle import Oracle
@@ -60,6 +63,8 @@ def test_prism_setup_matches_oracle(name):
         assert info["buckets"][2] > 0  # a region of more than 32 unknowns
     if name == "hex_star":
         assert info["max_cells"] == 12 and info["max_faces"] == 18  # took the general path
+    if name == "wheel17":
+        assert info["max_cells"] == 34
     for b, w in zip(info["buckets"], info["lanes"]):
         assert (w == 0) == (b == 0) and w in (0, 16, 32, 64)
     plan = g.setup_plan()
@@ -99,7 +104,7 @@ def test_prism_setup_plans_agree_bitwise(name, monkeypatch):
         assert np.array_equal(v, v0), spec
 
 
-@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet", "hex_star"])
+@pytest.mark.parametrize("name", ["delaunay", "delaunay_tpf", "wheel_all_dirichlet", "wheel17", "hex_star"])
 def test_prism_residual_and_jacobian_match_oracle(name):
     """TS and SNES residuals and Jacobians on prisms, through the C ABI,
     against the oracle at 1e-12 (residual: of its largest entry; Jacobian:
