@@ -89,6 +89,7 @@ struct FactorLanes {
   static constexpr int TPR = BS <= 256 ? 4 : 2;  // lanes per row (workgroup <= 1024 threads)
   static constexpr int NT = BS * TPR;
   static constexpr int WPL = (kPairW + TPR - 1) / TPR;  // pair words per lane
+  static constexpr int AHEAD = 3;                        // lower slots whose update lists are in flight
 };
 
 template <class TV, int BS>
@@ -135,9 +136,10 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   const int nlev = L.d_nlev[2 * pb];
   const int nsl = L.d_nslot[base + t];
   const int di0 = L.d_diag[base + t];
-  // The update lists of lower slot p+1 (count and this lane's packed pair
-  // words) are fetched while slot p is eliminated, and slot 0's before the
-  // level loop, so global-memory latency is off the elimination chain.
+  // The update lists of lower slots p+1 .. p+AHEAD (count and this lane's
+  // packed pair words) are in flight while slot p is eliminated, the first
+  // ones fetched before the level loop, so global-memory latency is off the
+  // elimination chain.
   const int64_t pbase = (int64_t)pb * kPair * BS;
   const int64_t nbase = (int64_t)pb * kLU * BS;
   auto fetch = [&](int p, int &cnt, uint32_t (&w)[WPL]) {
@@ -148,33 +150,64 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
       w[i] = (p < di0 && wq < kPairW) ? L.d_pair[pbase + (int64_t)(p * kPairW + wq) * BS + t] : 0u;
     }
   };
-  int ncur;
-  uint32_t cur[WPL];
-  fetch(0, ncur, cur);
+  // update lists of the next kAhead lower slots in flight (a rotating queue)
+  constexpr int kAhead = FactorLanes<BS>::AHEAD;
+  int nq[kAhead];
+  uint32_t qw[kAhead][WPL];
+#pragma unroll
+  for (int d = 0; d < kAhead; d++) fetch(d, nq[d], qw[d]);
   __syncthreads();
   for (int l = 0; l < nlev; l++) {
     if (lev == l && nsl > 0) {
+      // A slot's LDS reads go out as one batch: its own value, the pivot of
+      // row k (finished at an earlier level), the pair sources in row k and
+      // the pair targets in row t (distinct slots above p, so no read of the
+      // batch waits for a write of the same slot), and the next slot's
+      // column; the next column's diagonal slot is read behind the writes.
+      int k = di0 > 0 ? lc[0][t] : t;
+      int dk = dg[k];
       for (int p = 0; p < di0; p++) {  // lower entries, increasing column
-        int nnext;
-        uint32_t nxt[WPL];
-        fetch(p + 1, nnext, nxt);
-        const int k = lc[p][t];
-        const TV lik = a[p][t] / a[dg[k]][k];
+        const int ncur = nq[0];
+        uint32_t cur[WPL];
+#pragma unroll
+        for (int i = 0; i < WPL; i++) cur[i] = qw[0][i];
+#pragma unroll
+        for (int d = 0; d + 1 < kAhead; d++) {
+          nq[d] = nq[d + 1];
+#pragma unroll
+          for (int i = 0; i < WPL; i++) qw[d][i] = qw[d + 1][i];
+        }
+        fetch(p + kAhead, nq[kAhead - 1], qw[kAhead - 1]);
+        TV src[2 * WPL], tgt[2 * WPL];
+#pragma unroll
+        for (int i = 0; i < WPL; i++) {
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int q = 2 * (sub + TPR * i) + h;  // pair index of this lane's half-word
+            const uint32_t w = (q < ncur) ? (h ? (cur[i] >> 16) : (cur[i] & 0xffffu)) : 0u;  // unused pairs read slot 0
+            src[2 * i + h] = a[w >> 8][k];
+            tgt[2 * i + h] = a[w & 0xff][t];
+          }
+        }
+        const TV pv = a[dk][k];
+        const TV ap = a[p][t];
+        const int kn = (p + 1 < di0) ? lc[p + 1][t] : k;
+        const TV lik = ap / pv;
         __builtin_amdgcn_wave_barrier();  // every lane has read a[p][t] before it is overwritten
         if (sub == 0) a[p][t] = lik;
 #pragma unroll
         for (int i = 0; i < WPL; i++) {
 #pragma unroll
           for (int h = 0; h < 2; h++) {
-            const int q = 2 * (sub + TPR * i) + h;  // pair index of this lane's half-word
+            const int q = 2 * (sub + TPR * i) + h;
             const uint32_t w = h ? (cur[i] >> 16) : (cur[i] & 0xffffu);
-            if (q < ncur) a[w & 0xff][t] -= lik * a[w >> 8][k];
+            if (q < ncur) a[w & 0xff][t] = tgt[2 * i + h] - lik * src[2 * i + h];
           }
         }
+        const int dkn = dg[kn];
         __builtin_amdgcn_wave_barrier();
-        ncur = nnext;
-#pragma unroll
-        for (int i = 0; i < WPL; i++) cur[i] = nxt[i];
+        k = kn;
+        dk = dkn;
       }
       if (sub == 0 && a[dg[t]][t] == (TV)0) atomicOr(status, 1);
     }
