@@ -37,10 +37,15 @@ struct CoarseLayout {
 };
 
 // A1[b][q] = sum over the block's rows and slots whose column lies in block
-// nbr[b][q] of J (one workgroup per block, one thread per row).
+// nbr[b][q] of J (one workgroup per block, one thread per row).  Each thread
+// forms its 32 per-neighbour sums, one register reduce-scatter per wavefront
+// gives the wavefront's 32 sums (combining lanes in the same xor pairs as a
+// shuffle-down tree), and thread q adds the wavefronts' sums in order.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, int W, const double *__restrict__ J) {
-  __shared__ double sh[BS / 64];
+  static_assert(kCQ == 32, "one reduce-scatter value per neighbour");
+  constexpr int NW = BS / 64;
+  __shared__ double wsum[NW][kCQ];
   const int b = blockIdx.x, t = threadIdx.x;
   const int64_t e0 = ((int64_t)b * W) * BS + t;  // block-contiguous ELL: slot k at e0 + k*BS
   uint8_t q_of[32];
@@ -54,13 +59,25 @@ __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, 
       v_of[k] = q_of[k] != 255 ? J[e0 + k * BS] : 0.0;
     }
   }
+  int nq = 0;  // neighbour blocks (uniform over the workgroup)
+  while (nq < kCQ && C.nbr[b * kCQ + nq] >= 0) nq++;
+  double v[kCQ];
+#pragma unroll
   for (int q = 0; q < kCQ; q++) {
-    if (C.nbr[b * kCQ + q] < 0) break;  // uniform over the workgroup
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 32; k++) s += (q_of[k] == q) ? v_of[k] : 0.0;
-    s = block_sum_n<BS>(s, sh);
-    if (t == 0) C.A1[b * kCQ + q] = s;
+    v[q] = s;
+  }
+  const double r = wave_reduce_scatter<kCQ>(v);  // lanes 2q and 2q+1 hold neighbour q's sum
+  const int lane = t & 63;
+  if ((lane & 1) == 0) wsum[t >> 6][lane >> 1] = r;
+  __syncthreads();
+  if (t < nq) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) s += wsum[w][t];
+    C.A1[b * kCQ + t] = s;
   }
 }
 
