@@ -224,6 +224,24 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   if (sub == 0) fac[sbase + (int64_t)(2 * kLU) * BS + t] = nsl > 0 ? (TV)1 / a[di][t] : (TV)0;
 }
 
+// This thread's share of p[0..n) in a workgroup of NT threads: the sum of
+// p[c] over c = t, t + NT, ... in increasing c, its loads issued eight at a
+// time (a rolled loop would wait for each load before the next).
+template <int NT, class TV>
+__device__ inline TV strided_sum(const TV *__restrict__ p, int n) {
+  TV s = 0;
+  int c = threadIdx.x;
+  for (; c + 7 * NT < n; c += 8 * NT) {
+    TV v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = p[c + i * NT];
+#pragma unroll
+    for (int i = 0; i < 8; i++) s += v[i];
+  }
+  for (; c < n; c += NT) s += p[c];
+  return s;
+}
+
 // Sum over an NT-thread workgroup; every thread gets the result.
 template <int NT, class TV>
 __device__ inline TV block_sum_n(TV v, TV *sh) {
@@ -457,7 +475,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
     if (stop_s) return;  // uniform over the workgroup
     const TV *part = static_cast<const TV *>(na.part);
     TV s = 0;
-    for (int c = t; c < na.nchunk; c += BS) s += part[c];
+    s = strided_sum<BS>(part, na.nchunk);
     s = block_sum_n<BS>(s, shn);
     const double hn = sqrt((double)s);
     sc = (TV)(hn > 0.0 ? 1.0 / hn : 0.0);
@@ -622,7 +640,7 @@ __global__ void k_reduce_partials(int nv, int nchunk, const TV *__restrict__ par
   if (stop && *stop) return;
   const int j = blockIdx.x;
   TV s = 0;
-  for (int c = threadIdx.x; c < nchunk; c += kRedBlock) s += partial[(int64_t)j * nchunk + c];
+  s = strided_sum<kRedBlock>(partial + (int64_t)j * nchunk, nchunk);
   s = block_sum(s, sh);
   if (threadIdx.x == 0) out[j] = (double)s;
 }
@@ -778,7 +796,7 @@ __global__ void __launch_bounds__(kRedBlock) k_norm_arnoldi(int nchunk, const TV
   __shared__ TV sh[kRedBlock / 64];
   if (gd->done) return;
   TV s = 0;
-  for (int c = threadIdx.x; c < nchunk; c += kRedBlock) s += partial[c];
+  s = strided_sum<kRedBlock>(partial, nchunk);
   s = block_sum(s, sh);
   if (threadIdx.x == 0) {
     h[j + 1] = (double)s;
