@@ -405,38 +405,45 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32
   y[row] = blk_row<TV, TI>(ell_base(p, bs, W), bs, bw ? bw[p / bs] : W, row, ecol, eb, es, a, x);
 }
 
-// Arnoldi column j: Hessenberg entries from h[0..j] (dots) and h[j+1] (||w||^2),
-// previous Givens rotations, the new rotation, residual estimate and stop test.
-__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h,
+// Arnoldi column j: Hessenberg entries from h[0..j] (dots) and hj1 = h[j+1]
+// (||w||^2), previous Givens rotations, the new rotation, residual estimate
+// and stop test.  Rotation i turns (entry i, entry i+1) into (final entry i,
+// the entry i+1 that rotation i+1 reads), so the running entry stays in a
+// register and nothing written is read back from memory.
+__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h, double hj1,
                                       double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
                                       double *__restrict__ g) {
   const int ld = m + 1;
-  const double hn = sqrt(h[j + 1]);
-  for (int i = 0; i <= j; i++) H[i + j * ld] = h[i];
-  H[j + 1 + j * ld] = hn;
+  const double hn = sqrt(hj1);
+  double a = h[0];
   for (int i = 0; i < j; i++) {
-    const double a = H[i + j * ld], c = H[i + 1 + j * ld];
-    H[i + j * ld] = cs[i] * a + sn[i] * c;
-    H[i + 1 + j * ld] = -sn[i] * a + cs[i] * c;
+    const double c = h[i + 1], ci = cs[i], si = sn[i];
+    H[i + j * ld] = ci * a + si * c;
+    a = -si * a + ci * c;
   }
-  const double a = H[j + j * ld], c = H[j + 1 + j * ld];
+  const double c = hn;
   const double d = sqrt(a * a + c * c);
-  cs[j] = d > 0 ? a / d : 1.0;
-  sn[j] = d > 0 ? c / d : 0.0;
+  const double csj = d > 0 ? a / d : 1.0, snj = d > 0 ? c / d : 0.0;
+  cs[j] = csj;
+  sn[j] = snj;
   H[j + j * ld] = d;
   H[j + 1 + j * ld] = 0.0;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  gd->its++;
+  const double gj = g[j], gj1 = -snj * gj;
+  g[j + 1] = gj1;
+  g[j] = csj * gj;
+  const int its = gd->its + 1;
+  gd->its = its;
   gd->jcur = j + 1;
-  gd->resid = fabs(g[j + 1]);
+  const double resid = fabs(gj1);
+  gd->resid = resid;
   gd->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
-  const bool bad = !isfinite(gd->resid) || !isfinite(hn);
-  const int conv = !bad && (gd->resid <= gd->tol || hn == 0.0);
+  const bool bad = !isfinite(resid) || !isfinite(hn);
+  const int conv = !bad && (resid <= gd->tol || hn == 0.0);
   gd->converged = conv;
   if (bad) gd->diverged = 1;
-  if (conv || bad || gd->its >= gd->maxit) gd->finished = 1;
-  if (gd->finished || j + 1 == m) gd->done = 1;
+  const bool fin = conv || bad || its >= gd->maxit;  // finished is 0 on entry (the step runs only while not done)
+  if (fin) gd->finished = 1;
+  if (fin || j + 1 == m) gd->done = 1;
 }
 
 // ---------------------------------------------------------------- fused Arnoldi matvec
@@ -481,7 +488,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
     sc = (TV)(hn > 0.0 ? 1.0 / hn : 0.0);
     if (b == 0 && t == 0) {
       na.h[na.j + 1] = (double)s;
-      arnoldi_update(gd, na.j, na.m, na.h, na.H, na.cs, na.sn, na.g);
+      arnoldi_update(gd, na.j, na.m, na.h, (double)s, na.H, na.cs, na.sn, na.g);
     }
   } else {
     if (stop && *stop) return;  // uniform over the workgroup
@@ -783,7 +790,7 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
 __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
                                double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
   if (gd->done) return;
-  arnoldi_update(gd, j, m, h, H, cs, sn, g);
+  arnoldi_update(gd, j, m, h, h[j + 1], H, cs, sn, g);
 }
 
 // h[j+1] = sum of the nchunk norm partials (fixed order, as k_reduce_partials)
@@ -800,7 +807,7 @@ __global__ void __launch_bounds__(kRedBlock) k_norm_arnoldi(int nchunk, const TV
   s = block_sum(s, sh);
   if (threadIdx.x == 0) {
     h[j + 1] = (double)s;
-    arnoldi_update(gd, j, m, h, H, cs, sn, g);
+    arnoldi_update(gd, j, m, h, (double)s, H, cs, sn, g);
   }
 }
 
@@ -814,6 +821,28 @@ __global__ void k_hess_solve(GmresDev *gd, int m, const double *__restrict__ H, 
     for (int k = i + 1; k < j; k++) s -= H[i + k * ld] * y[k];
     y[i] = s / H[i + i * ld];
   }
+}
+
+// The same with the cycle's Hessenberg columns staged in LDS by the
+// workgroup and y formed in LDS, so the one thread's back substitution waits
+// on no global load or store (same operations, same order).  Dynamic LDS:
+// ((m + 1) * m + m) doubles.
+constexpr int kHessLdsMaxM = 64;
+__global__ void __launch_bounds__(256) k_hess_solve_lds(GmresDev *gd, int m, const double *__restrict__ H,
+                                                        const double *__restrict__ g, double *__restrict__ y) {
+  extern __shared__ double hs[];
+  const int ld = m + 1, j = gd->jcur;
+  for (int e = threadIdx.x; e < ld * j; e += blockDim.x) hs[e] = H[e];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  gd->jlast = j;
+  double *ys = hs + (size_t)ld * m;
+  for (int i = j - 1; i >= 0; i--) {
+    double s = g[i];
+    for (int k = i + 1; k < j; k++) s -= hs[i + k * ld] * ys[k];
+    ys[i] = s / hs[i + i * ld];
+  }
+  for (int i = 0; i < j; i++) y[i] = ys[i];
 }
 
 // x += sum_{k<jcur} y[k] V_k

@@ -795,7 +795,10 @@ static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doub
     if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, k.part, gd, j, m, ws.h, k.dH, k.dcs, k.dsn, k.dg);
   }
   ctx->kt.tag = -1;
-  k_hess_solve<<<1, 1, 0, st>>>(gd, m, k.dH, k.dg, k.dy);
+  if (m <= kHessLdsMaxM)
+    k_hess_solve_lds<<<1, 256, sizeof(double) * (size_t)((m + 1) * m + m), st>>>(gd, m, k.dH, k.dg, k.dy);
+  else
+    k_hess_solve<<<1, 1, 0, st>>>(gd, m, k.dH, k.dg, k.dy);
   // (booked at the solution's read and write only: the basis vectors it reads depend on the cycle's length)
   KT(ctx, st, "k_gemv_update", (double)N * 16, k_gemv_update_dev<TV, double><<<g256, 256, 0, st>>>(N, m, k.V, N, k.dy, gd, yvec));
   // the next cycle's start, w = M^-1 (b - J y) in FP64 and its norm; nothing
