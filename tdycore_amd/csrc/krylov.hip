@@ -410,14 +410,16 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32
 // and stop test.  Rotation i turns (entry i, entry i+1) into (final entry i,
 // the entry i+1 that rotation i+1 reads), so the running entry stays in a
 // register and nothing written is read back from memory.
-__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *__restrict__ h, double hj1,
+// hr, csr, snr: where h[0..j], cs[0..j-1] and sn[0..j-1] are read (the
+// arrays themselves, or copies in LDS); cs[j] and sn[j] are written to cs, sn.
+__device__ inline void arnoldi_update(GmresDev *gd, int j, int m, const double *hr, double hj1,
                                       double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
-                                      double *__restrict__ g) {
+                                      double *__restrict__ g, const double *csr, const double *snr) {
   const int ld = m + 1;
   const double hn = sqrt(hj1);
-  double a = h[0];
+  double a = hr[0];
   for (int i = 0; i < j; i++) {
-    const double c = h[i + 1], ci = cs[i], si = sn[i];
+    const double c = hr[i + 1], ci = csr[i], si = snr[i];
     H[i + j * ld] = ci * a + si * c;
     a = -si * a + ci * c;
   }
@@ -488,7 +490,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
     sc = (TV)(hn > 0.0 ? 1.0 / hn : 0.0);
     if (b == 0 && t == 0) {
       na.h[na.j + 1] = (double)s;
-      arnoldi_update(gd, na.j, na.m, na.h, (double)s, na.H, na.cs, na.sn, na.g);
+      arnoldi_update(gd, na.j, na.m, na.h, (double)s, na.H, na.cs, na.sn, na.g, na.cs, na.sn);
     }
   } else {
     if (stop && *stop) return;  // uniform over the workgroup
@@ -790,7 +792,7 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
 __global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
                                double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
   if (gd->done) return;
-  arnoldi_update(gd, j, m, h, h[j + 1], H, cs, sn, g);
+  arnoldi_update(gd, j, m, h, h[j + 1], H, cs, sn, g, cs, sn);
 }
 
 // h[j+1] = sum of the nchunk norm partials (fixed order, as k_reduce_partials)
@@ -801,13 +803,21 @@ __global__ void __launch_bounds__(kRedBlock) k_norm_arnoldi(int nchunk, const TV
                                                             double *__restrict__ H, double *__restrict__ cs,
                                                             double *__restrict__ sn, double *__restrict__ g) {
   __shared__ TV sh[kRedBlock / 64];
+  __shared__ double hs[kRedBlock], css[kRedBlock], sns[kRedBlock];  // h, cs, sn read by the update
   if (gd->done) return;
+  const int t = threadIdx.x;
+  const bool staged = j < kRedBlock;  // the columns fit: the update reads them from LDS
+  if (staged) {
+    if (t <= j) hs[t] = h[t];
+    if (t < j) { css[t] = cs[t]; sns[t] = sn[t]; }
+  }
   TV s = 0;
   s = strided_sum<kRedBlock>(partial, nchunk);
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) {
+  s = block_sum(s, sh);  // its barriers also order the LDS copies above
+  if (t == 0) {
     h[j + 1] = (double)s;
-    arnoldi_update(gd, j, m, h, (double)s, H, cs, sn, g);
+    if (staged) arnoldi_update(gd, j, m, hs, (double)s, H, cs, sn, g, css, sns);
+    else arnoldi_update(gd, j, m, h, (double)s, H, cs, sn, g, cs, sn);
   }
 }
 
