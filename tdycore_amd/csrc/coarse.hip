@@ -336,61 +336,126 @@ __global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
       if (r0 + a < n && c0 + b < n) C.A2[(int64_t)(r0 + a) * n + dst[c0 + b]] = m[a][b];
 }
 
-// Level 1 and 2 of an application, one workgroup per group g:
+// Level 1 and 2 of an application for group g (one workgroup of kL1Threads):
 //   z1[b] = (G1 rc1)[b] + (A2^-1 r2)[g]  for the blocks b of g,
 // with r2[h] the sum of rc1 over the blocks of group h.  Every workgroup
-// forms all of r2 itself (ng*gsz reads, in a fixed order), so one kernel
-// finishes the coarse solve and the prolongation only adds z1[block].
-// Every load is issued at the start (the group's G1 rows, its A2 row and
-// the member lists, then rc1 through them), so the kernel waits on memory
-// twice rather than once per loop trip; the sums keep their fixed order.
-constexpr int kL1Threads = 4 * kNG;
-constexpr int kL1Quarter = kGMax / 4;  // members per lane of an r2 sum
-__global__ void __launch_bounds__(kL1Threads) k_coarse_level1(CoarseLayout C, const int *__restrict__ stop) {
-  __shared__ double rg[kGMax];
+// forms all of r2 itself (ng*gsz reads, in a fixed order), so one launch
+// finishes the coarse solve and the prolongation only adds z1[block].  The
+// loads that do not depend on rc1 (this thread's G1 entries, its A2 entry)
+// go first, the member lists' loads eight at a time.  z1 of the group's
+// blocks is left in zs (LDS, zero past the members) and written to cd.z1;
+// the members in ms.  Every thread calls it.
+constexpr int kL1Threads = 256;
+constexpr int kL1Chunks = kL1Threads / kGMax;  // column chunks of the G1 product
+constexpr int kL1Cols = kGMax / kL1Chunks;     // columns per chunk
+static_assert(kL1Threads >= 2 * kNG, "two threads per group for r2");
+__device__ inline void coarse_z1(const CoarseDot &cd, int g, double *zs, int *ms) {
   __shared__ double r2s[kNG];
+  __shared__ double rg[kGMax];
+  __shared__ double zc[kL1Chunks][kGMax];
   __shared__ double sh[kL1Threads / 64];
-  if (stop && *stop) return;
-  const int g = blockIdx.x, t = threadIdx.x, G = C.gsz, ng = C.ng;
-  // this thread's row of the group's G1 (column-major: entry (t, j) at j*G + t)
-  double gv[kGMax];
-  const double *g1 = C.G1 + (int64_t)g * G * G + t;
+  const int t = threadIdx.x, G = cd.gsz, ng = cd.ng;
+  const int i = t % kGMax, ch = t / kGMax;  // row of G1, column chunk
+  double gv[kL1Cols];
+  const double *g1 = cd.G1 + (int64_t)g * G * G;  // column-major per group: entry (i, j) at j*G + i
 #pragma unroll
-  for (int j = 0; j < kGMax; j++) gv[j] = (t < G && j < G) ? g1[j * G] : 0.0;
-  const double a2 = t < ng ? C.A2[(int64_t)g * ng + t] : 0.0;
-  // r2[h]: lanes 4h..4h+3 each sum a quarter of group h's members, then
-  // ((q0 + q1) + (q2 + q3)) on all four lanes
-  const int h = t >> 2, qt = t & 3, QG = (G + 3) / 4;
-  const int m0 = qt * QG, nm = h < ng ? max(0, min(G, m0 + QG) - m0) : 0;
-  int bm[kL1Quarter];
-#pragma unroll
-  for (int k = 0; k < kL1Quarter; k++) bm[k] = k < nm ? C.gmem[h * G + m0 + k] : -1;
-  const int bt = t < G ? C.gmem[g * G + t] : -1;
-  double rv[kL1Quarter];
-#pragma unroll
-  for (int k = 0; k < kL1Quarter; k++) rv[k] = bm[k] >= 0 ? C.rc1[bm[k]] : 0.0;
-  const double rt = bt >= 0 ? C.rc1[bt] : 0.0;
-  {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kL1Quarter; k++)
-      if (bm[k] >= 0) s += rv[k];
-    const double o1 = __shfl_xor(s, 1, 64);
-    const double p = (t & 1) ? o1 + s : s + o1;
-    const double o2 = __shfl_xor(p, 2, 64);
-    if (h < ng && qt == 0) r2s[h] = p + o2;
+  for (int u = 0; u < kL1Cols; u++) {
+    const int j = ch * kL1Cols + u;
+    gv[u] = (i < G && j < G) ? g1[j * G + i] : 0.0;
   }
-  if (t < kGMax) rg[t] = rt;
+  const double a2 = t < ng ? cd.A2[(int64_t)g * ng + t] : 0.0;
+  // r2[h]: threads 2h and 2h+1 sum the two halves of group h's members in
+  // order, then half 0 + half 1
+  {
+    const int h = t >> 1, hf = t & 1, HG = (G + 1) / 2;
+    const int m0 = hf * HG, nm = h < ng ? max(0, min(G, m0 + HG) - m0) : 0;
+    double s = 0.0;
+    for (int k0 = 0; k0 < nm; k0 += 8) {
+      int bm[8];
+      double rv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) bm[u] = k0 + u < nm ? cd.gmem[h * G + m0 + k0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < 8; u++) rv[u] = bm[u] >= 0 ? cd.rc1[bm[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (bm[u] >= 0) s += rv[u];
+    }
+    const double o = __shfl_xor(s, 1, 64);
+    if (h < ng && hf == 0) r2s[h] = s + o;
+  }
+  const int bt = t < G ? cd.gmem[g * G + t] : -1;
+  if (t < kGMax) {
+    rg[t] = bt >= 0 ? cd.rc1[bt] : 0.0;
+    ms[t] = bt;
+  }
   __syncthreads();
   // z2 = row g of A2^-1 times r2
   const double z2 = block_sum_n<kL1Threads>(t < ng ? a2 * r2s[t] : 0.0, sh);
-  // padded members have zero rows, columns and rg
-  if (t < G) {
-    double z = 0.0;
+  // G1 rg by column chunks, the chunks added in order; padded members have
+  // zero rows, columns and rg
+  double z = 0.0;
 #pragma unroll
-    for (int j = 0; j < kGMax; j++)
-      if (j < G) z += gv[j] * rg[j];
-    if (bt >= 0) C.z1[bt] = z + z2;
+  for (int u = 0; u < kL1Cols; u++) z += gv[u] * rg[ch * kL1Cols + u];
+  zc[ch][i] = z;
+  __syncthreads();
+  if (t < kGMax) {
+    double zz = 0.0;
+    if (t < G) {
+      zz = zc[0][t];
+#pragma unroll
+      for (int c = 1; c < kL1Chunks; c++) zz += zc[c][t];
+      zz += z2;
+      if (bt >= 0) cd.z1[bt] = zz;
+    }
+    zs[t] = bt >= 0 ? zz : 0.0;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kL1Threads) k_coarse_level1(CoarseDot cd, const int *__restrict__ stop) {
+  __shared__ double zs[kGMax];
+  __shared__ int ms[kGMax];
+  if (stop && *stop) return;
+  coarse_z1(cd, blockIdx.x, zs, ms);
+}
+
+// In a multi-dot launch: z1 of group g as above, then the group's coarse
+// terms of the dots with the nv basis vectors.  V_v . (w_bj + P z1) splits
+// into V_v . w_bj (the row-chunk workgroups) and RV_v . z1 with RV_v the
+// block sums of V_v; this group adds sum over its blocks of RV_v[b] z1[b],
+// eight members per thread and the eight chunks in order, to partial[v *
+// pstride + g].
+template <class TV>
+__device__ void coarse_level1_dots(const CoarseDot &cd, int g, int nv, TV *partial, int pstride) {
+  static_assert(kL1Threads == kRedBlock, "the multi-dot's workgroup size");
+  __shared__ double zs[kGMax];
+  __shared__ int ms[kGMax];
+  __shared__ double cps[kGMax / 8][32];
+  coarse_z1(cd, g, zs, ms);
+  const int t = threadIdx.x, v = t % 32, c = t / 32;  // vector, chunk of eight members
+  for (int v0 = 0; v0 < nv; v0 += 32) {
+    double q = 0.0;
+    if (v0 + v < nv) {
+      const double *rv = cd.RV + (int64_t)(v0 + v) * cd.rv_ld;
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int b = ms[c * 8 + u];
+        x[u] = b >= 0 ? rv[b] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) q += x[u] * zs[c * 8 + u];
+    }
+    cps[c][v] = q;
+    __syncthreads();
+    if (t < 32 && v0 + t < nv) {
+      double p = cps[0][t];
+#pragma unroll
+      for (int k = 1; k < kGMax / 8; k++) p += cps[k][t];
+      partial[(int64_t)(v0 + t) * pstride + g] = (TV)p;
+    }
+    __syncthreads();
   }
 }
 

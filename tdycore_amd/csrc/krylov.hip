@@ -265,6 +265,35 @@ __device__ inline void restrict_block(const TV *x, double *rc1, int b, int t) {
   if (t == 0) rc1[b] = (double)s;
 }
 
+// The same restriction, and in the same pass the block sum of v (this
+// thread's entry of the basis vector) into rv[b]; rc1[b] is the value
+// restrict_block gives.
+template <class TV, int BS>
+__device__ inline void restrict_block2(const TV *x, double *rc1, TV v, double *rv, int b, int t) {
+  __shared__ TV sh[2][BS / 64];
+  TV a = x[t];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    v += __shfl_down(v, off, 64);
+  }
+  if ((t & 63) == 0) {
+    sh[0][t >> 6] = a;
+    sh[1][t >> 6] = v;
+  }
+  __syncthreads();
+  if (t == 0) {
+    TV s = 0, q = 0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; w++) {
+      s += sh[0][w];
+      q += sh[1][w];
+    }
+    rc1[b] = (double)s;
+    rv[b] = (double)q;
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- ILU(0) apply: y = (LU)^-1 r
 // Forward and backward sweeps over the block's levels with x (BS+1 entries,
 // x[BS] == 0 the target of padded slots) in LDS.  Every thread of the block
@@ -471,7 +500,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
                                                       GmresDev *__restrict__ gd, TV *__restrict__ vj,
                                                       const TV *__restrict__ fac, TV *__restrict__ out,
                                                       const int *__restrict__ stop, double *__restrict__ rc1,
-                                                      NormArgs na) {
+                                                      NormArgs na, double *__restrict__ rv) {
   __shared__ TV x[BS + 1];
   __shared__ TV shn[BS / 64];
   __shared__ int stop_s;
@@ -503,9 +532,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
   ecol_addr(L.d_pat, BS, W, p, eb, es);
   const int wb = L.d_bw ? L.d_bw[b] : W;  // slots past the block's longest row are padding
   x[t] = row >= 0 ? sc * blk_row<TV, TI, true>(((int64_t)b * W) * BS + t, BS, wb, row, ecol, eb, es, a, u) : (TV)0;
-  if (row >= 0) vj[row] = sc * u[row];
+  const TV vv = row >= 0 ? sc * u[row] : (TV)0;
+  if (row >= 0) vj[row] = vv;
   if (t == 0) x[BS] = (TV)0;
-  if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform
+  // rc1 and rv are uniform; rv: the block sums of V_j, for the coarse terms of the multi-dot
+  if (rc1 && rv) restrict_block2<TV, BS>(x, rc1, vv, rv, b, t);
+  else if (rc1) restrict_block<TV, BS>(x, rc1, b, t);
   bj_sweeps<TV, BS>(L, fac, x, b, t);
   if (row >= 0) out[row] = x[t];
 }
@@ -569,6 +601,20 @@ __device__ inline TV wave_reduce_scatter(TV (&v)[32]) {
   return r;
 }
 
+// The coarse correction's level-1/2 solve run by the first ng workgroups of
+// a multi-dot launch (ng = 0: none), with the group's terms of the dots.
+struct CoarseDot {
+  int ng = 0, gsz = 0;
+  const int32_t *gmem = nullptr;                // ng*gsz blocks of each group (-1 pad)
+  const double *G1 = nullptr, *A2 = nullptr;    // group inverses, inverted level-2 operator
+  const double *rc1 = nullptr;                  // restricted vector (one value per block)
+  const double *RV = nullptr;                   // block sums of the basis vectors, rv_ld per vector
+  int64_t rv_ld = 0;
+  double *z1 = nullptr;                         // coarse correction per block (out)
+};
+template <class TV>
+__device__ void coarse_level1_dots(const CoarseDot &cd, int g, int nv, TV *partial, int pstride);
+
 // Coarse prolongation applied on the fly: w[i] += z1[pos[i] >> bs_log2]
 // (pos = block-layout position of row i).  z1 == nullptr: none.
 struct Prolong {
@@ -603,12 +649,19 @@ __device__ inline void load2(const TV *__restrict__ p, int64_t r, int64_t n, boo
 template <class TV, int NS>
 __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                     TV *__restrict__ w, TV *__restrict__ partial, int nchunk,
-                                                    Prolong pro, const int *__restrict__ stop = nullptr) {
+                                                    Prolong pro, const int *__restrict__ stop = nullptr,
+                                                    CoarseDot cd = CoarseDot{}) {
   static_assert(Red<TV>::E == 2, "two rows per thread");
   __shared__ TV red[kRedBlock / 64][NS];
   if (stop && *stop) return;
+  // partial[j*nchunk + c]: c < cd.ng the coarse workgroups' terms, then one per row chunk
+  if ((int)blockIdx.x < cd.ng) {
+    coarse_level1_dots<TV>(cd, blockIdx.x, nv, partial, nchunk);
+    return;
+  }
+  const int chunk = blockIdx.x - cd.ng;
   const int t = threadIdx.x, lane = t & 63, wv_id = t >> 6;
-  const int64_t r0 = (int64_t)blockIdx.x * Red<TV>::chunk + 2 * t;
+  const int64_t r0 = (int64_t)chunk * Red<TV>::chunk + 2 * t;
   const bool vec = (ldv & 1) == 0;
   TV w0, w1;
   load2(w, r0, n, true, w0, w1);
@@ -635,7 +688,7 @@ __global__ void __launch_bounds__(kRedBlock) k_mdot(int64_t n, int nv, const TV 
       TV s = 0;
 #pragma unroll
       for (int q = 0; q < kRedBlock / 64; q++) s += red[q][t];
-      partial[(int64_t)(j0 + t) * nchunk + blockIdx.x] = s;
+      partial[(int64_t)(j0 + t) * nchunk + cd.ng + chunk] = s;
     }
     __syncthreads();
   }
@@ -660,13 +713,18 @@ template <class TV>
 __global__ void __launch_bounds__(kRedBlock) k_maxpy_norm(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv,
                                                           const double *__restrict__ h, TV *__restrict__ w,
                                                           TV *__restrict__ partial,
-                                                          const int *__restrict__ stop = nullptr) {
+                                                          const int *__restrict__ stop = nullptr,
+                                                          Prolong pro = Prolong{}) {
   __shared__ TV sh[kRedBlock / 64];
   if (stop && *stop) return;
   const int64_t r0 = (int64_t)blockIdx.x * Red<TV>::chunk + 2 * threadIdx.x;
   const bool vec = (ldv & 1) == 0;
   TV x0, x1;
   load2(w, r0, n, true, x0, x1);
+  if (pro.z1) {  // the coarse prolongation, when the multi-dot left it out
+    if (r0 < n) x0 += (TV)pro.z1[pro.pos[r0] >> pro.bs_log2];
+    if (r0 + 1 < n) x1 += (TV)pro.z1[pro.pos[r0 + 1] >> pro.bs_log2];
+  }
   for (int j = 0; j < nv; j++) {
     TV v0, v1;
     load2(V + (int64_t)j * ldv, r0, n, vec, v0, v1);

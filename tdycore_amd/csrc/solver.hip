@@ -370,6 +370,7 @@ struct SolverWS {
   void *V = nullptr, *w = nullptr, *z = nullptr, *b = nullptr, *x = nullptr, *Jk = nullptr, *fac = nullptr,
        *partial = nullptr;
   double *dpartial = nullptr, *h = nullptr;
+  double *RV = nullptr;  // (restart + 1) * nblocks: block sums of the basis vectors (coarse correction on)
   double *hs = nullptr;  // Hessenberg, Givens cs/sn, g, y of the device GMRES cycle
   GmresDev *gdev = nullptr;
   int *status = nullptr;
@@ -518,7 +519,7 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
   char **cp[] = {(char **)&ws.V, (char **)&ws.w, (char **)&ws.z, (char **)&ws.b, (char **)&ws.x, (char **)&ws.fac,
                  (char **)&ws.partial};
   size_t cnt[] = {(size_t)(o.ksp_restart + 1) * n, (size_t)n, (size_t)n, (size_t)n, (size_t)n,
-                  (size_t)hb.nblocks * kW * hb.bs, (size_t)(o.ksp_restart + 2) * ws.nchunk};
+                  (size_t)hb.nblocks * kW * hb.bs, (size_t)(o.ksp_restart + 2) * (ws.nchunk + kNG)};
   for (int i = 0; i < 7; i++)
     if ((rc = ws_alloc(ws, cp[i], cnt[i] * tv))) return rc;
   if ((rc = ws_alloc(ws, (float **)&ws.Jk, nnz))) return rc;  // FP32 copy of the Jacobian
@@ -544,7 +545,8 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
       if ((rc = ws_alloc(ws, &C.A1, (size_t)hc.nb * kCQ)) ||
           (rc = ws_alloc(ws, &C.G1, (size_t)hc.ng * hc.gsz * hc.gsz)) ||
           (rc = ws_alloc(ws, &C.A2, (size_t)hc.ng * hc.ng)) || (rc = ws_alloc(ws, &C.rc1, (size_t)hc.nb)) ||
-          (rc = ws_alloc(ws, &C.z1, (size_t)hc.nb)))
+          (rc = ws_alloc(ws, &C.z1, (size_t)hc.nb)) ||
+          (rc = ws_alloc(ws, &ws.RV, (size_t)(o.ksp_restart + 1) * hc.nb)))
         return rc;
       C.status = ws.status;
       ws.coarse = true;
@@ -647,10 +649,18 @@ static KBufs<TV> kbufs(tdyhip_ctx *ctx, int m) {
 }
 
 // level-1/2 solve of the coarse correction: z1 per block
+static CoarseDot coarse_dot(const SolverWS &ws) {
+  const CoarseLayout &C = ws.cl;
+  CoarseDot cd;
+  cd.ng = C.ng; cd.gsz = C.gsz; cd.gmem = C.gmem; cd.G1 = C.G1; cd.A2 = C.A2; cd.rc1 = C.rc1; cd.z1 = C.z1;
+  cd.RV = ws.RV; cd.rv_ld = C.nb;
+  return cd;
+}
+
 static void coarse_solve(tdyhip_ctx *ctx, const int *stp, hipStream_t st) {
   const CoarseLayout &C = ctx->ws->cl;
   KT(ctx, st, "k_coarse_level1", (double)C.ng * C.gsz * C.gsz * 8 + (double)C.nb * 24 + (double)C.ng * C.ng * 8,
-     k_coarse_level1<<<C.ng, kL1Threads, 0, st>>>(C, stp));
+     k_coarse_level1<<<C.ng, kL1Threads, 0, st>>>(coarse_dot(*ctx->ws), stp));
 }
 
 // out = M^-1 in (block-Jacobi ILU(0), plus the coarse correction when on)
@@ -674,7 +684,7 @@ static void pc_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *in, TV *out,
 // partials and the launch also finishes that step's Arnoldi column
 template <class TV>
 static void spmv_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *u, TV *vj, TV *out, const int *stp,
-                       const NormArgs *na, hipStream_t st) {
+                       const NormArgs *na, double *rv, hipStream_t st) {
   SolverWS &ws = *ctx->ws;
   const int W = ctx->jell_w;
   const double by = bytes_spmv(ctx, ws, sizeof(TV), ws.d_ecol16 != nullptr) + bytes_apply(ws, sizeof(TV));
@@ -686,11 +696,11 @@ static void spmv_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *u, TV *vj,
 #define TDY_SPMV(TI, EC, NRM)                                                                                      \
   do {                                                                                                             \
     if (ws.bj.bs == 64)                                                                                            \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, TI, NRM><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 64, TI, NRM><<<nb, 64, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv, rv)); \
     else if (ws.bj.bs == 256)                                                                                      \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, TI, NRM><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 256, TI, NRM><<<nb, 256, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv, rv)); \
     else                                                                                                           \
-      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, TI, NRM><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv)); \
+      KT(ctx, st, "k_bj_spmv_apply", by, k_bj_spmv_apply<TV, 512, TI, NRM><<<nb, 512, 0, st>>>(ws.bj, ws.NB, W, EC, Jk, u, gd, vj, fac, out, stp, rc1, nv, rv)); \
   } while (0)
   if (ws.d_ecol16) {
     if (na) TDY_SPMV(int16_t, ws.d_ecol16, true);
@@ -700,8 +710,8 @@ static void spmv_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *u, TV *vj,
     else TDY_SPMV(int32_t, ws.d_ecol, false);
   }
 #undef TDY_SPMV
-  // the prolongation of the coarse correction is added by the multi-dot that follows
-  if (ws.coarse) coarse_solve(ctx, stp, st);
+  // the coarse solve runs in the multi-dot launch that follows (its first
+  // workgroups), and the update kernel adds the prolongation
 }
 
 // ws.h[0] = ||v||^2
@@ -766,12 +776,21 @@ static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doub
   const unsigned g256 = (unsigned)((N + 255) / 256);
   GmresDev *gd = k.gd;
   const int *stop = &gd->done;
-  Prolong pro;  // coarse prolongation folded into the multi-dot
+  // With the coarse correction, w = w_bj + P z1 (w_bj the block-Jacobi part
+  // the fused matvec leaves): the multi-dot's first cd.ng workgroups solve for
+  // z1 and add the coarse terms RV_i . z1 of h_i = V_i . w (RV_i the block sums
+  // of V_i, formed by the matvec that formed V_i), the other workgroups the
+  // terms V_i . w_bj; the update kernel adds the prolongation P z1 to w.
+  Prolong pro;
+  CoarseDot cd;
   if (ws.coarse) {
     pro.z1 = ws.cl.z1;
     pro.pos = ws.d_perm;
     pro.bs_log2 = ws.bj.bs == 64 ? 6 : ws.bj.bs == 256 ? 8 : 9;
+    cd = coarse_dot(ws);
   }
+  const int pst = nch + cd.ng;  // partials per vector: the coarse groups', then the row chunks'
+  const unsigned gmd = (unsigned)pst;
   // The unnormalised Arnoldi vector alternates between w and z: step j reads
   // it from one and leaves the next in the other.
   for (int j = 0; j < m; j++) {
@@ -781,17 +800,20 @@ static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doub
     // timer tags: the launches of step j work only if the cycle reached
     // column j; a fused matvec also finishes column j-1, so it counts with it
     ctx->kt.tag = (fuse_norm && j > 0) ? j - 1 : j;
-    spmv_apply<TV>(ctx, k, u, k.V + (int64_t)j * N, ov, stop, (fuse_norm && j > 0) ? &na : nullptr, st);
+    spmv_apply<TV>(ctx, k, u, k.V + (int64_t)j * N, ov, stop, (fuse_norm && j > 0) ? &na : nullptr,
+                   ws.coarse ? ws.RV + (int64_t)j * ws.cl.nb : nullptr, st);
     ctx->kt.tag = j;
     // h = V^T o, o -= V h, ||o||^2, Givens update of column j.  (A single
     // kernel per reduction with a last-block hand-off is much slower here:
     // the device-scope fences it needs write back the per-XCD L2s.)
-    const double bmd = (double)(j + 2) * N * sizeof(TV) + (pro.z1 ? (double)N * (4 + sizeof(TV)) : 0.0);
-    if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
-    else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
-    else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, nch, pro, stop));
-    k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, nch, k.part, ws.h, stop);
-    KT(ctx, st, "k_maxpy_norm", (double)(j + 3) * N * sizeof(TV), k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ws.h, ov, k.part, stop));
+    const Prolong none;
+    const double bmd = (double)(j + 2) * N * sizeof(TV);
+    if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
+    else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
+    else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
+    k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, pst, k.part, ws.h, stop);
+    const double bmx = (double)(j + 3) * N * sizeof(TV) + (pro.z1 ? (double)N * 4 : 0.0);
+    KT(ctx, st, "k_maxpy_norm", bmx, k_maxpy_norm<TV><<<nch, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ws.h, ov, k.part, stop, pro));
     if (!fuse_norm || j == m - 1) k_norm_arnoldi<TV><<<1, kRedBlock, 0, st>>>(nch, k.part, gd, j, m, ws.h, k.dH, k.dcs, k.dsn, k.dg);
   }
   ctx->kt.tag = -1;
