@@ -126,12 +126,25 @@ __device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col
     for (int j = t; j < n; j += nt) M[k * ld + j] = (j == k) ? inv : M[k * ld + j] * inv;
     for (int i = t; i < n; i += nt) col[i] = (i == k) ? 0.0 : M[i * ld + k];
     __syncthreads();
-    for (int i = t >> 6; i < n; i += nt >> 6) {
-      if (i == k) continue;
-      const double ci = col[i];
-      for (int j = t & 63; j < n; j += 64) {
-        const double a = (j == k) ? 0.0 : M[i * ld + j];
-        M[i * ld + j] = a - ci * M[k * ld + j];
+    // rows i != k: M[i][j] = (j == k ? 0 : M[i][j]) - col[i] * M[k][j]; a
+    // thread's column over eight rows at a time, all eight read before any is
+    // written (entries are independent; row k is not written here)
+    const int rs = nt >> 6;
+    for (int j = t & 63; j < n; j += 64) {
+      const double mk = M[k * ld + j];
+      for (int i0 = t >> 6; i0 < n; i0 += 8 * rs) {
+        double a[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = i0 + u * rs;
+          a[u] = (i < n && j != k) ? M[i * ld + j] : 0.0;
+          c[u] = i < n ? col[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = i0 + u * rs;
+          if (i < n && i != k) M[i * ld + j] = a[u] - c[u] * mk;
+        }
       }
     }
     __syncthreads();
