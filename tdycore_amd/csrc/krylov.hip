@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t NB, int W, const int32
   if (row < 0) return;
   int64_t eb, es;
   ecol_addr(pat, bs, W, p, eb, es);
-  y[row] = blk_row<TV, TI>(ell_base(p, bs, W), bs, bw ? bw[p / bs] : W, row, ecol, eb, es, a, x);
+  y[row] = blk_row<TV, TI, true>(ell_base(p, bs, W), bs, bw ? bw[p / bs] : W, row, ecol, eb, es, a, x);
 }
 
 // Arnoldi column j: Hessenberg entries from h[0..j] (dots) and hj1 = h[j+1]
