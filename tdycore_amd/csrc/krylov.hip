@@ -542,6 +542,32 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_
   if (row >= 0) out[row] = x[t];
 }
 
+// A restart's preconditioned residual out = M_BJ^-1 (b - J y) in one pass:
+// the block's rows of b - J y (FP64, the difference taken in FP64 and then
+// rounded to TV, as a separate product and difference would give) go into
+// LDS, restricted for the coarse level when rc1 is set, and the sweeps follow.
+template <class TV, int BS, class TI>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_bj_resid_apply(BJLayout L, int W, const TI *__restrict__ ecol,
+                                                       const double *__restrict__ J, const double *__restrict__ y,
+                                                       const double *__restrict__ bvec, const TV *__restrict__ fac,
+                                                       TV *__restrict__ out, const int *__restrict__ stop,
+                                                       double *__restrict__ rc1) {
+  __shared__ TV x[BS + 1];
+  if (stop && *stop) return;  // uniform over the workgroup
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int64_t p = (int64_t)b * BS + t;
+  const int row = L.d_row[p];
+  int64_t eb, es;
+  ecol_addr(L.d_pat, BS, W, p, eb, es);
+  const int wb = L.d_bw ? L.d_bw[b] : W;
+  x[t] = row >= 0 ? (TV)(bvec[row] - blk_row<double, TI, true>(((int64_t)b * W) * BS + t, BS, wb, row, ecol, eb, es, J, y))
+                  : (TV)0;
+  if (t == 0) x[BS] = (TV)0;
+  if (rc1) restrict_block<TV, BS>(x, rc1, b, t);  // rc1 is uniform
+  bj_sweeps<TV, BS>(L, fac, x, b, t);
+  if (row >= 0) out[row] = x[t];
+}
+
 // ---------------------------------------------------------------- BLAS-1 with deterministic reductions
 constexpr int kRedBlock = 256;
 constexpr int kRedPerThread = 2;

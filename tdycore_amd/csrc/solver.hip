@@ -679,6 +679,37 @@ static void pc_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const TV *in, TV *out,
   else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
 }
 
+// out = M^-1 (b - J y) for a restart: the block-Jacobi part in one fused
+// pass (k_bj_resid_apply), then the coarse correction as in pc_apply
+template <class TV>
+static void resid_pc_apply(tdyhip_ctx *ctx, const KBufs<TV> &k, const double *bvec, const double *y, TV *out,
+                           const int *stp, hipStream_t st) {
+  SolverWS &ws = *ctx->ws;
+  const int W = ctx->jell_w;
+  const bool i16 = ws.d_ecol16 != nullptr;
+  const double by = bytes_spmv(ctx, ws, sizeof(double), i16) + bytes_apply(ws, sizeof(TV)) + (double)ws.n * 8;
+  const unsigned nb = ws.bj.nblocks;
+#define TDY_RESID(TI, EC)                                                                                           \
+  do {                                                                                                             \
+    if (ws.bj.bs == 64)                                                                                            \
+      KT(ctx, st, "k_bj_resid_apply", by, k_bj_resid_apply<TV, 64, TI><<<nb, 64, 0, st>>>(ws.bj, W, EC, ws.J, y, bvec, k.fac, out, stp, k.rc1)); \
+    else if (ws.bj.bs == 256)                                                                                      \
+      KT(ctx, st, "k_bj_resid_apply", by, k_bj_resid_apply<TV, 256, TI><<<nb, 256, 0, st>>>(ws.bj, W, EC, ws.J, y, bvec, k.fac, out, stp, k.rc1)); \
+    else                                                                                                           \
+      KT(ctx, st, "k_bj_resid_apply", by, k_bj_resid_apply<TV, 512, TI><<<nb, 512, 0, st>>>(ws.bj, W, EC, ws.J, y, bvec, k.fac, out, stp, k.rc1)); \
+  } while (0)
+  if (i16) TDY_RESID(int16_t, ws.d_ecol16);
+  else TDY_RESID(int32_t, ws.d_ecol);
+#undef TDY_RESID
+  if (!ws.coarse) return;
+  const CoarseLayout &C = ws.cl;
+  coarse_solve(ctx, stp, st);
+  const double bpro = (double)ws.NB * (4 + 2 * sizeof(TV)) + (double)C.nb * 8;
+  if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 64><<<C.nb, 64, 0, st>>>(C, ws.bj.d_row, out, stp));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 256><<<C.nb, 256, 0, st>>>(C, ws.bj.d_row, out, stp));
+  else KT(ctx, st, "k_coarse_prolong", bpro, k_coarse_prolong<TV, 512><<<C.nb, 512, 0, st>>>(C, ws.bj.d_row, out, stp));
+}
+
 // V_j = u / ||u|| and out = M^-1 A V_j in one pass (block-ordered SpMV fused
 // into the ILU sweeps); with na, ||u|| comes from the previous step's norm
 // partials and the launch also finishes that step's Arnoldi column
@@ -827,9 +858,7 @@ static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doub
   // once the solve has finished
   const int *fin = &gd->finished;
   ctx->kt.tag = kTagRestart;
-  spmv_blk64(ctx, ws, yvec, ws.JY, st, fin);
-  KT(ctx, st, "k_sub_to", (double)N * (16 + sizeof(TV)), k_sub_to<TV><<<g256, 256, 0, st>>>(N, bvec, ws.JY, k.z, fin));
-  pc_apply<TV>(ctx, k, k.z, k.w, fin, st);
+  resid_pc_apply<TV>(ctx, k, bvec, yvec, k.w, fin, st);
   norm2_to_h0<TV>(ctx, k.w, fin, st);
   ctx->kt.tag = -1;
   k_cycle_start<<<1, 1, 0, st>>>(gd, ws.h, k.dg, m, o.ksp_rtol, o.ksp_atol, 0);
