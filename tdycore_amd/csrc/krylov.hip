@@ -794,17 +794,6 @@ __global__ void k_scale_copy(int64_t n, TV a, const TV *__restrict__ x, TV *__re
   if (i < n) y[i] = a * x[i];
 }
 
-// x += sum_j c[j] V_j
-template <class TV, class TX>
-__global__ void k_gemv_update(int64_t n, int nv, const TV *__restrict__ V, int64_t ldv, const double *__restrict__ c,
-                              TX *__restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  TV s = 0;
-  for (int j = 0; j < nv; j++) s += (TV)c[j] * V[j * ldv + i];
-  x[i] += (TX)s;
-}
-
 template <class TA, class TB>
 __global__ void k_convert(int64_t n, const TA *__restrict__ a, TB *__restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -871,12 +860,6 @@ __global__ void k_cycle_start(GmresDev *gd, const double *__restrict__ h, double
   if (bad) gd->diverged = 1;
   gd->finished = conv || bad || gd->its >= gd->maxit;
   gd->done = gd->finished;
-}
-
-__global__ void k_arnoldi_step(GmresDev *gd, int j, int m, const double *__restrict__ h, double *__restrict__ H,
-                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g) {
-  if (gd->done) return;
-  arnoldi_update(gd, j, m, h, h[j + 1], H, cs, sn, g, cs, sn);
 }
 
 // h[j+1] = sum of the nchunk norm partials (fixed order, as k_reduce_partials)
@@ -949,15 +932,6 @@ __global__ void k_gemv_update_dev(int64_t n, int mmax, const TV *__restrict__ V,
   TV s = 0;
   for (int j = 0; j < nv; j++) s += (TV)c[j] * V[j * ldv + i];
   x[i] += (TX)s;
-}
-
-// out = (TV)(a - b), the difference taken in FP64
-template <class TV>
-__global__ void k_sub_to(int64_t n, const double *__restrict__ a, const double *__restrict__ b, TV *__restrict__ out,
-                         const int *__restrict__ stop = nullptr) {
-  if (stop && *stop) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (TV)(a[i] - b[i]);
 }
 
 // w = x - lambda*y
