@@ -839,8 +839,9 @@ static void gmres_cycle(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doub
     // the device-scope fences it needs write back the per-XCD L2s.)
     const Prolong none;
     const double bmd = (double)(j + 2) * N * sizeof(TV);
+    // FP32: two passes of 16 beyond 16 vectors (67 VGPRs against 83 for one of 32)
     if (j < 8) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 8><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
-    else if (j < 16) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
+    else if (j < 16 || sizeof(TV) == 4) KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 16><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
     else KT(ctx, st, "k_mdot", bmd, k_mdot<TV, 32><<<gmd, kRedBlock, 0, st>>>(N, j + 1, k.V, N, ov, k.part, pst, none, stop, cd));
     k_reduce_partials<TV><<<j + 1, kRedBlock, 0, st>>>(j + 1, pst, k.part, ws.h, stop);
     const double bmx = (double)(j + 3) * N * sizeof(TV) + (pro.z1 ? (double)N * 4 : 0.0);
