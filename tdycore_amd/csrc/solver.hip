@@ -385,6 +385,10 @@ struct SolverWS {
   double lu_mean = 0.0;          // mean per-block L/U slot count the sweeps read
   double w_mean = 0.0;           // mean per-block ELL slot count the SpMV reads
   CoarseLayout cl;
+  // the coarse setup's own stream and its fork/join events: it runs beside
+  // the ILU(0) factor, as both only read the Jacobian
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // one GMRES restart cycle captured as a graph, per Krylov precision
   // (index 0 FP64, 1 FP32); its kernels only touch workspace arrays
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
@@ -408,6 +412,10 @@ static void ws_free(SolverWS &ws) {
   for (hipGraphExec_t &e : ws.cycle_exec)
     if (e) (void)hipGraphExecDestroy(e);
   for (auto &e : ws.newton_graphs) (void)hipGraphExecDestroy(e.exec);
+  if (ws.side) (void)hipStreamSynchronize(ws.side);
+  if (ws.fork_ev) (void)hipEventDestroy(ws.fork_ev);
+  if (ws.join_ev) (void)hipEventDestroy(ws.join_ev);
+  if (ws.side) (void)hipStreamDestroy(ws.side);
   for (void *p : ws.allocs) (void)hipFree(p);
   ws.allocs.clear();
   ws = SolverWS();
@@ -549,6 +557,12 @@ static int ensure_ws(tdyhip_ctx *ctx, const tdyhip_solver_opts &o) {
           (rc = ws_alloc(ws, &ws.RV, (size_t)(o.ksp_restart + 1) * hc.nb)))
         return rc;
       C.status = ws.status;
+      if (hipStreamCreateWithFlags(&ws.side, hipStreamNonBlocking) != hipSuccess) ws.side = nullptr;
+      if (!ws.side || hipEventCreateWithFlags(&ws.fork_ev, hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&ws.join_ev, hipEventDisableTiming) != hipSuccess) {
+        set_error("hipStreamCreate / hipEventCreate (coarse setup stream) failed");
+        return TDYHIP_ERR_HIP;
+      }
       ws.coarse = true;
     }
   }
@@ -763,22 +777,27 @@ static void gmres_prep(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doubl
     KT(ctx, st, "k_convert_J", (double)ws.nnz * (8 + sizeof(TV)),
        k_convert<double, TV><<<(unsigned)((ws.nnz + 255) / 256), 256, 0, st>>>(ws.nnz, ws.J, k.Jk));
   k_gmres_init<<<1, 1, 0, st>>>(k.gd, ws.status, o.ksp_max_it);
-  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
-  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
-  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, FactorLanes<512>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
   // coarse correction: Galerkin rows, group inverses and the level-2 inverse
   // for this Jacobian; each application then adds its prolongation to the
-  // block-Jacobi result (the restriction is fused into the ILU sweeps)
+  // block-Jacobi result (the restriction is fused into the ILU sweeps).  Its
+  // latency-bound kernels run on the side stream, beside the factor, and
+  // join before the first preconditioner application.
+  const bool side = ws.coarse && hipEventRecord(ws.fork_ev, st) == hipSuccess &&
+                    hipStreamWaitEvent(ws.side, ws.fork_ev, 0) == hipSuccess;
   if (ws.coarse) {
+    const hipStream_t cs = side ? ws.side : st;
     const CoarseLayout &C = ws.cl;
     const int W = ctx->jell_w;
     const double brows = (double)ws.NB * W * (8 + 1) + (double)C.nb * kCQ * (4 + 8);
-    if (ws.bj.bs == 64) KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<64><<<C.nb, 64, 0, st>>>(C, ws.NB, W, ws.J));
-    else if (ws.bj.bs == 256) KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<256><<<C.nb, 256, 0, st>>>(C, ws.NB, W, ws.J));
-    else KT(ctx, st, "k_coarse_rows", brows, k_coarse_rows<512><<<C.nb, 512, 0, st>>>(C, ws.NB, W, ws.J));
-    KT(ctx, st, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, st>>>(C));
-    KT(ctx, st, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, st>>>(C));
+    if (ws.bj.bs == 64) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<64><<<C.nb, 64, 0, cs>>>(C, ws.NB, W, ws.J));
+    else if (ws.bj.bs == 256) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<256><<<C.nb, 256, 0, cs>>>(C, ws.NB, W, ws.J));
+    else KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<512><<<C.nb, 512, 0, cs>>>(C, ws.NB, W, ws.J));
+    KT(ctx, cs, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, cs>>>(C));
+    KT(ctx, cs, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, cs>>>(C));
   }
+  if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
+  else if (ws.bj.bs == 256) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 256><<<ws.bj.nblocks, FactorLanes<256>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
+  else KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 512><<<ws.bj.nblocks, FactorLanes<512>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
   k_convert<double, TV><<<g256, 256, 0, st>>>(k.N, bvec, k.b);
   // The solution accumulates in FP64 in place (yvec) in both precisions, and
   // every restart recomputes the true residual b - J y with the FP64 Jacobian
@@ -786,6 +805,10 @@ static void gmres_prep(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doubl
   // solve for a correction, so the FP32 form stops on the same residual as
   // the FP64 one.
   k_scale_copy<double><<<g256, 256, 0, st>>>(k.N, 0.0, bvec, yvec);
+  if (side && (hipEventRecord(ws.join_ev, ws.side) != hipSuccess || hipStreamWaitEvent(st, ws.join_ev, 0) != hipSuccess)) {
+    set_error("coarse setup join failed");
+    (void)hipStreamSynchronize(ws.side);  // outside a capture this still orders it; a capture fails to end
+  }
   pc_apply<TV>(ctx, k, k.b, k.w, nullptr, st);  // r0 = M^-1 b  (y0 = 0)
   norm2_to_h0<TV>(ctx, k.w, nullptr, st);
   k_cycle_start<<<1, 1, 0, st>>>(k.gd, ws.h, k.dg, k.m, o.ksp_rtol, o.ksp_atol, 1);
