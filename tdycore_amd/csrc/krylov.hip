@@ -89,11 +89,15 @@ struct FactorLanes {
   static constexpr int TPR = BS <= 256 ? 4 : 2;  // lanes per row (workgroup <= 1024 threads)
   static constexpr int NT = BS * TPR;
   static constexpr int WPL = (kPairW + TPR - 1) / TPR;  // pair words per lane
-  static constexpr int AHEAD = 3;                        // lower slots whose update lists are in flight
+  static constexpr int AHEAD = 2;                        // lower slots whose update lists are in flight
+  // waves per SIMD the registers are budgeted for (at most 64 VGPRs at 8):
+  // two 16-wave workgroups per CU, as many as the LDS holds at BS <= 256
+  static constexpr int WAVES = BS <= 256 ? 8 : 4;
 };
 
 template <class TV, int BS>
-__global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, const double *__restrict__ J,
+__global__ void __launch_bounds__(FactorLanes<BS>::NT) __attribute__((amdgpu_waves_per_eu(FactorLanes<BS>::WAVES)))
+k_bj_factor(BJLayout L, const double *__restrict__ J,
                                                                    TV *__restrict__ fac, int *__restrict__ status) {
   constexpr int TPR = FactorLanes<BS>::TPR, NT = FactorLanes<BS>::NT, WPL = FactorLanes<BS>::WPL;
   __shared__ TV a[kW][BS];
@@ -139,21 +143,34 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
   // The update lists of lower slots p+1 .. p+AHEAD (count and this lane's
   // packed pair words) are in flight while slot p is eliminated, the first
   // ones fetched before the level loop, so global-memory latency is off the
-  // elimination chain.
-  const int64_t pbase = (int64_t)pb * kPair * BS;
-  const int64_t nbase = (int64_t)pb * kLU * BS;
-  auto fetch = [&](int p, int &cnt, uint32_t (&w)[WPL]) {
-    cnt = (p < di0) ? L.d_npair[nbase + (int64_t)p * BS + t] : 0;
+  // elimination chain.  The slot loop is unrolled and the loads carry no
+  // condition (the tables hold kLU slots per row; a lane's words past the
+  // row's kPairW re-read the last one and are never used), so each list
+  // lands in registers of its own and the wait before slot p is for slot p's
+  // loads alone.  (A rotating queue in a rolled loop copies registers whose
+  // loads are pending, which waits for every load at every slot.  At 1M
+  // cells the FP32 factor went from 1579 to 1372 us; FP64, 1709 -> 1744 us,
+  // is held by its LDS chain instead.)
+  // A slot's table rows are addressed as a uniform base plus this lane's
+  // 32-bit offset.  The base takes an opaque zero made where it is used, so
+  // the compiler cannot hoist the thirteen slots' addresses out of the level
+  // loop, where they would hold 64-bit addresses in vector registers.
+  const uint32_t *__restrict__ pairs = L.d_pair + (int64_t)pb * kPair * BS;
+  const uint8_t *__restrict__ npair = L.d_npair + (int64_t)pb * kLU * BS;
+  uint32_t woff[WPL];
 #pragma unroll
-    for (int i = 0; i < WPL; i++) {
-      const int wq = sub + TPR * i;
-      w[i] = (p < di0 && wq < kPairW) ? L.d_pair[pbase + (int64_t)(p * kPairW + wq) * BS + t] : 0u;
-    }
+  for (int i = 0; i < WPL; i++) woff[i] = (uint32_t)(min(sub + TPR * i, kPairW - 1) * BS + t);
+  auto fetch = [&](int p, int &cnt, uint32_t (&w)[WPL]) {
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    cnt = (npair + (p * BS + z))[(uint32_t)t];
+    const uint32_t *__restrict__ pp = pairs + (p * kPairW * BS + z);
+#pragma unroll
+    for (int i = 0; i < WPL; i++) w[i] = pp[woff[i]];
   };
-  // update lists of the next kAhead lower slots in flight (a rotating queue)
   constexpr int kAhead = FactorLanes<BS>::AHEAD;
-  int nq[kAhead];
-  uint32_t qw[kAhead][WPL];
+  int nq[kLU];
+  uint32_t qw[kLU][WPL];
 #pragma unroll
   for (int d = 0; d < kAhead; d++) fetch(d, nq[d], qw[d]);
   __syncthreads();
@@ -164,50 +181,53 @@ __global__ void __launch_bounds__(FactorLanes<BS>::NT) k_bj_factor(BJLayout L, c
       // the pair targets in row t (distinct slots above p, so no read of the
       // batch waits for a write of the same slot), and the next slot's
       // column; the next column's diagonal slot is read behind the writes.
+      // (the first lists pass through an empty asm here, so nothing derived
+      // from them is hoisted out of the level loop into spilled registers)
+#pragma unroll
+      for (int d = 0; d < kAhead; d++) {
+        asm volatile("" : "+v"(nq[d]));
+#pragma unroll
+        for (int i = 0; i < WPL; i++) asm volatile("" : "+v"(qw[d][i]));
+      }
       int k = di0 > 0 ? lc[0][t] : t;
       int dk = dg[k];
-      for (int p = 0; p < di0; p++) {  // lower entries, increasing column
-        const int ncur = nq[0];
-        uint32_t cur[WPL];
 #pragma unroll
-        for (int i = 0; i < WPL; i++) cur[i] = qw[0][i];
+      for (int p = 0; p < kLU; p++) {  // lower entries, increasing column
+        if (p + kAhead < kLU) fetch(p + kAhead, nq[p + kAhead], qw[p + kAhead]);
+        if (p < di0) {
+          const int ncur = nq[p];
+          const uint32_t *cur = qw[p];
+          TV src[2 * WPL], tgt[2 * WPL];
 #pragma unroll
-        for (int d = 0; d + 1 < kAhead; d++) {
-          nq[d] = nq[d + 1];
+          for (int i = 0; i < WPL; i++) {
 #pragma unroll
-          for (int i = 0; i < WPL; i++) qw[d][i] = qw[d + 1][i];
-        }
-        fetch(p + kAhead, nq[kAhead - 1], qw[kAhead - 1]);
-        TV src[2 * WPL], tgt[2 * WPL];
-#pragma unroll
-        for (int i = 0; i < WPL; i++) {
-#pragma unroll
-          for (int h = 0; h < 2; h++) {
-            const int q = 2 * (sub + TPR * i) + h;  // pair index of this lane's half-word
-            const uint32_t w = (q < ncur) ? (h ? (cur[i] >> 16) : (cur[i] & 0xffffu)) : 0u;  // unused pairs read slot 0
-            src[2 * i + h] = a[w >> 8][k];
-            tgt[2 * i + h] = a[w & 0xff][t];
+            for (int h = 0; h < 2; h++) {
+              const int q = 2 * (sub + TPR * i) + h;  // pair index of this lane's half-word
+              const uint32_t w = (q < ncur) ? (h ? (cur[i] >> 16) : (cur[i] & 0xffffu)) : 0u;  // unused pairs read slot 0
+              src[2 * i + h] = a[w >> 8][k];
+              tgt[2 * i + h] = a[w & 0xff][t];
+            }
           }
-        }
-        const TV pv = a[dk][k];
-        const TV ap = a[p][t];
-        const int kn = (p + 1 < di0) ? lc[p + 1][t] : k;
-        const TV lik = ap / pv;
-        __builtin_amdgcn_wave_barrier();  // every lane has read a[p][t] before it is overwritten
-        if (sub == 0) a[p][t] = lik;
+          const TV pv = a[dk][k];
+          const TV ap = a[p][t];
+          const int kn = (p + 1 < di0) ? lc[p + 1][t] : k;
+          const TV lik = ap / pv;
+          __builtin_amdgcn_wave_barrier();  // every lane has read a[p][t] before it is overwritten
+          if (sub == 0) a[p][t] = lik;
 #pragma unroll
-        for (int i = 0; i < WPL; i++) {
+          for (int i = 0; i < WPL; i++) {
 #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            const int q = 2 * (sub + TPR * i) + h;
-            const uint32_t w = h ? (cur[i] >> 16) : (cur[i] & 0xffffu);
-            if (q < ncur) a[w & 0xff][t] = tgt[2 * i + h] - lik * src[2 * i + h];
+            for (int h = 0; h < 2; h++) {
+              const int q = 2 * (sub + TPR * i) + h;
+              const uint32_t w = h ? (cur[i] >> 16) : (cur[i] & 0xffffu);
+              if (q < ncur) a[w & 0xff][t] = tgt[2 * i + h] - lik * src[2 * i + h];
+            }
           }
+          const int dkn = dg[kn];
+          __builtin_amdgcn_wave_barrier();
+          k = kn;
+          dk = dkn;
         }
-        const int dkn = dg[kn];
-        __builtin_amdgcn_wave_barrier();
-        k = kn;
-        dk = dkn;
       }
       if (sub == 0 && a[dg[t]][t] == (TV)0) atomicOr(status, 1);
     }
