@@ -781,18 +781,26 @@ static void gmres_prep(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doubl
   // for this Jacobian; each application then adds its prolongation to the
   // block-Jacobi result (the restriction is fused into the ILU sweeps).  Its
   // latency-bound kernels run on the side stream, beside the factor, and
-  // join before the first preconditioner application.
-  const bool side = ws.coarse && hipEventRecord(ws.fork_ev, st) == hipSuccess &&
-                    hipStreamWaitEvent(ws.side, ws.fork_ev, 0) == hipSuccess;
-  if (ws.coarse) {
-    const hipStream_t cs = side ? ws.side : st;
-    const CoarseLayout &C = ws.cl;
+  // join before the first preconditioner application.  With the FP64
+  // factor, whose workgroups leave no CU room for the group inverses' LDS,
+  // the rows and group inverses run first and only the one-workgroup level-2
+  // inverse goes beside it.
+  const CoarseLayout &C = ws.cl;
+  auto rows_groups = [&](hipStream_t cs) {
     const int W = ctx->jell_w;
     const double brows = (double)ws.NB * W * (8 + 1) + (double)C.nb * kCQ * (4 + 8);
     if (ws.bj.bs == 64) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<64><<<C.nb, 64, 0, cs>>>(C, ws.NB, W, ws.J));
     else if (ws.bj.bs == 256) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<256><<<C.nb, 256, 0, cs>>>(C, ws.NB, W, ws.J));
     else KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<512><<<C.nb, 512, 0, cs>>>(C, ws.NB, W, ws.J));
     KT(ctx, cs, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, cs>>>(C));
+  };
+  const bool early = ws.coarse && sizeof(TV) == 8;
+  if (early) rows_groups(st);
+  const bool side = ws.coarse && hipEventRecord(ws.fork_ev, st) == hipSuccess &&
+                    hipStreamWaitEvent(ws.side, ws.fork_ev, 0) == hipSuccess;
+  if (ws.coarse) {
+    const hipStream_t cs = side ? ws.side : st;
+    if (!early) rows_groups(cs);
     KT(ctx, cs, "k_coarse_invert", (double)C.ng * C.ng * 16, k_coarse_invert<<<1, 1024, 0, cs>>>(C));
   }
   if (ws.bj.bs == 64) KT(ctx, st, "k_bj_factor", bytes_factor(ws, sizeof(TV)), k_bj_factor<TV, 64><<<ws.bj.nblocks, FactorLanes<64>::NT, 0, st>>>(ws.bj, ws.J, k.fac, ws.status));
