@@ -81,100 +81,131 @@ __global__ void __launch_bounds__(BS) k_coarse_rows(CoarseLayout C, int64_t NB, 
   }
 }
 
-// In-place Gauss-Jordan inverse of the n x n matrix M (row-major in LDS,
-// leading dimension ld) with row pivoting; all threads of the workgroup
-// call it (blockDim a multiple of 64, n <= 128).  piv and col are LDS
-// scratch of n entries.  Returns false if a pivot is zero.  The pivot of
-// column k is the first row i >= k of largest |M[i][k]|, found by the first
-// wavefront; the elimination runs one row per 64-lane group.
-__device__ inline bool gj_invert(double *M, int n, int ld, int *piv, double *col, int *flag) {
-  const int t = threadIdx.x, nt = blockDim.x;
-  if (t == 0) *flag = 0;
-  __syncthreads();
+// Gauss-Jordan inverse with row pivoting of an n x n matrix (n <= N), held
+// as 4x4 register tiles, one per thread of an NT = (N/4)^2-thread
+// workgroup: thread t holds rows 4*(t/(N/4)).. and columns 4*(t%(N/4))..
+// (zero outside n x n).  The pivot of column k is the first row i >= k of
+// largest |M[i][k]|; row k is swapped with it and scaled,
+//   M[k][j] = (j == k ? 1 : M[k][j]) / M[k][k] (as inv or M[k][j] * inv),
+// and every other row becomes M[i][j] = (j == k ? 0 : M[i][j]) - M[i][k] M[k][j];
+// each step publishes the pivot row, the row it replaces and the pivot
+// column through LDS (s).  On true, column c < n of the tiles is column
+// s.dst[c] of the inverse.  false (for every thread): a zero or NaN pivot.
+template <int N>
+struct TiledGJ {
+  static constexpr int TB = 4, NTC = N / TB, NT = NTC * NTC;
+  static_assert(NTC <= 32, "the pivot search reduces over one wavefront");
+  double rowp[N], rowk[N], colk[N];
+  double cv[NTC];
+  int ci[NTC];
+  int piv[N], src[N], dst[N];
+  int flag, pk;
+  double dkk;
+};
+
+template <int N>
+__device__ inline bool gj_invert_tiles(double (&m)[4][4], int n, TiledGJ<N> &s) {
+  constexpr int TB = TiledGJ<N>::TB, NTC = TiledGJ<N>::NTC;
+  const int t = threadIdx.x;
+  const int rb = t / NTC, cb = t % NTC, r0 = rb * TB, c0 = cb * TB;
+  if (t == 0) s.flag = 0;
   for (int k = 0; k < n; k++) {
-    if (t < 64) {
+    const int kb = k / TB, ka = k % TB;
+    // pivot of column k: the first row i >= k of largest |M[i][k]|
+    if (cb == kb) {
       double mx = -1.0;
       int p = n;
-      for (int i = k + t; i < n; i += 64) {
-        const double a = fabs(M[i * ld + k]);
-        if (a > mx) { mx = a; p = i; }
-      }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
+      for (int a = 0; a < TB; a++) {
+        const int i = r0 + a;
+        const double v = fabs(m[a][ka]);
+        if (i >= k && i < n && v > mx) { mx = v; p = i; }
+      }
+      s.cv[rb] = mx;
+      s.ci[rb] = p;
+      if (rb == kb) s.dkk = m[ka][ka];
+    }
+    __syncthreads();
+    if (t < 64) {
+      double mx = t < NTC ? s.cv[t] : -1.0;
+      int p = t < NTC ? s.ci[t] : n;
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) {
         const double om = __shfl_xor(mx, off, 64);
         const int op = __shfl_xor(p, off, 64);
         if (om > mx || (om == mx && op < p)) { mx = om; p = op; }
       }
       if (t == 0) {
-        piv[k] = p;
-        const double d = M[k * ld + k];
-        if (!(mx > 0.0) || d != d) *flag = 1;
+        s.piv[k] = p;
+        s.pk = p;
+        if (!(mx > 0.0) || s.dkk != s.dkk) s.flag = 1;
       }
     }
     __syncthreads();
-    if (*flag) return false;
-    const int p = piv[k];
-    if (p != k)
-      for (int j = t; j < n; j += nt) {
-        const double a = M[k * ld + j];
-        M[k * ld + j] = M[p * ld + j];
-        M[p * ld + j] = a;
-      }
-    __syncthreads();
-    const double inv = 1.0 / M[k * ld + k];
-    __syncthreads();
-    for (int j = t; j < n; j += nt) M[k * ld + j] = (j == k) ? inv : M[k * ld + j] * inv;
-    for (int i = t; i < n; i += nt) col[i] = (i == k) ? 0.0 : M[i * ld + k];
-    __syncthreads();
-    // rows i != k: M[i][j] = (j == k ? 0 : M[i][j]) - col[i] * M[k][j]; a
-    // thread's column over eight rows at a time, all eight read before any is
-    // written (entries are independent; row k is not written here)
-    const int rs = nt >> 6;
-    for (int j = t & 63; j < n; j += 64) {
-      const double mk = M[k * ld + j];
-      for (int i0 = t >> 6; i0 < n; i0 += 8 * rs) {
-        double a[8], c[8];
+    if (s.flag) return false;  // uniform over the workgroup
+    const int p = s.pk, pb = p / TB, pa = p % TB;
+    // publish row p (the new row k) and row k (the new row p)
+    if (rb == pb)
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int i = i0 + u * rs;
-          a[u] = (i < n && j != k) ? M[i * ld + j] : 0.0;
-          c[u] = i < n ? col[i] : 0.0;
-        }
+      for (int b = 0; b < TB; b++) s.rowp[c0 + b] = m[pa][b];
+    if (rb == kb)
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int i = i0 + u * rs;
-          if (i < n && i != k) M[i * ld + j] = a[u] - c[u] * mk;
-        }
-      }
+      for (int b = 0; b < TB; b++) s.rowk[c0 + b] = m[ka][b];
+    __syncthreads();
+    const double inv = 1.0 / s.rowp[k];
+    double sr[TB];  // row k after the swap and scaling: inv at column k
+#pragma unroll
+    for (int b = 0; b < TB; b++) sr[b] = (c0 + b == k) ? inv : s.rowp[c0 + b] * inv;
+    if (p != k && rb == pb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[pa][b] = s.rowk[c0 + b];
+    if (rb == kb)
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[ka][b] = sr[b];
+    // column k after the swap (0 at row k)
+    if (cb == kb)
+#pragma unroll
+      for (int a = 0; a < TB; a++) s.colk[r0 + a] = (r0 + a == k) ? 0.0 : m[a][ka];
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < TB; a++) {
+      const int i = r0 + a;
+      if (i == k) continue;
+      const double c = s.colk[i];
+#pragma unroll
+      for (int b = 0; b < TB; b++) m[a][b] = ((c0 + b == k) ? 0.0 : m[a][b]) - c * sr[b];
     }
-    __syncthreads();
   }
-  // undo the row interchanges as column interchanges, last first
-  for (int k = n - 1; k >= 0; k--) {
-    const int p = piv[k];
-    if (p != k)
-      for (int i = t; i < n; i += nt) {
-        const double a = M[i * ld + k];
-        M[i * ld + k] = M[i * ld + p];
-        M[i * ld + p] = a;
-      }
-    __syncthreads();
+  // undo the row interchanges as column interchanges, last first: the
+  // result's column j is column src[j] of the tiles
+  __syncthreads();
+  if (t == 0) {
+    for (int j = 0; j < n; j++) s.src[j] = j;
+    for (int k = n - 1; k >= 0; k--) {
+      const int q = s.src[k];
+      s.src[k] = s.src[s.piv[k]];
+      s.src[s.piv[k]] = q;
+    }
   }
+  __syncthreads();
+  if (t < n) s.dst[s.src[t]] = t;
+  __syncthreads();
   return true;
 }
 
-// One workgroup (kGMax threads) per group: invert the group's diagonal block
-// of A1 into G1 and form the group's row of A2.  The members' neighbour
-// lists (block, group, A1 value) are staged in LDS first, all their loads in
-// flight together.
-__global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
+// One workgroup per group (TiledGJ<kGMax>::NT threads): invert the group's
+// diagonal block of A1 into G1 and form the group's row of A2.  The
+// members' neighbour lists (block, group, A1 value) are staged in LDS first,
+// all their loads in flight together; the block is assembled in LDS and
+// inverted in registers.
+constexpr int kGroupThreads = TiledGJ<kGMax>::NT;
+__global__ void __launch_bounds__(kGroupThreads) k_coarse_groups(CoarseLayout C) {
   constexpr int LD = kGMax + 1;  // odd row pitch: column reads do not hit one LDS bank
   constexpr int NQ = kGMax * kCQ;
-  constexpr int PER = NQ / kGMax;  // neighbour entries per thread
+  constexpr int PER = NQ / kGroupThreads;  // neighbour entries per thread
+  constexpr int TB = TiledGJ<kGMax>::TB, NTC = TiledGJ<kGMax>::NTC;
   __shared__ double M[kGMax * LD];
-  __shared__ double col[kGMax];
-  __shared__ int piv[kGMax];
-  __shared__ int flag;
+  __shared__ TiledGJ<kGMax> gs;
   __shared__ int mem[kGMax];
   __shared__ int gq[NQ];     // group of each member's neighbour q (-1 none)
   __shared__ int nq[NQ];     // the neighbour block
@@ -189,7 +220,7 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     double av[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-      const int e = t + r * kGMax, i = e / kCQ, q = e % kCQ;
+      const int e = t + r * kGroupThreads, i = e / kCQ, q = e % kCQ;
       nb[r] = i < n ? C.nbr[mem[i] * kCQ + q] : -1;
       av[r] = i < n ? C.A1[mem[i] * kCQ + q] : 0.0;
     }
@@ -198,17 +229,17 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     for (int r = 0; r < PER; r++) gv[r] = nb[r] >= 0 ? C.grp[nb[r]] : -1;
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-      const int e = t + r * kGMax;
+      const int e = t + r * kGroupThreads;
       nq[e] = nb[r];
       gq[e] = gv[r];
       aq[e] = nb[r] >= 0 ? av[r] : 0.0;
     }
   }
-  for (int e = t; e < n * LD; e += blockDim.x) M[e] = 0.0;
+  for (int e = t; e < n * LD; e += kGroupThreads) M[e] = 0.0;
   __syncthreads();
   // row i of the diagonal block: A1 of member i at the columns of member j
   // (a neighbour list holds each block once)
-  for (int e = t; e < n * kCQ; e += blockDim.x) {
+  for (int e = t; e < n * kCQ; e += kGroupThreads) {
     if (gq[e] != g) continue;
     const int i = e / kCQ, bj = nq[e];
     int j = 0;
@@ -216,137 +247,60 @@ __global__ void __launch_bounds__(kGMax) k_coarse_groups(CoarseLayout C) {
     if (j < n) M[i * LD + j] = aq[e];
   }
   __syncthreads();
-  const bool ok = gj_invert(M, n, LD, piv, col, &flag);
-  if (!ok) {
+  const int r0 = (t / NTC) * TB, c0 = (t % NTC) * TB;
+  double m[TB][TB];
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++) m[a][b] = (r0 + a < n && c0 + b < n) ? M[(r0 + a) * LD + c0 + b] : 0.0;
+  if (!gj_invert_tiles<kGMax>(m, n, gs)) {
     if (t == 0) atomicOr(C.status, 1);
     return;
   }
-  // G1 column-major per group (entry (i, j) at j*G + i): level 1 reads it coalesced
-  for (int e = t; e < G * G; e += blockDim.x) {
-    const int j = e / G, i = e % G;
-    C.G1[(int64_t)g * G * G + e] = (i < n && j < n) ? M[i * LD + j] : 0.0;
-  }
+  // G1 column-major per group (entry (i, j) at j*G + i, zero outside n x n):
+  // level 1 reads it coalesced
+  double *g1 = C.G1 + (int64_t)g * G * G;
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++) {
+      const int i = r0 + a, c = c0 + b;
+      if (i >= G || c >= G) continue;
+      if (c < n) g1[gs.dst[c] * G + i] = i < n ? m[a][b] : 0.0;
+      else g1[c * G + i] = 0.0;
+    }
   // A2[g][h] = sum over members and their neighbour entries in group h, in
   // the fixed order of the list
-  for (int h = t; h < C.ng; h += blockDim.x) {
-    double s = 0.0;
-    for (int e = 0; e < n * kCQ; e++) s += (gq[e] == h) ? aq[e] : 0.0;
-    C.A2[(int64_t)g * C.ng + h] = s;
+  for (int h = t; h < C.ng; h += kGroupThreads) {
+    double sum = 0.0;
+    for (int e = 0; e < n * kCQ; e++) sum += (gq[e] == h) ? aq[e] : 0.0;
+    C.A2[(int64_t)g * C.ng + h] = sum;
   }
 }
 
-// Single workgroup of 1024 threads: A2 <- A2^-1 (ng <= kNG) by the
-// Gauss-Jordan steps of gj_invert (same pivots, same operations per entry,
-// so the same result), with the matrix in registers: thread t holds the 4x4
-// tile of rows 4*(t/32).. and columns 4*(t%32)..; each step publishes the
-// pivot row, the row it replaces and the pivot column through LDS.
-__global__ void __launch_bounds__(1024) k_coarse_invert(CoarseLayout C) {
-  constexpr int NT = 1024, TB = 4, NTC = kNG / TB;  // tiles per row of tiles
-  static_assert(NTC * NTC == NT, "one 4x4 tile per thread");
-  __shared__ double rowp[kNG], rowk[kNG], colk[kNG];
-  __shared__ double cv[NTC];
-  __shared__ int ci[NTC];
-  __shared__ int piv[kNG], src[kNG], dst[kNG];
-  __shared__ int flag, pk;
-  __shared__ double dkk;
+// Single workgroup of 1024 threads: A2 <- A2^-1 (ng <= kNG), the matrix in
+// registers (gj_invert_tiles).
+__global__ void __launch_bounds__(TiledGJ<kNG>::NT) k_coarse_invert(CoarseLayout C) {
+  constexpr int TB = TiledGJ<kNG>::TB, NTC = TiledGJ<kNG>::NTC;
+  static_assert(TiledGJ<kNG>::NT == 1024, "one 4x4 tile per thread of 1024");
+  __shared__ TiledGJ<kNG> gs;
   const int n = C.ng, t = threadIdx.x;
-  const int rb = t / NTC, cb = t % NTC, r0 = rb * TB, c0 = cb * TB;
+  const int r0 = (t / NTC) * TB, c0 = (t % NTC) * TB;
   double m[TB][TB];
 #pragma unroll
   for (int a = 0; a < TB; a++)
 #pragma unroll
     for (int b = 0; b < TB; b++)
       m[a][b] = (r0 + a < n && c0 + b < n) ? C.A2[(int64_t)(r0 + a) * n + c0 + b] : 0.0;
-  if (t == 0) flag = 0;
-  for (int k = 0; k < n; k++) {
-    const int kb = k / TB, ka = k % TB;
-    // pivot of column k: the first row i >= k of largest |M[i][k]|
-    if (cb == kb) {
-      double mx = -1.0;
-      int p = n;
-#pragma unroll
-      for (int a = 0; a < TB; a++) {
-        const int i = r0 + a;
-        const double v = fabs(m[a][ka]);
-        if (i >= k && i < n && v > mx) { mx = v; p = i; }
-      }
-      cv[rb] = mx;
-      ci[rb] = p;
-      if (rb == kb) dkk = m[ka][ka];
-    }
-    __syncthreads();
-    if (t < 64) {
-      double mx = t < NTC ? cv[t] : -1.0;
-      int p = t < NTC ? ci[t] : n;
-#pragma unroll
-      for (int off = 16; off > 0; off >>= 1) {
-        const double om = __shfl_xor(mx, off, 64);
-        const int op = __shfl_xor(p, off, 64);
-        if (om > mx || (om == mx && op < p)) { mx = om; p = op; }
-      }
-      if (t == 0) {
-        piv[k] = p;
-        pk = p;
-        if (!(mx > 0.0) || dkk != dkk) flag = 1;
-      }
-    }
-    __syncthreads();
-    if (flag) {
-      if (t == 0) atomicOr(C.status, 1);
-      return;  // uniform over the workgroup
-    }
-    const int p = pk, pb = p / TB, pa = p % TB;
-    // publish row p (the new row k) and row k (the new row p)
-    if (rb == pb)
-#pragma unroll
-      for (int b = 0; b < TB; b++) rowp[c0 + b] = m[pa][b];
-    if (rb == kb)
-#pragma unroll
-      for (int b = 0; b < TB; b++) rowk[c0 + b] = m[ka][b];
-    __syncthreads();
-    const double inv = 1.0 / rowp[k];
-    double sr[TB];  // row k after the swap and scaling: inv at column k
-#pragma unroll
-    for (int b = 0; b < TB; b++) sr[b] = (c0 + b == k) ? inv : rowp[c0 + b] * inv;
-    if (p != k && rb == pb)
-#pragma unroll
-      for (int b = 0; b < TB; b++) m[pa][b] = rowk[c0 + b];
-    if (rb == kb)
-#pragma unroll
-      for (int b = 0; b < TB; b++) m[ka][b] = sr[b];
-    // column k after the swap (0 at row k)
-    if (cb == kb)
-#pragma unroll
-      for (int a = 0; a < TB; a++) colk[r0 + a] = (r0 + a == k) ? 0.0 : m[a][ka];
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < TB; a++) {
-      const int i = r0 + a;
-      if (i == k) continue;
-      const double c = colk[i];
-#pragma unroll
-      for (int b = 0; b < TB; b++) m[a][b] = ((c0 + b == k) ? 0.0 : m[a][b]) - c * sr[b];
-    }
+  if (!gj_invert_tiles<kNG>(m, n, gs)) {
+    if (t == 0) atomicOr(C.status, 1);
+    return;
   }
-  // undo the row interchanges as column interchanges, last first: the
-  // result's column j is column src[j] of M
-  __syncthreads();
-  if (t == 0) {
-    for (int j = 0; j < n; j++) src[j] = j;
-    for (int k = n - 1; k >= 0; k--) {
-      const int q = src[k];
-      src[k] = src[piv[k]];
-      src[piv[k]] = q;
-    }
-  }
-  __syncthreads();
-  if (t < n) dst[src[t]] = t;
-  __syncthreads();
 #pragma unroll
   for (int a = 0; a < TB; a++)
 #pragma unroll
     for (int b = 0; b < TB; b++)
-      if (r0 + a < n && c0 + b < n) C.A2[(int64_t)(r0 + a) * n + dst[c0 + b]] = m[a][b];
+      if (r0 + a < n && c0 + b < n) C.A2[(int64_t)(r0 + a) * n + gs.dst[c0 + b]] = m[a][b];
 }
 
 // Level 1 and 2 of an application for group g (one workgroup of kL1Threads):

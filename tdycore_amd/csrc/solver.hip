@@ -792,7 +792,7 @@ static void gmres_prep(tdyhip_ctx *ctx, const tdyhip_solver_opts &o, const doubl
     if (ws.bj.bs == 64) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<64><<<C.nb, 64, 0, cs>>>(C, ws.NB, W, ws.J));
     else if (ws.bj.bs == 256) KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<256><<<C.nb, 256, 0, cs>>>(C, ws.NB, W, ws.J));
     else KT(ctx, cs, "k_coarse_rows", brows, k_coarse_rows<512><<<C.nb, 512, 0, cs>>>(C, ws.NB, W, ws.J));
-    KT(ctx, cs, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGMax, 0, cs>>>(C));
+    KT(ctx, cs, "k_coarse_groups", (double)C.nb * kCQ * 16, k_coarse_groups<<<C.ng, kGroupThreads, 0, cs>>>(C));
   };
   const bool early = ws.coarse && sizeof(TV) == 8;
   if (early) rows_groups(st);
