@@ -30,6 +30,7 @@ EXPORTS = [
 ]
 
 
+ERR_ARG = 1
 ERR_SINGULAR = 3
 ERR_NOCONV = 4
 ERR_CANCELLED = 6
@@ -112,7 +113,7 @@ def load():
         "tdyhip_fuser_create": [P], "tdyhip_fuser_destroy": [P], "tdyhip_set_fuser": [P, P],
         "tdyhip_fuser_stats": [P, P], "tdyhip_fuser_set_window": [P, C.c_int32],
         # test hook (exported, not declared in the header)
-        "tdyhip_test_dense_inverse": [P, C.c_int32, P],
+        "tdyhip_test_dense_inverse": [P, C.c_int32, C.c_int32, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -303,6 +303,31 @@ __global__ void __launch_bounds__(TiledGJ<kNG>::NT) k_coarse_invert(CoarseLayout
       if (r0 + a < n && c0 + b < n) C.A2[(int64_t)(r0 + a) * n + gs.dst[c0 + b]] = m[a][b];
 }
 
+// Test kernel: A <- A^-1 for one dense n x n row-major matrix (n <= N) by
+// gj_invert_tiles<N> in one workgroup; N = kGMax is the group inverses'
+// width, kNG the level-2 inverse's.  *status set on a zero pivot.
+template <int N>
+__global__ void __launch_bounds__(TiledGJ<N>::NT) k_test_dense_invert(double *A, int n, int *status) {
+  constexpr int TB = TiledGJ<N>::TB, NTC = TiledGJ<N>::NTC;
+  __shared__ TiledGJ<N> gs;
+  const int t = threadIdx.x;
+  const int r0 = (t / NTC) * TB, c0 = (t % NTC) * TB;
+  double m[TB][TB];
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++) m[a][b] = (r0 + a < n && c0 + b < n) ? A[(r0 + a) * n + c0 + b] : 0.0;
+  if (!gj_invert_tiles<N>(m, n, gs)) {
+    if (t == 0) *status = 1;
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < TB; a++)
+#pragma unroll
+    for (int b = 0; b < TB; b++)
+      if (r0 + a < n && c0 + b < n) A[(r0 + a) * n + gs.dst[c0 + b]] = m[a][b];
+}
+
 // Level 1 and 2 of an application for group g (one workgroup of kL1Threads):
 //   z1[b] = (G1 rc1)[b] + (A2^-1 r2)[g]  for the blocks b of g,
 // with r2[h] the sum of rc1 over the blocks of group h.  Every workgroup

@@ -1506,34 +1506,36 @@ int tdyhip_run_tdyti(tdyhip_ctx *ctx, double *U, double dt_init, double growth, 
 }
 
 // Test hook, not part of the interface in include/tdyhip.h: Ainv = A^-1 for
-// one dense n x n row-major matrix (1 <= n <= 128) by the coarse
-// correction's level-2 inverse kernel, on the current device.  A zero pivot
-// returns TDYHIP_ERR_SINGULAR.
-int tdyhip_test_dense_inverse(const double *A, int32_t n, double *Ainv) {
+// one dense n x n row-major matrix by the coarse correction's register-tiled
+// Gauss-Jordan at width 64 (the group inverses, n <= 64) or 128 (the level-2
+// inverse, n <= 128), on the current device.  A zero pivot returns
+// TDYHIP_ERR_SINGULAR.
+int tdyhip_test_dense_inverse(const double *A, int32_t n, int32_t width, double *Ainv) {
   using namespace tdyhip;
-  if (!A || !Ainv || n < 1 || n > kNG) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
+  if (!A || !Ainv || (width != kGMax && width != kNG) || n < 1 || n > width) { set_error("bad argument"); return TDYHIP_ERR_ARG; }
   const size_t bytes = sizeof(double) * (size_t)n * n;
-  CoarseLayout C{};
-  C.ng = n;
+  double *dA = nullptr;
+  int *dst = nullptr;
   int rc = 0, status = 0;
-  if (hipMalloc((void **)&C.A2, bytes) != hipSuccess) { set_error("hipMalloc failed"); return TDYHIP_ERR_MEM; }
-  if (hipMalloc((void **)&C.status, sizeof(int)) != hipSuccess) {
-    (void)hipFree(C.A2);
+  if (hipMalloc((void **)&dA, bytes) != hipSuccess) { set_error("hipMalloc failed"); return TDYHIP_ERR_MEM; }
+  if (hipMalloc((void **)&dst, sizeof(int)) != hipSuccess) {
+    (void)hipFree(dA);
     set_error("hipMalloc failed");
     return TDYHIP_ERR_MEM;
   }
-  if (hipMemcpy(C.A2, A, bytes, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(C.status, &status, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(dA, A, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dst, &status, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     rc = TDYHIP_ERR_HIP;
   } else {
-    k_coarse_invert<<<1, 1024>>>(C);
+    if (width == kGMax) k_test_dense_invert<kGMax><<<1, TiledGJ<kGMax>::NT>>>(dA, n, dst);
+    else k_test_dense_invert<kNG><<<1, TiledGJ<kNG>::NT>>>(dA, n, dst);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
-        hipMemcpy(&status, C.status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(Ainv, C.A2, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(&status, dst, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(Ainv, dA, bytes, hipMemcpyDeviceToHost) != hipSuccess)
       rc = TDYHIP_ERR_HIP;
   }
-  (void)hipFree(C.A2);
-  (void)hipFree(C.status);
+  (void)hipFree(dA);
+  (void)hipFree(dst);
   if (rc) { set_error("dense inverse failed"); return rc; }
   if (status) { set_error("singular matrix (zero pivot)"); return TDYHIP_ERR_SINGULAR; }
   return 0;
